@@ -1,0 +1,156 @@
+/*
+ * tqec_hip.h — C ABI of the MI355X tensor-network decoding library.
+ *
+ * The library executes a flattened contraction plan of a TNMMAP factor graph
+ * over a batch of syndromes on one GPU (gfx950) and returns, per syndrome,
+ * the unnormalised joint probability P(syndrome, logical class) for every
+ * logical class, the arg-max class and a tie flag.
+ *
+ * It replaces, for the TNMMAP decoding path of TensorQEC.jl:
+ *   - the per-syndrome evidence update  (src/decoding/tndecoder.jl:148-158,
+ *     240-246  `update_syndrome!`),
+ *   - the contraction `ct.code(ct.tensors...)` evaluated by OMEinsum
+ *     (src/decoding/tndecoder.jl:162, 250),
+ *   - `findmax(mar)` (src/decoding/tndecoder.jl:163, 251).
+ * The host keeps building the factor graph and the contraction tree
+ * (src/decoding/tndecoder.jl:97-146, 186-219) and hands the tree over as a
+ * post-order list of pairwise steps.
+ *
+ * Conventions
+ *   - Every variable (label) has size 2.  A tensor of rank r is 2^r doubles;
+ *     bit j of the flat index is the value of the tensor's j-th label, which
+ *     is the column-major layout of a Julia Array{Float64} with all sizes 2.
+ *   - Node ids: leaves are 0..n_leaves-1, step i produces node n_leaves+i.
+ *   - All functions return TQEC_OK (0) or a negative TQEC_E* code and never
+ *     abort; tqec_last_error() returns a thread-local message.
+ *   - A plan is immutable after creation.  Per-call device workspace is kept
+ *     per (plan, stream); calls on different streams may run concurrently,
+ *     calls on the same stream are ordered by the stream.
+ *   - tqec_decode_batch takes host pointers and synchronises the stream
+ *     before returning; tqec_decode_batch_device takes device pointers and
+ *     only enqueues work on the stream.
+ */
+#ifndef TQEC_HIP_H
+#define TQEC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TQEC_OK 0
+#define TQEC_EINVAL (-1)      /* malformed descriptor or argument       */
+#define TQEC_ENOMEM (-2)      /* host or device allocation failed        */
+#define TQEC_EDEVICE (-3)     /* HIP runtime error                       */
+#define TQEC_EUNSUPPORTED (-4)/* plan exceeds a limit of the chosen mode */
+#define TQEC_ENULL (-5)       /* null handle or pointer                  */
+
+/* Execution modes of tqec_decode_batch*. */
+#define TQEC_MODE_PLAIN_F64 0 /* one batched kernel per plan step, FP64,
+                                 intermediates in HBM                     */
+#define TQEC_MODE_FUSED_F64 1 /* whole plan per workgroup tile, FP64,
+                                 intermediates in LDS                     */
+#define TQEC_MODE_FUSED_F32 2 /* fused, FP32 with per-tensor power-of-two
+                                 scaling; syndromes whose top-2 class
+                                 margin is below the re-check threshold are
+                                 re-contracted in FP64                    */
+
+/*
+ * Flattened contraction plan.  Mirrors the NestedEinsum that
+ * `optimize_code(DynamicEinCode(ixs, iy), size_dict, TreeSA())` returns in
+ * src/decoding/tndecoder.jl:140-144 and :213-217.
+ */
+typedef struct tqec_plan_desc {
+    int32_t n_vars;                 /* labels are 0..n_vars-1               */
+    int32_t n_leaves;
+    const int32_t *leaf_rank;       /* [n_leaves]                           */
+    const int32_t *leaf_labels;     /* concatenated, sum(leaf_rank) entries */
+    const double *leaf_data;        /* concatenated 2^rank values per leaf;
+                                       entries of evidence leaves ignored  */
+    const int32_t *leaf_evidence;   /* [n_leaves]: syndrome bit whose value
+                                       selects the one-hot vector of an
+                                       evidence leaf (rank 1), or -1       */
+    int32_t n_steps;                /* must be n_leaves - 1                 */
+    const int32_t *step_lhs;        /* [n_steps] node ids                   */
+    const int32_t *step_rhs;        /* [n_steps] node ids                   */
+    const int32_t *step_out_rank;   /* [n_steps]                            */
+    const int32_t *step_out_labels; /* concatenated                         */
+    int32_t n_out;                  /* rank of the result                   */
+    const int32_t *out_labels;      /* result label order (the einsum iy)   */
+    int32_t n_syndrome_bits;        /* bytes per syndrome row               */
+    int32_t n_sliced;               /* labels fixed by a syndrome bit       */
+    const int32_t *sliced_label;    /* [n_sliced] label id                  */
+    const int32_t *sliced_bit;      /* [n_sliced] syndrome bit              */
+} tqec_plan_desc;
+
+typedef struct tqec_plan tqec_plan;
+
+/* Mirrors `contraction_complexity(ct)` (src/decoding/tndecoder.jl:273-279)
+ * and adds what the executor derived from the plan. */
+typedef struct tqec_stats {
+    double tc;              /* multiply-adds over all steps               */
+    double rw;              /* elements read + written over all steps     */
+    int32_t sc;             /* log2 of the widest tensor                  */
+    int32_t n_steps;
+    int32_t n_dep_steps;    /* steps executed per syndrome                */
+    double dep_tc;          /* multiply-adds per syndrome                 */
+    double dep_rw;          /* per-syndrome elements read+written         */
+    int64_t plain_ws_entries;  /* per-syndrome HBM workspace, in entries  */
+    int64_t fused_lds_entries; /* per-syndrome LDS footprint, in entries  */
+    int32_t n_out;          /* number of logical classes = 2^n_out        */
+    int32_t fused_tile_f64; /* syndromes per workgroup, FUSED_F64         */
+    int32_t fused_tile_f32; /* syndromes per workgroup, FUSED_F32         */
+} tqec_stats;
+
+/* Build a plan.  The descriptor's arrays are copied; the caller keeps them. */
+int tqec_plan_create(const tqec_plan_desc *desc, tqec_plan **out);
+
+void tqec_plan_destroy(tqec_plan *plan);
+
+int tqec_plan_stats(const tqec_plan *plan, tqec_stats *out);
+
+/*
+ * Decode n syndromes.
+ *   syndromes  n x n_syndrome_bits bytes, row-major, each 0 or 1
+ *   marginals  n x 2^n_out doubles: row b is P(syndrome_b, class) for every
+ *              class, classes in column-major order of out_labels (NULL to
+ *              skip)
+ *   classes    n: index of the first maximum (Julia `findmax`) (NULL to skip)
+ *   tie_flags  n: 1 when the top-2 relative gap is <= tie_tol (NULL to skip)
+ *   stream     hipStream_t or NULL for the null stream
+ */
+int tqec_decode_batch(tqec_plan *plan, const uint8_t *syndromes, int64_t n, int32_t mode,
+                      double *marginals, int32_t *classes, uint8_t *tie_flags, void *stream);
+
+/* Same with device pointers; enqueues on `stream` and returns. */
+int tqec_decode_batch_device(tqec_plan *plan, const uint8_t *d_syndromes, int64_t n, int32_t mode,
+                             double *d_marginals, int32_t *d_classes, uint8_t *d_tie_flags,
+                             void *stream);
+
+/* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
+int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);
+
+/* Syndromes re-contracted in FP64 by the last FUSED_F32 call on `stream`
+ * (host-visible after the call has completed). */
+int tqec_last_recheck_count(tqec_plan *plan, void *stream, int64_t *count);
+
+/* Kernel timing with HIP events recorded on the call's stream (off by
+ * default).  While enabled, every decode call records events around its
+ * main contraction (the fused kernel, or the whole step sequence of the
+ * plain executor) and around the FP64 re-check kernel.
+ * tqec_kernel_time_totals waits for the recorded events of all calls on
+ * `stream` since the previous query and returns the summed milliseconds and
+ * the number of calls, then clears them. */
+int tqec_plan_set_timing(tqec_plan *plan, int enable);
+int tqec_kernel_time_totals(tqec_plan *plan, void *stream, double *main_ms, double *recheck_ms,
+                            int64_t *calls);
+
+/* Thread-local description of the last error. */
+const char *tqec_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TQEC_HIP_H */

@@ -1,0 +1,46 @@
+// plan_compile.hpp — host-side compilation of a tqec_plan_desc into the
+// per-syndrome step program executed by the kernels.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+#include "tqec_hip.h"
+
+namespace tqec {
+
+struct CompiledPlan {
+    // ---- descriptor copy -------------------------------------------------
+    int n_vars = 0, n_leaves = 0, n_steps = 0, n_out = 0, n_syn = 0;
+    std::vector<std::vector<int>> raw_labels;   // per leaf, as given
+    std::vector<std::vector<double>> raw_data;  // per leaf
+    std::vector<int> leaf_ev;                   // one-hot evidence bit or -1
+    std::vector<int> lhs, rhs;
+    std::vector<int> out_labels;
+    std::vector<int> slice_bit_of_label;        // -1 when not sliced
+
+    // ---- derived ---------------------------------------------------------
+    std::vector<std::vector<int>> labels;       // effective labels per node
+    std::vector<char> dep;                      // syndrome-dependent node
+    std::vector<std::vector<double>> value;     // constant nodes (labels order)
+    std::vector<std::vector<int>> mem;          // memory order of operands
+
+    std::vector<StepDesc> steps;                // dependent steps, execution order
+    std::vector<double> const_pool;             // kConst data (mem order, scaled)
+    std::vector<double> sliced_pool;            // kSliced variants
+    int64_t ws_entries = 0;                     // per-syndrome workspace (entries)
+    int n_slots = 0;                            // scale slots
+    int64_t root_off = 0;
+    int root_slot = 0;
+
+    // ---- statistics ------------------------------------------------------
+    double tc = 0, rw = 0, dep_tc = 0, dep_rw = 0;
+    int sc = 0;
+};
+
+// Returns TQEC_OK or an error code; `err` receives a message on failure.
+int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err);
+
+}  // namespace tqec

@@ -1,0 +1,858 @@
+// tqec_hip.hip — HIP kernels and C ABI of the TNMMAP contraction executor
+// for MI355X (gfx950).
+//
+// Two executors share one compiled plan (plan_compile.cpp):
+//
+//  * plain  — one launch per syndrome-dependent plan step, FP64, every
+//             intermediate in HBM laid out [entry][syndrome] so a wavefront
+//             reads 64 consecutive syndromes of one entry (fully coalesced)
+//             and all index arithmetic is wave-uniform.
+//  * fused  — one persistent kernel runs the whole per-syndrome step program
+//             for a tile of G syndromes per workgroup with every
+//             intermediate resident in LDS ([entry][g], bank-conflict free),
+//             one barrier per step.  FP32 carries a per-tensor, per-syndrome
+//             power-of-two exponent (max tracked with 64-bit tagged LDS
+//             atomics) so nothing underflows; syndromes whose top-2 class
+//             margin is below the re-check threshold are queued and
+//             re-contracted by the FP64 fused kernel.
+//
+// Reference behaviour replaced: src/decoding/tndecoder.jl:148-165 and
+// :240-253 (evidence update, OMEinsum contraction, findmax).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+#include "plan_compile.hpp"
+#include "tqec_hip.h"
+
+using namespace tqec;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define TQ_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_error(TQEC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Pools {
+    const T *cpool;
+    const T *spool;
+};
+
+// Plain executor operand fetch: syndrome b of a chunk, workspace stride ldw.
+template <typename T>
+__device__ __forceinline__ T plain_load(const OperandDesc &op, uint32_t idx, int64_t b, const T *__restrict__ ws,
+                                        int64_t ldw, Pools<T> pools, const uint8_t *__restrict__ synT) {
+    switch (op.kind) {
+        case kInter:
+            return ws[(op.off + (int64_t)idx) * ldw + b];
+        case kConst:
+            return pools.cpool[op.off + idx];
+        case kSliced: {
+            uint32_t v = 0;
+            for (int j = 0; j < op.nev; ++j) v |= (uint32_t)synT[(int64_t)op.ev_bit[j] * ldw + b] << j;
+            return pools.spool[op.off + ((int64_t)v << op.rank) + idx];
+        }
+        default:
+            return (T)((idx & 1u) == (uint32_t)synT[(int64_t)op.ev_bit[0] * ldw + b]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T plain_load_sum(const OperandDesc &op, uint32_t base, int nk, int64_t b,
+                                            const T *__restrict__ ws, int64_t ldw, Pools<T> pools,
+                                            const uint8_t *__restrict__ synT) {
+    if (op.nsingle == 0) return plain_load(op, base, b, ws, ldw, pools, synT);
+    T s = 0;
+    const uint32_t ns = 1u << op.nsingle;
+    for (uint32_t q = 0; q < ns; ++q) s += plain_load(op, base | (q << nk), b, ws, ldw, pools, synT);
+    return s;
+}
+
+// Transpose a chunk of syndromes [n][nbits] (row-major bytes) to [bit][ldw].
+__global__ void transpose_syndromes_kernel(const uint8_t *__restrict__ syn, int64_t n, int nbits,
+                                           uint8_t *__restrict__ synT, int64_t ldw) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const uint8_t *row = syn + b * nbits;
+    for (int i = 0; i < nbits; ++i) synT[(int64_t)i * ldw + b] = row[i] & 1u;
+}
+
+// One plan step for a chunk: thread = (output entry c, syndrome b).
+template <typename T>
+__global__ void __launch_bounds__(256) plain_step_kernel(StepDesc st, T *__restrict__ ws, int64_t ldw, int64_t nb,
+                                                         Pools<T> pools, const uint8_t *__restrict__ synT) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t nc = 1u << st.rc;
+    const uint32_t nkk = 1u << st.nk;
+    const int cexp = st.a.cexp + st.b.cexp;
+    for (uint32_t c = blockIdx.y; c < nc; c += gridDim.y) {
+        const uint32_t ar = gather_bits(c, st.a);
+        const uint32_t br = gather_bits(c, st.b);
+        T acc = 0;
+        for (uint32_t k = 0; k < nkk; ++k) {
+            const T va = plain_load_sum(st.a, ar | k, st.nk, b, ws, ldw, pools, synT);
+            const T vb = plain_load_sum(st.b, br | k, st.nk, b, ws, ldw, pools, synT);
+            acc += va * vb;
+        }
+        if (cexp != 0) acc = ldexp(acc, cexp);
+        ws[(st.c_off + (int64_t)c) * ldw + b] = acc;
+    }
+}
+
+// First maximum (Julia findmax order) and top-2 gap of one syndrome.
+__global__ void plain_finalize_kernel(const double *__restrict__ ws, int64_t ldw, int64_t nb, int64_t root_off,
+                                      int nout, double tie_tol, int64_t out_base, double *__restrict__ marginals,
+                                      int32_t *__restrict__ classes, uint8_t *__restrict__ ties) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double best = -1, second = -1;
+    int bi = 0;
+    for (int j = 0; j < nout; ++j) {
+        const double x = ws[(root_off + j) * ldw + b];
+        if (marginals) marginals[(out_base + b) * nout + j] = x;
+        if (j == 0 || x > best) {
+            if (j > 0) second = best;
+            best = x;
+            bi = j;
+        } else if (x > second) {
+            second = x;
+        }
+    }
+    if (classes) classes[out_base + b] = bi;
+    if (ties) ties[out_base + b] = (nout > 1 && second >= 0 && (best - second) <= tie_tol * best) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fused executor
+// ---------------------------------------------------------------------------
+
+struct FusedArgs {
+    const StepDesc *steps;
+    int nsteps;
+    const void *cpool;
+    const void *spool;
+    const uint8_t *syn;       // [n][nbits]
+    int64_t n;
+    int nbits;
+    int glog;                 // log2 G
+    int nout;
+    int64_t ws_entries;
+    int nslots;
+    int64_t root_off;
+    int root_slot;
+    const int32_t *list;      // optional: syndrome ids to process
+    const int32_t *list_count;
+    double *marginals;
+    int32_t *classes;
+    uint8_t *ties;
+    double tie_tol;
+    double recheck_tol;
+    int32_t *recheck_list;    // FP32 pass: low-margin syndromes are appended here
+    int32_t *recheck_count;
+};
+
+template <typename T>
+__device__ __forceinline__ T fused_load(const OperandDesc &op, uint32_t idx, int g, int glog, const T *work,
+                                        const T *__restrict__ cpool, const T *__restrict__ spool,
+                                        const uint8_t *synl) {
+    switch (op.kind) {
+        case kInter:
+            return work[((op.off + idx) << glog) + g];
+        case kConst:
+            return cpool[op.off + idx];
+        case kSliced: {
+            uint32_t v = 0;
+            for (int j = 0; j < op.nev; ++j) v |= (uint32_t)synl[(op.ev_bit[j] << glog) + g] << j;
+            return spool[op.off + ((int64_t)v << op.rank) + idx];
+        }
+        default:
+            return (T)((idx & 1u) == (uint32_t)synl[(op.ev_bit[0] << glog) + g]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T fused_load_sum(const OperandDesc &op, uint32_t base, int nk, int g, int glog,
+                                            const T *work, const T *__restrict__ cpool,
+                                            const T *__restrict__ spool, const uint8_t *synl) {
+    if (op.nsingle == 0) return fused_load(op, base, g, glog, work, cpool, spool, synl);
+    T s = 0;
+    const uint32_t ns = 1u << op.nsingle;
+    for (uint32_t q = 0; q < ns; ++q) s += fused_load(op, base | (q << nk), g, glog, work, cpool, spool, synl);
+    return s;
+}
+
+__device__ __forceinline__ uint32_t max_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ uint32_t max_bits(double v) { return __float_as_uint((float)v); }
+
+// LDS layout: work[ws_entries][G] T | mx[nslots][G] u64 | ex[nslots][G] i32 | syn[nbits][G] u8
+template <typename T, bool SCALE>
+__global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int G = 1 << args.glog;
+    const int glog = args.glog;
+    T *work = reinterpret_cast<T *>(smem);
+    size_t off = (size_t)args.ws_entries * G * sizeof(T);
+    off = (off + 15) & ~size_t(15);
+    unsigned long long *mx = reinterpret_cast<unsigned long long *>(smem + off);
+    int32_t *ex = reinterpret_cast<int32_t *>(smem + off + (SCALE ? (size_t)args.nslots * G * 8 : 0));
+    uint8_t *synl = smem + off + (SCALE ? (size_t)args.nslots * G * 12 : 0);
+    int32_t *lowf = reinterpret_cast<int32_t *>(synl + (((size_t)args.nbits * G + 3) & ~size_t(3)));
+    const T *cpool = reinterpret_cast<const T *>(args.cpool);
+    const T *spool = reinterpret_cast<const T *>(args.spool);
+
+    const int tid = threadIdx.x;
+    const int g = tid & (G - 1);
+    const int lane = tid & 63;
+    const int64_t count = args.list ? (int64_t)(*args.list_count) : args.n;
+    const int64_t ntiles = (count + G - 1) >> glog;
+
+    if (SCALE) {
+        for (int i = tid; i < args.nslots * G; i += blockDim.x) mx[i] = 0ull;
+    }
+    // An operand whose largest entry is below 2^-100 means its step lost
+    // more than 100 binary orders of magnitude: such syndromes are re-checked.
+    constexpr int kLowExp = -100;
+    uint32_t iter = 0;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter) {
+        const int64_t b0 = t << glog;
+        // ---- syndrome bits of the tile -> LDS [bit][g] ----
+        if (!args.list) {
+            const int64_t rows = min((int64_t)G, count - b0);
+            const uint8_t *src = args.syn + b0 * args.nbits;
+            const int tot = G * args.nbits;
+            for (int i = tid; i < tot; i += blockDim.x) {
+                const int r = i / args.nbits, bit = i - r * args.nbits;
+                synl[(bit << glog) + r] = (r < rows) ? (src[i] & 1u) : 0u;
+            }
+        } else {
+            const int tot = G * args.nbits;
+            for (int i = tid; i < tot; i += blockDim.x) {
+                const int r = i / args.nbits, bit = i - r * args.nbits;
+                const int64_t q = b0 + r;
+                synl[(bit << glog) + r] =
+                    (q < count) ? (args.syn[(int64_t)args.list[q] * args.nbits + bit] & 1u) : 0u;
+            }
+        }
+        if (SCALE && tid < G) lowf[tid] = 0;
+        __syncthreads();
+        const uint32_t tag_base = iter * (uint32_t)args.nsteps + 1u;
+        for (int s = 0; s < args.nsteps; ++s) {
+            const StepDesc &st = args.steps[s];
+            int ea = 0, eb = 0, shift = 0;
+            if (SCALE) {
+                int na = 0, nb = 0;
+                if (st.a.kind == kInter) {
+                    const float m = __uint_as_float((uint32_t)(mx[(st.a.slot << glog) + g] & 0xffffffffull));
+                    na = (m > 0.f) ? ilogbf(m) : 0;
+                    ea = ex[(st.a.slot << glog) + g] + na;
+                } else {
+                    ea = st.a.cexp;
+                }
+                if (st.b.kind == kInter) {
+                    const float m = __uint_as_float((uint32_t)(mx[(st.b.slot << glog) + g] & 0xffffffffull));
+                    nb = (m > 0.f) ? ilogbf(m) : 0;
+                    eb = ex[(st.b.slot << glog) + g] + nb;
+                } else {
+                    eb = st.b.cexp;
+                }
+                shift = -(na + nb);
+                if ((st.a.kind == kInter && na < kLowExp) || (st.b.kind == kInter && nb < kLowExp)) lowf[g] = 1;
+            }
+            const uint32_t nkk = 1u << st.nk;
+            const int total = (1 << st.rc) << glog;
+            T lmax = 0;
+            for (int w = tid; w < total; w += blockDim.x) {
+                const uint32_t c = (uint32_t)(w >> glog);
+                const uint32_t ar = gather_bits(c, st.a);
+                const uint32_t br = gather_bits(c, st.b);
+                T acc = 0;
+                for (uint32_t k = 0; k < nkk; ++k) {
+                    const T va = fused_load_sum<T>(st.a, ar | k, st.nk, g, glog, work, cpool, spool, synl);
+                    const T vb = fused_load_sum<T>(st.b, br | k, st.nk, g, glog, work, cpool, spool, synl);
+                    acc += va * vb;
+                }
+                if (SCALE) {
+                    acc = ldexp(acc, shift);
+                    lmax = acc > lmax ? acc : lmax;
+                } else {
+                    const int ce = st.a.cexp + st.b.cexp;
+                    if (ce != 0) acc = ldexp(acc, ce);
+                }
+                work[((st.c_off + c) << glog) + g] = acc;
+            }
+            if (SCALE) {
+                for (int o = G; o < 64; o <<= 1) {
+                    const float other = __shfl_xor((float)lmax, o);
+                    lmax = other > (float)lmax ? (T)other : lmax;
+                }
+                if (lane < G) {
+                    const unsigned long long tagged =
+                        ((unsigned long long)(tag_base + (uint32_t)s) << 32) | (unsigned long long)max_bits(lmax);
+                    atomicMax(&mx[(st.c_slot << glog) + g], tagged);
+                }
+                if (tid < G) ex[(st.c_slot << glog) + g] = ea + eb;
+            }
+            __syncthreads();
+        }
+        // ---- per-syndrome result ----
+        if (tid < G) {
+            const int64_t q = b0 + tid;
+            if (q < count) {
+                const int64_t b = args.list ? (int64_t)args.list[q] : q;
+                int e = 0;
+                if (SCALE) {
+                    const float m = __uint_as_float((uint32_t)(mx[(args.root_slot << glog) + g] & 0xffffffffull));
+                    e = ex[(args.root_slot << glog) + g];
+                    (void)m;
+                }
+                T best = 0, second = -1;
+                int bi = 0;
+                for (int j = 0; j < args.nout; ++j) {
+                    const T x = work[((args.root_off + j) << glog) + g];
+                    if (args.marginals) args.marginals[b * args.nout + j] = ldexp((double)x, e);
+                    if (j == 0 || x > best) {
+                        if (j > 0) second = best;
+                        best = x;
+                        bi = j;
+                    } else if (x > second) {
+                        second = x;
+                    }
+                }
+                if (args.classes) args.classes[b] = bi;
+                const bool low = args.nout > 1 && second >= 0 && (double)(best - second) <= args.tie_tol * (double)best;
+                if (args.ties) args.ties[b] = low ? 1 : 0;
+                if (SCALE && args.recheck_list) {
+                    const bool rc = !(best > 0) || lowf[tid] != 0 ||
+                                    (args.nout > 1 && second >= 0 &&
+                                     (double)(best - second) <= args.recheck_tol * (double)best);
+                    if (rc) {
+                        const int32_t slot = atomicAdd(args.recheck_count, 1);
+                        args.recheck_list[slot] = (int32_t)b;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void zero_i32_kernel(int32_t *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct DeviceData {
+    StepDesc *steps = nullptr;
+    double *cpool64 = nullptr, *spool64 = nullptr;
+    float *cpool32 = nullptr, *spool32 = nullptr;
+};
+
+struct Workspace {
+    int device = -1;
+    void *ws = nullptr;        // plain workspace
+    size_t ws_bytes = 0;
+    uint8_t *synT = nullptr;
+    size_t synT_bytes = 0;
+    int32_t *list = nullptr;   // recheck list + count
+    int64_t list_cap = 0;
+    int32_t *count = nullptr;
+    int32_t *host_count = nullptr;
+    uint8_t *d_syn = nullptr; size_t d_syn_bytes = 0;   // staging for the host API
+    double *d_mar = nullptr; size_t d_mar_bytes = 0;
+    int32_t *d_cls = nullptr; size_t d_cls_bytes = 0;
+    uint8_t *d_tie = nullptr; size_t d_tie_bytes = 0;
+    int64_t last_recheck = -1;
+    // Per-call HIP events (main start/stop, re-check start/stop), accumulated
+    // while timing is enabled and consumed by tqec_kernel_time_totals.
+    std::vector<std::array<hipEvent_t, 4>> ev;
+    std::vector<std::array<char, 4>> ev_used;
+    size_t n_ev = 0;
+};
+
+struct FusedCfg {
+    int glog = -1;      // -1: unsupported
+    int threads = 256;
+    size_t lds = 0;
+    int blocks = 0;
+};
+
+}  // namespace
+
+struct tqec_plan {
+    CompiledPlan cp;
+    double tie_tol = 1e-12;
+    double recheck_tol = 1e-3;
+    int64_t plain_chunk_cap_bytes = int64_t(8) << 30;
+    std::mutex mu;
+    std::map<int, DeviceData> dev;
+    std::map<std::pair<int, hipStream_t>, Workspace> ws;
+    FusedCfg f64, f32;
+    bool cfg_done = false;
+    bool timing = false;
+};
+
+namespace {
+
+template <typename T>
+int upload(const std::vector<double> &h, T **d) {
+    std::vector<T> tmp(h.size() ? h.size() : 1, T(0));
+    for (size_t i = 0; i < h.size(); ++i) tmp[i] = (T)h[i];
+    TQ_HIP(hipMalloc((void **)d, tmp.size() * sizeof(T)));
+    TQ_HIP(hipMemcpy(*d, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return TQEC_OK;
+}
+
+int ensure_device(tqec_plan *p, int device, DeviceData **out) {
+    auto it = p->dev.find(device);
+    if (it != p->dev.end()) {
+        *out = &it->second;
+        return TQEC_OK;
+    }
+    DeviceData dd;
+    const auto &cp = p->cp;
+    TQ_HIP(hipMalloc((void **)&dd.steps, std::max<size_t>(1, cp.steps.size()) * sizeof(StepDesc)));
+    TQ_HIP(hipMemcpy(dd.steps, cp.steps.data(), cp.steps.size() * sizeof(StepDesc), hipMemcpyHostToDevice));
+    int rc;
+    if ((rc = upload<double>(cp.const_pool, &dd.cpool64))) return rc;
+    if ((rc = upload<double>(cp.sliced_pool, &dd.spool64))) return rc;
+    if ((rc = upload<float>(cp.const_pool, &dd.cpool32))) return rc;
+    if ((rc = upload<float>(cp.sliced_pool, &dd.spool32))) return rc;
+    p->dev[device] = dd;
+    *out = &p->dev[device];
+    return TQEC_OK;
+}
+
+size_t fused_lds_bytes(const CompiledPlan &cp, int glog, size_t elem, bool scale) {
+    const size_t G = size_t(1) << glog;
+    size_t b = (size_t)cp.ws_entries * G * elem;
+    b = (b + 15) & ~size_t(15);
+    if (scale) b += (size_t)cp.n_slots * G * 12;
+    b += ((size_t)cp.n_syn * G + 3) & ~size_t(3);
+    if (scale) b += G * sizeof(int32_t);
+    return b;
+}
+
+FusedCfg choose_fused(const CompiledPlan &cp, size_t elem, bool scale, size_t lds_limit, int n_cu) {
+    FusedCfg c;
+    // Largest tile (up to 64 syndromes) whose LDS image leaves room for two
+    // workgroups per CU; fall back to one workgroup per CU.
+    for (int pass = 0; pass < 2 && c.glog < 0; ++pass) {
+        const size_t budget = pass == 0 ? lds_limit / 2 : lds_limit;
+        for (int gl = 6; gl >= 0; --gl) {
+            size_t b = fused_lds_bytes(cp, gl, elem, scale);
+            if (b <= budget) {
+                c.glog = gl;
+                c.lds = b;
+                break;
+            }
+        }
+    }
+    if (c.glog >= 0) {
+        c.threads = 256;
+        const int per_cu = std::max<int>(1, (int)(lds_limit / std::max<size_t>(c.lds, 1)));
+        c.blocks = n_cu * std::min(per_cu, 8);
+    }
+    return c;
+}
+
+int configure(tqec_plan *p) {
+    if (p->cfg_done) return TQEC_OK;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    TQ_HIP(hipGetDeviceProperties(&prop, dev));
+    size_t lds = prop.sharedMemPerBlock;
+    if (prop.maxSharedMemoryPerMultiProcessor > 0) lds = std::min(lds, (size_t)prop.maxSharedMemoryPerMultiProcessor);
+    const int ncu = prop.multiProcessorCount;
+    p->f64 = choose_fused(p->cp, sizeof(double), false, lds, ncu);
+    p->f32 = choose_fused(p->cp, sizeof(float), true, lds, ncu);
+    if (p->f64.glog >= 0) {
+        TQ_HIP(hipFuncSetAttribute((const void *)fused_kernel<double, false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->f64.lds));
+    }
+    if (p->f32.glog >= 0) {
+        TQ_HIP(hipFuncSetAttribute((const void *)fused_kernel<float, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->f32.lds));
+    }
+    p->cfg_done = true;
+    return TQEC_OK;
+}
+
+template <typename T>
+int grow(T **ptr, size_t *cap, size_t need) {
+    if (*cap >= need) return TQEC_OK;
+    if (*ptr) TQ_HIP(hipFree(*ptr));
+    *ptr = nullptr;
+    *cap = 0;
+    TQ_HIP(hipMalloc((void **)ptr, need));
+    *cap = need;
+    return TQEC_OK;
+}
+
+// Record event i (0/1: main start/stop, 2/3: re-check start/stop) of the
+// current call; event 0 opens a new call slot.
+int record(tqec_plan *p, Workspace &w, int i, hipStream_t stream) {
+    if (!p->timing) return TQEC_OK;
+    if (i == 0) {
+        if (w.n_ev == w.ev.size()) {
+            std::array<hipEvent_t, 4> e{};
+            for (auto &x : e) TQ_HIP(hipEventCreate(&x));
+            w.ev.push_back(e);
+            w.ev_used.push_back({0, 0, 0, 0});
+        }
+        w.ev_used[w.n_ev] = {0, 0, 0, 0};
+        ++w.n_ev;
+    }
+    if (w.n_ev == 0) return TQEC_OK;
+    TQ_HIP(hipEventRecord(w.ev[w.n_ev - 1][i], stream));
+    w.ev_used[w.n_ev - 1][i] = 1;
+    return TQEC_OK;
+}
+
+int run_plain(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+    const CompiledPlan &cp = p->cp;
+    const int64_t per = std::max<int64_t>(1, cp.ws_entries) * (int64_t)sizeof(double);
+    int64_t chunk = std::max<int64_t>(256, p->plain_chunk_cap_bytes / per);
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(n, 1));
+    chunk = (chunk + 255) / 256 * 256;
+    int rc;
+    if ((rc = grow((char **)&w.ws, &w.ws_bytes, (size_t)(per * chunk)))) return rc;
+    if ((rc = grow(&w.synT, &w.synT_bytes, (size_t)std::max(1, cp.n_syn) * chunk))) return rc;
+    Pools<double> pools{dd->cpool64, dd->spool64};
+    const int nout = 1 << cp.n_out;
+    if ((rc = record(p, w, 0, stream))) return rc;
+    for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+        const int64_t nb = std::min(chunk, n - s0);
+        const dim3 blk(256);
+        const dim3 grd((unsigned)((nb + 255) / 256));
+        hipLaunchKernelGGL(transpose_syndromes_kernel, grd, blk, 0, stream, d_syn + s0 * cp.n_syn, nb, cp.n_syn,
+                           w.synT, chunk);
+        for (const StepDesc &st : cp.steps) {
+            const unsigned ny = (unsigned)std::min<int64_t>(int64_t(1) << st.rc, 65535);
+            hipLaunchKernelGGL(plain_step_kernel<double>, dim3(grd.x, ny), blk, 0, stream, st, (double *)w.ws,
+                               chunk, nb, pools, (const uint8_t *)w.synT);
+        }
+        hipLaunchKernelGGL(plain_finalize_kernel, grd, blk, 0, stream, (const double *)w.ws, chunk, nb, cp.root_off,
+                           nout, p->tie_tol, s0, d_mar, d_cls, d_tie);
+    }
+    if ((rc = record(p, w, 1, stream))) return rc;
+    TQ_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+FusedArgs base_args(tqec_plan *p, DeviceData *dd, const uint8_t *d_syn, int64_t n, int glog, bool f32) {
+    const CompiledPlan &cp = p->cp;
+    FusedArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.steps = dd->steps;
+    a.nsteps = (int)cp.steps.size();
+    a.cpool = f32 ? (const void *)dd->cpool32 : (const void *)dd->cpool64;
+    a.spool = f32 ? (const void *)dd->spool32 : (const void *)dd->spool64;
+    a.syn = d_syn;
+    a.n = n;
+    a.nbits = cp.n_syn;
+    a.glog = glog;
+    a.nout = 1 << cp.n_out;
+    a.ws_entries = cp.ws_entries;
+    a.nslots = cp.n_slots;
+    a.root_off = cp.root_off;
+    a.root_slot = cp.root_slot;
+    a.tie_tol = p->tie_tol;
+    a.recheck_tol = p->recheck_tol;
+    return a;
+}
+
+int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, int mode, double *d_mar,
+              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+    const FusedCfg &c64 = p->f64;
+    if (c64.glog < 0)
+        return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP64)");
+    if (mode == TQEC_MODE_FUSED_F64) {
+        FusedArgs a = base_args(p, dd, d_syn, n, c64.glog, false);
+        a.marginals = d_mar;
+        a.classes = d_cls;
+        a.ties = d_tie;
+        const int64_t ntiles = (n + (1 << c64.glog) - 1) >> c64.glog;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, c64.blocks));
+        int rc0;
+        if ((rc0 = record(p, w, 0, stream))) return rc0;
+        hipLaunchKernelGGL((fused_kernel<double, false>), dim3(grid), dim3(c64.threads), c64.lds, stream, a);
+        if ((rc0 = record(p, w, 1, stream))) return rc0;
+        TQ_HIP(hipGetLastError());
+        return TQEC_OK;
+    }
+    const FusedCfg &c32 = p->f32;
+    if (c32.glog < 0)
+        return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP32)");
+    int rc = TQEC_OK;
+    if (w.list_cap < n) {
+        if (w.list) TQ_HIP(hipFree(w.list));
+        w.list = nullptr;
+        w.list_cap = 0;
+        TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+        w.list_cap = n;
+    }
+    if (!w.count) {
+        TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
+        TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
+    }
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
+    FusedArgs a = base_args(p, dd, d_syn, n, c32.glog, true);
+    a.marginals = d_mar;
+    a.classes = d_cls;
+    a.ties = d_tie;
+    a.recheck_list = w.list;
+    a.recheck_count = w.count;
+    const int64_t ntiles = (n + (1 << c32.glog) - 1) >> c32.glog;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, c32.blocks));
+    if ((rc = record(p, w, 0, stream))) return rc;
+    hipLaunchKernelGGL((fused_kernel<float, true>), dim3(grid), dim3(c32.threads), c32.lds, stream, a);
+    if ((rc = record(p, w, 1, stream))) return rc;
+    // FP64 re-check of the low-margin syndromes (device-side list, no host sync)
+    FusedArgs r = base_args(p, dd, d_syn, n, c64.glog, false);
+    r.list = w.list;
+    r.list_count = w.count;
+    r.marginals = d_mar;
+    r.classes = d_cls;
+    r.ties = d_tie;
+    const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(
+        (n + (1 << c64.glog) - 1) >> c64.glog, c64.blocks));
+    if ((rc = record(p, w, 2, stream))) return rc;
+    hipLaunchKernelGGL((fused_kernel<double, false>), dim3(rgrid), dim3(c64.threads), c64.lds, stream, r);
+    if ((rc = record(p, w, 3, stream))) return rc;
+    TQ_HIP(hipMemcpyAsync(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    TQ_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+int decode_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int mode, double *d_mar, int32_t *d_cls,
+                  uint8_t *d_tie, hipStream_t stream, Workspace **wout) {
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    int rc;
+    if ((rc = configure(p))) return rc;
+    DeviceData *dd;
+    if ((rc = ensure_device(p, dev, &dd))) return rc;
+    Workspace &w = p->ws[std::make_pair(dev, stream)];
+    w.device = dev;
+    if (wout) *wout = &w;
+    if (n == 0) return TQEC_OK;
+    if (mode == TQEC_MODE_PLAIN_F64) return run_plain(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+    if (mode == TQEC_MODE_FUSED_F64 || mode == TQEC_MODE_FUSED_F32)
+        return run_fused(p, dd, w, d_syn, n, mode, d_mar, d_cls, d_tie, stream);
+    return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+}
+
+}  // namespace
+
+extern "C" {
+
+int tqec_plan_create(const tqec_plan_desc *desc, tqec_plan **out) {
+    if (!out) return set_error(TQEC_ENULL, "null output handle");
+    *out = nullptr;
+    std::unique_ptr<tqec_plan> p(new (std::nothrow) tqec_plan());
+    if (!p) return set_error(TQEC_ENOMEM, "out of host memory");
+    std::string err;
+    int rc;
+    try {
+        rc = compile_plan(desc, p->cp, err);
+    } catch (const std::bad_alloc &) {
+        return set_error(TQEC_ENOMEM, "out of host memory while compiling the plan");
+    } catch (const std::exception &e) {
+        return set_error(TQEC_EINVAL, e.what());
+    }
+    if (rc != TQEC_OK) return set_error(rc, err);
+    *out = p.release();
+    return TQEC_OK;
+}
+
+void tqec_plan_destroy(tqec_plan *p) {
+    if (!p) return;
+    for (auto &kv : p->ws) {
+        Workspace &w = kv.second;
+        if (hipSetDevice(w.device) != hipSuccess) continue;
+        (void)hipStreamSynchronize(kv.first.second);
+        if (w.ws) (void)hipFree(w.ws);
+        if (w.synT) (void)hipFree(w.synT);
+        if (w.list) (void)hipFree(w.list);
+        if (w.count) (void)hipFree(w.count);
+        if (w.host_count) (void)hipHostFree(w.host_count);
+        if (w.d_syn) (void)hipFree(w.d_syn);
+        if (w.d_mar) (void)hipFree(w.d_mar);
+        if (w.d_cls) (void)hipFree(w.d_cls);
+        if (w.d_tie) (void)hipFree(w.d_tie);
+        for (auto &a : w.ev)
+            for (hipEvent_t e : a) (void)hipEventDestroy(e);
+    }
+    for (auto &kv : p->dev) {
+        if (hipSetDevice(kv.first) != hipSuccess) continue;
+        DeviceData &d = kv.second;
+        (void)hipFree(d.steps);
+        (void)hipFree(d.cpool64);
+        (void)hipFree(d.spool64);
+        (void)hipFree(d.cpool32);
+        (void)hipFree(d.spool32);
+    }
+    delete p;
+}
+
+int tqec_plan_stats(const tqec_plan *p, tqec_stats *s) {
+    if (!p || !s) return set_error(TQEC_ENULL, "null plan or stats pointer");
+    const CompiledPlan &cp = p->cp;
+    std::memset(s, 0, sizeof(*s));
+    s->tc = cp.tc;
+    s->rw = cp.rw;
+    s->sc = cp.sc;
+    s->n_steps = cp.n_steps;
+    s->n_dep_steps = (int32_t)cp.steps.size();
+    s->dep_tc = cp.dep_tc;
+    s->dep_rw = cp.dep_rw;
+    s->plain_ws_entries = cp.ws_entries;
+    s->fused_lds_entries = cp.ws_entries;
+    s->n_out = cp.n_out;
+    s->fused_tile_f64 = p->f64.glog >= 0 && p->cfg_done ? (1 << p->f64.glog) : 0;
+    s->fused_tile_f32 = p->f32.glog >= 0 && p->cfg_done ? (1 << p->f32.glog) : 0;
+    return TQEC_OK;
+}
+
+int tqec_plan_set_tolerances(tqec_plan *p, double tie_tol, double recheck_tol) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (!(tie_tol >= 0) || !(recheck_tol >= 0) || !std::isfinite(tie_tol) || !std::isfinite(recheck_tol))
+        return set_error(TQEC_EINVAL, "tolerances must be finite and non-negative");
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->tie_tol = tie_tol;
+    p->recheck_tol = recheck_tol;
+    return TQEC_OK;
+}
+
+int tqec_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int32_t mode, double *d_mar,
+                             int32_t *d_cls, uint8_t *d_tie, void *stream) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
+    if (n > 0 && !d_syn) return set_error(TQEC_ENULL, "null syndrome pointer");
+    if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
+    std::lock_guard<std::mutex> lk(p->mu);
+    return decode_device(p, d_syn, n, mode, d_mar, d_cls, d_tie, (hipStream_t)stream, nullptr);
+}
+
+int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode, double *mar, int32_t *cls,
+                      uint8_t *tie, void *stream) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
+    if (n > 0 && !syn) return set_error(TQEC_ENULL, "null syndrome pointer");
+    if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
+    if (mode < 0 || mode > 2) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+    std::lock_guard<std::mutex> lk(p->mu);
+    hipStream_t st = (hipStream_t)stream;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    Workspace &w = p->ws[std::make_pair(dev, st)];
+    w.device = dev;
+    if (n == 0) return TQEC_OK;
+    const int nout = 1 << p->cp.n_out;
+    int rc;
+    if ((rc = grow(&w.d_syn, &w.d_syn_bytes, (size_t)n * std::max(1, p->cp.n_syn)))) return rc;
+    if (mar && (rc = grow(&w.d_mar, &w.d_mar_bytes, (size_t)n * nout * sizeof(double)))) return rc;
+    if (cls && (rc = grow(&w.d_cls, &w.d_cls_bytes, (size_t)n * sizeof(int32_t)))) return rc;
+    if (tie && (rc = grow(&w.d_tie, &w.d_tie_bytes, (size_t)n))) return rc;
+    TQ_HIP(hipMemcpyAsync(w.d_syn, syn, (size_t)n * p->cp.n_syn, hipMemcpyHostToDevice, st));
+    Workspace *wp = nullptr;
+    if ((rc = decode_device(p, w.d_syn, n, mode, mar ? w.d_mar : nullptr, cls ? w.d_cls : nullptr,
+                            tie ? w.d_tie : nullptr, st, &wp)))
+        return rc;
+    if (mar) TQ_HIP(hipMemcpyAsync(mar, w.d_mar, (size_t)n * nout * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (cls) TQ_HIP(hipMemcpyAsync(cls, w.d_cls, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (tie) TQ_HIP(hipMemcpyAsync(tie, w.d_tie, (size_t)n, hipMemcpyDeviceToHost, st));
+    TQ_HIP(hipStreamSynchronize(st));
+    if (mode == TQEC_MODE_FUSED_F32 && wp && wp->host_count) wp->last_recheck = *wp->host_count;
+    return TQEC_OK;
+}
+
+int tqec_last_recheck_count(tqec_plan *p, void *stream, int64_t *count) {
+    if (!p || !count) return set_error(TQEC_ENULL, "null argument");
+    std::lock_guard<std::mutex> lk(p->mu);
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    auto it = p->ws.find(std::make_pair(dev, (hipStream_t)stream));
+    if (it == p->ws.end() || !it->second.host_count) {
+        *count = -1;
+        return TQEC_OK;
+    }
+    *count = *it->second.host_count;
+    return TQEC_OK;
+}
+
+int tqec_plan_set_timing(tqec_plan *p, int enable) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->timing = enable != 0;
+    return TQEC_OK;
+}
+
+int tqec_kernel_time_totals(tqec_plan *p, void *stream, double *main_ms, double *recheck_ms, int64_t *calls) {
+    if (!p || !main_ms || !recheck_ms || !calls) return set_error(TQEC_ENULL, "null argument");
+    std::lock_guard<std::mutex> lk(p->mu);
+    *main_ms = 0;
+    *recheck_ms = 0;
+    *calls = 0;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    auto it = p->ws.find(std::make_pair(dev, (hipStream_t)stream));
+    if (it == p->ws.end()) return TQEC_OK;
+    Workspace &w = it->second;
+    for (size_t c = 0; c < w.n_ev; ++c) {
+        float ms = 0;
+        if (w.ev_used[c][0] && w.ev_used[c][1]) {
+            TQ_HIP(hipEventSynchronize(w.ev[c][1]));
+            TQ_HIP(hipEventElapsedTime(&ms, w.ev[c][0], w.ev[c][1]));
+            *main_ms += ms;
+        }
+        if (w.ev_used[c][2] && w.ev_used[c][3]) {
+            TQ_HIP(hipEventSynchronize(w.ev[c][3]));
+            TQ_HIP(hipEventElapsedTime(&ms, w.ev[c][2], w.ev[c][3]));
+            *recheck_ms += ms;
+        }
+    }
+    *calls = (int64_t)w.n_ev;
+    w.n_ev = 0;
+    return TQEC_OK;
+}
+
+const char *tqec_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,232 @@
+"""ctypes binding of ``libtqec_hip.so`` (declared in include/tqec_hip.h).
+
+The product path always goes through this library; there is no CPU
+fallback.  Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtqec_hip.so")
+
+TQEC_OK = 0
+TQEC_EINVAL = -1
+TQEC_ENOMEM = -2
+TQEC_EDEVICE = -3
+TQEC_EUNSUPPORTED = -4
+TQEC_ENULL = -5
+
+MODE_PLAIN_F64 = 0
+MODE_FUSED_F64 = 1
+MODE_FUSED_F32 = 2
+MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32}
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_vars", ctypes.c_int32),
+        ("n_leaves", ctypes.c_int32),
+        ("leaf_rank", _i32p),
+        ("leaf_labels", _i32p),
+        ("leaf_data", _f64p),
+        ("leaf_evidence", _i32p),
+        ("n_steps", ctypes.c_int32),
+        ("step_lhs", _i32p),
+        ("step_rhs", _i32p),
+        ("step_out_rank", _i32p),
+        ("step_out_labels", _i32p),
+        ("n_out", ctypes.c_int32),
+        ("out_labels", _i32p),
+        ("n_syndrome_bits", ctypes.c_int32),
+        ("n_sliced", ctypes.c_int32),
+        ("sliced_label", _i32p),
+        ("sliced_bit", _i32p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("tc", ctypes.c_double),
+        ("rw", ctypes.c_double),
+        ("sc", ctypes.c_int32),
+        ("n_steps", ctypes.c_int32),
+        ("n_dep_steps", ctypes.c_int32),
+        ("dep_tc", ctypes.c_double),
+        ("dep_rw", ctypes.c_double),
+        ("plain_ws_entries", ctypes.c_int64),
+        ("fused_lds_entries", ctypes.c_int64),
+        ("n_out", ctypes.c_int32),
+        ("fused_tile_f64", ctypes.c_int32),
+        ("fused_tile_f32", ctypes.c_int32),
+    ]
+
+
+class TqecError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tqec error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported symbol.
+EXPORTS = [
+    ("tqec_plan_create", ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_plan_destroy", None, [ctypes.c_void_p]),
+    ("tqec_plan_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    ("tqec_decode_batch", ctypes.c_int,
+     [ctypes.c_void_p, _u8p, ctypes.c_int64, ctypes.c_int32, _f64p, _i32p, _u8p, ctypes.c_void_p]),
+    ("tqec_decode_batch_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p]),
+    ("tqec_plan_set_tolerances", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    ("tqec_last_recheck_count", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    ("tqec_plan_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("tqec_kernel_time_totals", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+      ctypes.POINTER(ctypes.c_int64)]),
+    ("tqec_last_error", ctypes.c_char_p, []),
+]
+
+
+def load(path: Optional[str] = None):
+    """Load the library (once).  Raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP library not built: {p} (run tensorqec.jl_amd/build.py)")
+    lib = ctypes.CDLL(p)
+    for name, res, args in EXPORTS:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().tqec_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int) -> None:
+    if rc != TQEC_OK:
+        raise TqecError(rc, last_error())
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class HipPlan:
+    """Owner of a ``tqec_plan*`` built from a :class:`planner.Plan`."""
+
+    def __init__(self, plan):
+        lib = load()
+        L = plan.n_leaves
+        ranks = _i32([len(l) for l in plan.leaf_labels])
+        labels = _i32([x for l in plan.leaf_labels for x in l]) if ranks.sum() else np.zeros(1, np.int32)
+        data = []
+        for t in range(L):
+            d = plan.leaf_data[t]
+            data.append(np.zeros(1 << len(plan.leaf_labels[t])) if d is None else np.asarray(d, dtype=np.float64))
+        data = np.ascontiguousarray(np.concatenate(data))
+        ev = _i32(plan.leaf_slot)
+        lhs, rhs = _i32(plan.lhs), _i32(plan.rhs)
+        orank = _i32([len(o) for o in plan.out_labels])
+        olab = _i32([x for o in plan.out_labels for x in o]) if orank.sum() else np.zeros(1, np.int32)
+        out = _i32(plan.out_labels_final) if plan.out_labels_final else np.zeros(1, np.int32)
+        sl = sorted(plan.evidence_label_bits.items())
+        sl_l = _i32([a for a, _ in sl]) if sl else np.zeros(1, np.int32)
+        sl_b = _i32([b for _, b in sl]) if sl else np.zeros(1, np.int32)
+        nvars = 1 + max([x for l in plan.leaf_labels for x in l] + [x for o in plan.out_labels for x in o] + [0])
+        self._keep = (ranks, labels, data, ev, lhs, rhs, orank, olab, out, sl_l, sl_b)
+        desc = PlanDesc(nvars, L, _ptr(ranks, _i32p), _ptr(labels, _i32p), _ptr(data, _f64p), _ptr(ev, _i32p),
+                        plan.n_steps, _ptr(lhs, _i32p), _ptr(rhs, _i32p), _ptr(orank, _i32p), _ptr(olab, _i32p),
+                        len(plan.out_labels_final), _ptr(out, _i32p), plan.n_syndrome_bits, len(sl),
+                        _ptr(sl_l, _i32p), _ptr(sl_b, _i32p))
+        h = ctypes.c_void_p()
+        check(lib.tqec_plan_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+        self.n_out = len(plan.out_labels_final)
+        self.n_syn = plan.n_syndrome_bits
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().tqec_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def stats(self) -> dict:
+        s = Stats()
+        check(load().tqec_plan_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def set_tolerances(self, tie_tol: float, recheck_tol: float) -> None:
+        check(load().tqec_plan_set_tolerances(self._h, float(tie_tol), float(recheck_tol)))
+
+    def decode(self, syndromes: np.ndarray, mode="fused_f32", want_marginals=True) -> Tuple:
+        """Host-buffer batch decode: returns (marginals or None, classes, ties)."""
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        s = np.ascontiguousarray(np.asarray(syndromes, dtype=np.uint8))
+        if s.ndim == 1:
+            s = s[None, :]
+        if s.ndim != 2 or s.shape[1] != self.n_syn:
+            raise ValueError(f"syndromes must be (B, {self.n_syn}); got {s.shape}")
+        B = s.shape[0]
+        nout = 1 << self.n_out
+        mar = np.empty((B, nout), dtype=np.float64) if want_marginals else None
+        cls = np.empty(B, dtype=np.int32)
+        tie = np.empty(B, dtype=np.uint8)
+        check(load().tqec_decode_batch(self._h, _ptr(s, _u8p), B, m,
+                                       _ptr(mar, _f64p) if mar is not None else None,
+                                       _ptr(cls, _i32p), _ptr(tie, _u8p), None))
+        return mar, cls, tie
+
+    def decode_device(self, d_syn: int, n: int, mode, d_mar: int, d_cls: int, d_tie: int, stream: int = 0) -> None:
+        """Device-pointer decode (integers are device addresses); enqueues only."""
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        check(load().tqec_decode_batch_device(self._h, ctypes.c_void_p(d_syn), int(n), m,
+                                              ctypes.c_void_p(d_mar) if d_mar else None,
+                                              ctypes.c_void_p(d_cls) if d_cls else None,
+                                              ctypes.c_void_p(d_tie) if d_tie else None,
+                                              ctypes.c_void_p(stream) if stream else None))
+
+    def set_timing(self, enable: bool) -> None:
+        check(load().tqec_plan_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_time_totals(self, stream: int = 0):
+        """(main kernel ms, FP64 re-check ms, calls) summed over the calls on
+        ``stream`` since the previous query (waits for their events)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        check(load().tqec_kernel_time_totals(self._h, ctypes.c_void_p(stream) if stream else None,
+                                             ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def last_recheck_count(self, stream: int = 0) -> int:
+        c = ctypes.c_int64()
+        check(load().tqec_last_recheck_count(self._h, ctypes.c_void_p(stream) if stream else None, ctypes.byref(c)))
+        return c.value

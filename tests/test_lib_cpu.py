@@ -1,0 +1,151 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol the
+header declares, and rejects malformed input with an error code (never an
+abort).  No compute call is made here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tensorqec import lib as L
+from tensorqec.codes import CSSTannerGraph, SurfaceCode
+from tensorqec.errors import iid_error
+from tensorqec import decoders as D
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    out = set()
+    for fn in os.listdir(os.path.join(REPO, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(REPO, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            out |= set(re.findall(r"\b(tqec_[a-z0-9_]+)\s*\(", txt))
+    return out
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    syms = declared_symbols()
+    assert "tqec_plan_create" in syms and "tqec_decode_batch" in syms
+    for s in sorted(syms):
+        assert hasattr(lib, s), s
+    # and the Python binding declares a signature for each of them
+    bound = {name for name, _, _ in L.EXPORTS}
+    assert syms <= bound, syms - bound
+
+
+def small_plan():
+    t = CSSTannerGraph.from_code(SurfaceCode(3, 3))
+    p = 0.05 / 3
+    ct = D.compile(D.TNMMAP(factorize=False, evidence="leaves", optimizer=D.GreedyMethod()), t,
+                   iid_error(p, p, p, 9))
+    return ct
+
+
+def desc_from(plan, **over):
+    """Build a PlanDesc (plus keep-alive arrays) from a Plan with overrides."""
+    i32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int32)) if len(a) else np.zeros(1, np.int32)
+    f = dict(
+        ranks=i32([len(l) for l in plan.leaf_labels]),
+        labels=i32([x for l in plan.leaf_labels for x in l]),
+        data=np.ascontiguousarray(np.concatenate([np.zeros(1 << len(l)) if d is None else d
+                                                  for l, d in zip(plan.leaf_labels, plan.leaf_data)])),
+        ev=i32(plan.leaf_slot), lhs=i32(plan.lhs), rhs=i32(plan.rhs),
+        orank=i32([len(o) for o in plan.out_labels]), olab=i32([x for o in plan.out_labels for x in o]),
+        out=i32(plan.out_labels_final), n_steps=plan.n_steps, n_leaves=plan.n_leaves,
+        n_out=len(plan.out_labels_final), n_syn=plan.n_syndrome_bits, n_vars=200)
+    f.update(over)
+    P = lambda a, t: a.ctypes.data_as(t)
+    d = L.PlanDesc(f["n_vars"], f["n_leaves"], P(f["ranks"], L._i32p), P(f["labels"], L._i32p),
+                   P(f["data"], L._f64p), P(f["ev"], L._i32p), f["n_steps"], P(f["lhs"], L._i32p),
+                   P(f["rhs"], L._i32p), P(f["orank"], L._i32p), P(f["olab"], L._i32p), f["n_out"],
+                   P(f["out"], L._i32p), f["n_syn"], 0, None, None)
+    return d, f
+
+
+def create(d):
+    h = ctypes.c_void_p()
+    rc = L.load().tqec_plan_create(ctypes.byref(d), ctypes.byref(h))
+    if rc == 0:
+        L.load().tqec_plan_destroy(h)
+    return rc
+
+
+def test_valid_descriptor_accepted():
+    ct = small_plan()
+    d, _ = desc_from(ct.plan)
+    assert create(d) == 0
+
+
+def test_malformed_descriptors_rejected():
+    ct = small_plan()
+    plan = ct.plan
+    lib = L.load()
+    h = ctypes.c_void_p()
+    assert lib.tqec_plan_create(None, ctypes.byref(h)) == L.TQEC_ENULL
+    d, _ = desc_from(plan, n_steps=plan.n_steps - 1)
+    assert create(d) == L.TQEC_EINVAL
+    bad = np.array([x for l in plan.leaf_labels for x in l], np.int32)
+    bad[0] = 10_000
+    d, _ = desc_from(plan, labels=bad)
+    assert create(d) == L.TQEC_EINVAL
+    assert "out of range" in L.last_error()
+    data = desc_from(plan)[1]["data"].copy()
+    data[3] = np.nan
+    d, _ = desc_from(plan, data=data)
+    assert create(d) == L.TQEC_EINVAL
+    data[3] = -0.5
+    d, _ = desc_from(plan, data=data)
+    assert create(d) == L.TQEC_EINVAL
+    assert "non-negative" in L.last_error()
+    lhs = np.array(plan.lhs, np.int32)
+    lhs[-1] = lhs[0]          # a node consumed twice
+    d, _ = desc_from(plan, lhs=lhs)
+    assert create(d) == L.TQEC_EINVAL
+    olab = [list(o) for o in plan.out_labels]
+    # drop a label that is still needed later: remove the first kept label of the first step with >1 label
+    for i, o in enumerate(olab[:-1]):
+        if len(o) > 1:
+            olab[i] = o[1:]
+            break
+    orank = np.array([len(o) for o in olab], np.int32)
+    d, _ = desc_from(plan, olab=np.array([x for o in olab for x in o], np.int32), orank=orank)
+    assert create(d) == L.TQEC_EINVAL
+    out = np.array(plan.out_labels_final[::-1][:1], np.int32)
+    d, _ = desc_from(plan, out=out, n_out=1)
+    assert create(d) == L.TQEC_EINVAL
+    ev = np.array(plan.leaf_slot, np.int32)
+    first_ev = int(np.flatnonzero(ev >= 0)[0])
+    ev[first_ev] = 99          # syndrome bit out of range
+    d, _ = desc_from(plan, ev=ev)
+    assert create(d) == L.TQEC_EINVAL
+
+
+def test_null_and_argument_errors():
+    lib = L.load()
+    assert lib.tqec_plan_stats(None, None) == L.TQEC_ENULL
+    assert lib.tqec_decode_batch(None, None, 1, 0, None, None, None, None) == L.TQEC_ENULL
+    assert lib.tqec_decode_batch_device(None, None, 1, 0, None, None, None, None) == L.TQEC_ENULL
+    ct = small_plan()
+    h = ct.hip.handle
+    assert lib.tqec_decode_batch(h, None, -1, 0, None, None, None, None) == L.TQEC_EINVAL
+    assert lib.tqec_decode_batch(h, None, 4, 0, None, None, None, None) == L.TQEC_ENULL
+    buf = np.zeros((1, 8), np.uint8)
+    assert lib.tqec_decode_batch(h, buf.ctypes.data_as(L._u8p), 1, 7, None, None, None, None) == L.TQEC_EINVAL
+    assert lib.tqec_plan_set_tolerances(h, -1.0, 1e-3) == L.TQEC_EINVAL
+    assert lib.tqec_plan_set_tolerances(h, float("nan"), 1e-3) == L.TQEC_EINVAL
+    lib.tqec_plan_destroy(None)  # no-op
+    with pytest.raises(ValueError):
+        ct.hip.decode(np.zeros((2, 7), np.uint8))   # wrong syndrome length
+
+
+def test_stats_mirror_plan():
+    ct = small_plan()
+    s = ct.hip.stats()
+    ps = ct.plan.stats()
+    assert s["tc"] == ps["tc"] and s["dep_tc"] == ps["tc_dep"] and s["sc"] == ps["sc"]
+    assert s["n_out"] == 2 and s["n_steps"] == ct.plan.n_steps
+    assert s["plain_ws_entries"] > 0
