@@ -17,6 +17,7 @@ import pytest
 from oracle import tn_oracle as O
 from oracle.cexec import OraclePlan
 from tensorqec import decoders as D
+from tensorqec import lib as L
 from tensorqec.codes import CSSTannerGraph, ShorCode, SteaneCode, SurfaceCode, ToricCode
 from tensorqec.errors import (CSSErrorPattern, check_logical_error, css_syndrome_rows, iid_error,
                               sample_depolarizing, syndrome_extraction)
@@ -37,7 +38,9 @@ def say(*a):
 def check_marginals(m, ref, mode):
     ref = np.asarray(ref)
     if mode == "fused_f32":
-        err = np.max(np.abs(m - ref) / ref.max(axis=1, keepdims=True))
+        valid = ref.max(axis=1) > 0          # impossible syndromes: every class exactly 0
+        assert np.all(m[~valid] == 0.0)
+        err = np.max(np.abs(m[valid] - ref[valid]) / ref[valid].max(axis=1, keepdims=True))
         assert err < 2e-5, err
         return
     nz = ref > 0
@@ -110,7 +113,15 @@ def test_seeded_batches_vs_c_oracle(code, p, net):
     S = css_syndrome_rows(ep, t)
     ct = D.compile(D.TNMMAP(**net), t, em)
     ref, _, _ = OraclePlan(ct.plan).decode(S)
+    lds_f64 = ct.hip.stats()["fused_lds_entries"] * 8
     for mode in MODES:
+        if mode != "plain_f64" and lds_f64 > 160 * 1024:
+            # a tensor this wide cannot stay in one workgroup's LDS: the fused
+            # executors refuse the plan instead of spilling
+            with pytest.raises(L.TqecError) as e:
+                D.decode_batch(ct, S, mode=mode)
+            assert e.value.code == L.TQEC_EUNSUPPORTED
+            continue
         r = D.decode_batch(ct, S, mode=mode)
         check_marginals(r.marginals, ref, mode)
         check_classes(r.classes, r.ties, ref)
@@ -166,7 +177,7 @@ def test_device_pointer_api_matches_host_api():
 
 def test_empty_and_ragged_batches():
     t = CSSTannerGraph.from_code(SurfaceCode(3, 3))
-    ct = D.compile(D.TNMMAP(), t, iid_error(0.02, 0.02, 0.02, 9))
+    ct = D.compile(D.TNMMAP(), t, iid_error(0.05 / 3, 0.05 / 3, 0.05 / 3, 9))
     for mode in MODES:
         m, c, tie = ct.hip.decode(np.zeros((0, 8), np.uint8), mode)
         assert m.shape == (0, 4)
