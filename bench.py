@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--config", default=None,
+                    help="JSON line of an earlier run: reuse its plan and tile choice (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (syndrome batches shard "
@@ -82,21 +84,53 @@ def main():
     d, p, B = args.d, args.p, args.batch
     t = CSSTannerGraph.from_code(SurfaceCode(d, d))
     em = iid_error(p / 3, p / 3, p / 3, t.nq)
-    t0 = time.time()
-    chosen = D.compile(D.TNMMAP(mode="fused_f32"), t, em)
-    plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
-    print(f"[bench] plans compiled in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-
     nb = args.warmup + args.steps
     eps = [sample_depolarizing(em, args.seed, B, first=i * B) for i in range(nb)]
     S_host = [css_syndrome_rows(e, t) for e in eps]
-    truth = [true_classes(e, chosen.lx, chosen.lz) for e in eps]
     S_dev = [torch.from_numpy(s).to(dev) for s in S_host]
+    stream = torch.cuda.current_stream().cuda_stream
+
+    t0 = time.time()
+    plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
+    # Candidate contraction trees for the fused executor (seed, width target);
+    # each is autotuned on the first batch and the fastest is kept.
+    if args.config:
+        pinned = json.loads(open(args.config).read().strip().splitlines()[-1])["config"]["plan"]
+        cands = [(pinned["seed"], pinned["sc_target"])]
+    else:
+        cands = [(1, 10), (2, 10), (3, 10), (1, 9), (3, None), (2, 9)]
+    best = None
+    for seed, sct in cands:
+        ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.TreeSA(seed=seed, sc_target=sct)), t, em)
+        if args.config:
+            ct.hip.set_tile("fused_f32", pinned["tile"], pinned["threads"])
+            tile, thr = pinned["tile"], pinned["threads"]
+            ms = 0.0
+        else:
+            tile, thr = ct.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            outs = [torch.empty((B, 4), dtype=torch.float64, device=dev), torch.empty(B, dtype=torch.int32, device=dev),
+                    torch.empty(B, dtype=torch.uint8, device=dev)]
+            ev0.record()
+            for _ in range(3):
+                ct.hip.decode_device(S_dev[0].data_ptr(), B, "fused_f32", outs[0].data_ptr(), outs[1].data_ptr(),
+                                     outs[2].data_ptr(), stream)
+            ev1.record()
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / 3
+        st = ct.hip.stats()
+        print(f"[bench] candidate seed={seed} sc_target={sct}: steps={st['n_dep_steps']} macs={st['dep_tc']:.0f} "
+              f"lds/syn={st['fused_lds_entries']} tile={tile} threads={thr} {ms:.3f} ms/batch",
+              file=sys.stderr, flush=True)
+        if best is None or ms < best[0]:
+            best = (ms, ct, {"seed": seed, "sc_target": sct, "tile": tile, "threads": thr})
+    chosen, plan_cfg = best[1], best[2]
+    print(f"[bench] plans compiled and tuned in {time.time() - t0:.1f}s: {plan_cfg}", file=sys.stderr, flush=True)
+    truth = [true_classes(e, chosen.lx, chosen.lz) for e in eps]
     nout = 1 << chosen.hip.n_out
     mar = torch.empty((B, nout), dtype=torch.float64, device=dev)
     cls = torch.empty(B, dtype=torch.int32, device=dev)
     tie = torch.empty(B, dtype=torch.uint8, device=dev)
-    stream = torch.cuda.current_stream().cuda_stream
     print(f"[bench] {nb} batches of {B} syndromes resident", file=sys.stderr, flush=True)
 
     def run(ct, mode, label):
@@ -191,7 +225,11 @@ def main():
         "config": {"workload": f"rotated surface code d={d}, code-capacity depolarizing p={p} (p/3 each), "
                                f"{B} syndromes per step, TNMMAP marginals over 4 logical classes",
                    "batch": B, "d": d, "p": p, "seed": args.seed,
-                   "chosen": "fused_f32 (chain-factorized network, evidence as slices, FP64 re-check)",
+                   "chosen": "fused_f32 (chain-factorized network, evidence as slices, subtrees on <= 4 "
+                             "syndrome bits tabulated, FP64 re-check)",
+                   "plan": plan_cfg,
+                   "plan_stats": {k: st[k] for k in ("n_dep_steps", "dep_tc", "dep_rw", "fused_lds_entries",
+                                                     "table_entries", "const_entries")},
                    "plain": "plain_f64 (reference network, one-hot evidence, one kernel per step)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,

@@ -82,6 +82,12 @@ typedef struct tqec_plan_desc {
     int32_t n_sliced;               /* labels fixed by a syndrome bit       */
     const int32_t *sliced_label;    /* [n_sliced] label id                  */
     const int32_t *sliced_bit;      /* [n_sliced] syndrome bit              */
+    int32_t tabulate_bits;          /* 0: only syndrome-independent subtrees
+                                       are contracted at plan creation;
+                                       1..4: subtrees that depend on at most
+                                       this many syndrome bits are also
+                                       contracted there, once per bit
+                                       pattern, and looked up per syndrome */
 } tqec_plan_desc;
 
 typedef struct tqec_plan tqec_plan;
@@ -101,6 +107,10 @@ typedef struct tqec_stats {
     int32_t n_out;          /* number of logical classes = 2^n_out        */
     int32_t fused_tile_f64; /* syndromes per workgroup, FUSED_F64         */
     int32_t fused_tile_f32; /* syndromes per workgroup, FUSED_F32         */
+    int64_t const_entries;  /* folded syndrome-independent operand data    */
+    int64_t table_entries;  /* tabulated operand data (all bit patterns)   */
+    int32_t tabulate_bits;  /* subtrees on at most this many syndrome bits
+                               were tabulated at plan creation             */
 } tqec_stats;
 
 /* Build a plan.  The descriptor's arrays are copied; the caller keeps them. */
@@ -134,6 +144,17 @@ int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol
 /* Syndromes re-contracted in FP64 by the last FUSED_F32 call on `stream`
  * (host-visible after the call has completed). */
 int tqec_last_recheck_count(tqec_plan *plan, void *stream, int64_t *count);
+
+/* Project autotuner: time every feasible (syndromes per workgroup, threads
+ * per workgroup) configuration of a fused mode on a trial batch already in
+ * device memory, keep the fastest for later calls of that mode and report it
+ * (tile / threads may be NULL).  Synchronises `stream`. */
+int tqec_plan_autotune(tqec_plan *plan, int32_t mode, const uint8_t *d_syndromes, int64_t n, void *stream,
+                       int32_t *tile, int32_t *threads);
+
+/* Pin the tile configuration of a fused mode (tile: syndromes per workgroup,
+ * a power of two <= 64; threads: 64, 128 or 256). */
+int tqec_plan_set_tile(tqec_plan *plan, int32_t mode, int32_t tile, int32_t threads);
 
 /* Kernel timing with HIP events recorded on the call's stream (off by
  * default).  While enabled, every decode call records events around its

@@ -1,13 +1,15 @@
 // plan.hpp — compiled contraction plan shared by the host compiler and the
 // HIP kernels.
 //
-// A plan step computes  C[c] = sum_{k, sa, sb} A[a(c) | sa<<nk | k] * B[b(c) | sb<<nk | k]
-// where c runs over C's entries in C's memory order, k over the labels
-// contracted between A and B (always the low nk bits of both operands), sa/sb
-// over labels summed on one side only (the next bits) and a(c)/b(c) move the
-// bits of c that belong to each operand into its high bits.  Every operand's
-// memory order is chosen by its single consumer, so no step ever needs a
-// separate transpose pass.
+// A plan step computes
+//   C[c] = sum_{k, sa, sb} A[a(c) | sa<<A.sshift | k<<A.kshift] * B[b(c) | sb<<B.sshift | k<<B.kshift]
+// where c runs over C's entries in C's memory order, a(c)/b(c) move the bits
+// of c that belong to each operand into its low bits (keeping C's order), sa
+// and sb run over labels summed on one side only and k over the labels
+// contracted between A and B (the top bits of both operands).  Every
+// operand's memory order is chosen by its single consumer, so no step ever
+// needs a separate transpose pass, and lanes that handle consecutive outputs
+// read consecutive (or identical) words.
 #pragma once
 #include <stdint.h>
 
@@ -22,7 +24,6 @@ enum OperandKind : int32_t {
     kInter = 0,   // per-syndrome intermediate in the workspace
     kConst = 1,   // syndrome-independent tensor in the constant pool
     kSliced = 2,  // leaf sliced by syndrome bits: one variant per bit pattern
-    kOneHot = 3,  // evidence leaf: value(i) = (i == syndrome bit)
 };
 
 struct Seg {
@@ -35,12 +36,14 @@ struct Seg {
 struct OperandDesc {
     int32_t kind;
     int32_t rank;      // effective rank (after slicing / pre-summing)
-    int32_t nsingle;   // labels summed on this side only
+    int32_t nsingle;   // labels summed on this side only (bits sshift..)
     int32_t nseg;
+    int32_t sshift;    // first bit of the single-side labels
+    int32_t kshift;    // first bit of the contracted labels
     int64_t off;       // kInter: entry offset per syndrome; kConst/kSliced: pool offset
     int32_t slot;      // kInter: scale slot; otherwise -1
     int32_t cexp;      // power-of-two exponent folded out of kConst/kSliced data
-    int32_t nev;       // kSliced: variant bits; kOneHot: 1
+    int32_t nev;       // kSliced: number of syndrome bits selecting the variant
     int32_t ev_bit[kMaxEv];
     Seg seg[kMaxSeg];
 };

@@ -148,7 +148,9 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
     if (d->n_out < 0 || d->n_out > kMaxOut) { err = fmt("n_out out of range (%ld)", d->n_out); return TQEC_EUNSUPPORTED; }
     if (d->n_syndrome_bits < 0 || d->n_syndrome_bits > (1 << 20)) { err = "n_syndrome_bits out of range"; return TQEC_EINVAL; }
     if (d->n_sliced < 0 || (d->n_sliced > 0 && (!d->sliced_label || !d->sliced_bit))) { err = "bad sliced arrays"; return TQEC_EINVAL; }
+    if (d->tabulate_bits < 0 || d->tabulate_bits > kMaxEv) { err = fmt("tabulate_bits must be in [0, %ld]", kMaxEv); return TQEC_EINVAL; }
     p.n_vars = NV; p.n_leaves = NL; p.n_steps = NS; p.n_out = d->n_out; p.n_syn = d->n_syndrome_bits;
+    p.fold_limit = d->tabulate_bits;
     p.slice_bit_of_label.assign(NV, -1);
     for (int i = 0; i < d->n_sliced; ++i) {
         int l = d->sliced_label[i], b = d->sliced_bit[i];
@@ -256,17 +258,84 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
         std::set<int> rs(rl.begin(), rl.end()), os(p.out_labels.begin(), p.out_labels.end());
         if (rs != os) { err = "root labels differ from out_labels"; return TQEC_EINVAL; }
     }
-    // ---- dependence, statistics, constant folding -------------------------
+    // ---- syndrome dependence ----------------------------------------------
+    // bits[x]: syndrome bits node x depends on.  Nodes that depend on no bit
+    // are contracted once here (constant folding); nodes that depend on at
+    // most `fold_bits` bits are tabulated here for every assignment of those
+    // bits (the device reads the right variant); only the rest run per
+    // syndrome on the device.
+    p.bits.assign(NL + NS, {});
+    for (int t = 0; t < NL; ++t) {
+        std::vector<int> &B = p.bits[t];
+        if (p.leaf_ev[t] >= 0) B.push_back(p.leaf_ev[t]);
+        for (int l : p.raw_labels[t])
+            if (p.slice_bit_of_label[l] >= 0) B.push_back(p.slice_bit_of_label[l]);
+        std::sort(B.begin(), B.end());
+        B.erase(std::unique(B.begin(), B.end()), B.end());
+        if ((int)B.size() > kMaxEv) {
+            err = fmt("leaf %ld depends on %ld syndrome bits (limit %ld)", t, (long)B.size(), kMaxEv);
+            return TQEC_EUNSUPPORTED;
+        }
+    }
+    for (int i = 0; i < NS; ++i) {
+        std::vector<int> u = p.bits[p.lhs[i]];
+        u.insert(u.end(), p.bits[p.rhs[i]].begin(), p.bits[p.rhs[i]].end());
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        p.bits[NL + i] = u;
+    }
+    const int root = NL + NS - 1;
+    if (p.bits[root].empty()) { err = "the result does not depend on any syndrome bit"; return TQEC_EUNSUPPORTED; }
+    p.fold_bits = std::min<int>(p.fold_limit, (int)p.bits[root].size() - 1);
     p.dep.assign(NL + NS, 0);
-    for (int t = 0; t < NL; ++t) p.dep[t] = (p.leaf_ev[t] >= 0) || (p.labels[t].size() != p.raw_labels[t].size());
+    for (int x = NL; x < NL + NS; ++x) p.dep[x] = (int)p.bits[x].size() > p.fold_bits;
+
+    // ---- statistics, constant folding and tabulation ----------------------
     p.value.resize(NL + NS);
+    p.var.resize(NL + NS);
+    auto restrict_bits = [&](int v, const std::vector<int> &from, const std::vector<int> &to) {
+        int r = 0;
+        for (size_t j = 0; j < to.size(); ++j) {
+            const int pos = (int)(std::find(from.begin(), from.end(), to[j]) - from.begin());
+            r |= ((v >> pos) & 1) << j;
+        }
+        return r;
+    };
     for (int t = 0; t < NL; ++t) {
         p.sc = std::max(p.sc, (int)p.labels[t].size());
-        if (!p.dep[t]) p.value[t] = p.raw_data[t];
+        if (p.bits[t].empty()) {
+            p.value[t] = p.raw_data[t];
+            continue;
+        }
+        const int nv = 1 << p.bits[t].size();
+        p.var[t].resize(nv);
+        if (p.leaf_ev[t] >= 0) {
+            p.var[t][0] = {1.0, 0.0};
+            p.var[t][1] = {0.0, 1.0};
+            continue;
+        }
+        const std::vector<int> &raw = p.raw_labels[t];
+        const size_t n_eff = size_t(1) << p.labels[t].size();
+        for (int v = 0; v < nv; ++v) p.var[t][v].assign(n_eff, 0.0);
+        for (uint64_t xi = 0; xi < (uint64_t(1) << raw.size()); ++xi) {
+            int v = 0;
+            uint64_t ki = 0;
+            int kpos = 0;
+            for (size_t q = 0; q < raw.size(); ++q) {
+                const int bit = (int)((xi >> q) & 1);
+                const int sb = p.slice_bit_of_label[raw[q]];
+                if (sb >= 0) {
+                    const int j = (int)(std::find(p.bits[t].begin(), p.bits[t].end(), sb) - p.bits[t].begin());
+                    v |= bit << j;
+                } else {
+                    ki |= uint64_t(bit) << kpos++;
+                }
+            }
+            p.var[t][v][ki] += p.raw_data[t][xi];
+        }
     }
     for (int i = 0; i < NS; ++i) {
         int node = NL + i, a = p.lhs[i], b = p.rhs[i];
-        p.dep[node] = p.dep[a] || p.dep[b];
         std::set<int> u(p.labels[a].begin(), p.labels[a].end());
         u.insert(p.labels[b].begin(), p.labels[b].end());
         double w = std::ldexp(1.0, (int)u.size());
@@ -283,15 +352,25 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
                 err = fmt("step %ld: per-syndrome tensor of rank %ld exceeds the limit %ld", i, (long)p.labels[node].size(), kMaxRank);
                 return TQEC_EUNSUPPORTED;
             }
-        } else {
-            if (u.size() > 26) { err = fmt("constant step %ld too wide (%ld labels)", i, (long)u.size()); return TQEC_EUNSUPPORTED; }
+            continue;
+        }
+        if (u.size() > 26) { err = fmt("host-evaluated step %ld too wide (%ld labels)", i, (long)u.size()); return TQEC_EUNSUPPORTED; }
+        if (p.bits[node].empty()) {
             p.value[node] = host_contract(p.value[a], p.labels[a], p.value[b], p.labels[b], p.labels[node]);
-            p.value[a].clear(); p.value[a].shrink_to_fit();
-            p.value[b].clear(); p.value[b].shrink_to_fit();
+        } else {
+            const int nv = 1 << p.bits[node].size();
+            p.var[node].resize(nv);
+            for (int v = 0; v < nv; ++v) {
+                const std::vector<double> &A = p.bits[a].empty() ? p.value[a] : p.var[a][restrict_bits(v, p.bits[node], p.bits[a])];
+                const std::vector<double> &B = p.bits[b].empty() ? p.value[b] : p.var[b][restrict_bits(v, p.bits[node], p.bits[b])];
+                p.var[node][v] = host_contract(A, p.labels[a], B, p.labels[b], p.labels[node]);
+            }
+        }
+        for (int x : {a, b}) {
+            std::vector<double>().swap(p.value[x]);
+            std::vector<std::vector<double>>().swap(p.var[x]);
         }
     }
-    const int root = NL + NS - 1;
-    if (!p.dep[root]) { err = "the result does not depend on any syndrome bit"; return TQEC_EUNSUPPORTED; }
 
     // ---- layouts, top-down -----------------------------------------------
     p.mem.assign(NL + NS, {});
@@ -318,16 +397,18 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
         auto build = [&](int x, const std::vector<int> &S, const std::vector<int> &R, OperandDesc &op) -> int {
             std::memset(&op, 0, sizeof(op));
             op.slot = -1;
-            std::vector<int> order = K;
-            const bool dep_x = p.dep[x];
-            const bool presum = !dep_x || (x < NL && p.leaf_ev[x] < 0);  // const or sliced leaf
-            if (!presum) order.insert(order.end(), S.begin(), S.end());
-            order.insert(order.end(), R.begin(), R.end());
+            const bool inter = p.dep[x];
+            // memory order: kept labels in C's order (low bits), then labels
+            // summed on this side only, then the contracted labels (high bits)
+            std::vector<int> order = R;
+            if (inter) order.insert(order.end(), S.begin(), S.end());
+            order.insert(order.end(), K.begin(), K.end());
             p.mem[x] = order;
             op.rank = (int)order.size();
-            op.nsingle = presum ? 0 : (int)S.size();
-            // bit-field moves from C's index into the operand's high bits
-            int dst = st.nk + op.nsingle;
+            op.nsingle = inter ? (int)S.size() : 0;
+            op.sshift = (int)R.size();
+            op.kshift = (int)R.size() + op.nsingle;
+            int dst = 0;
             int nseg = 0;
             for (size_t ci = 0; ci < C.size();) {
                 if (!contains(R, C[ci])) { ++ci; continue; }
@@ -342,65 +423,34 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
                 ci = cj;
             }
             op.nseg = nseg;
-            if (!dep_x) {
+            if (inter) {
+                op.kind = kInter;
+            } else if (p.bits[x].empty()) {
                 op.kind = kConst;
                 std::vector<double> v = host_reduce(p.value[x], p.labels[x], order);
                 op.cexp = normalise(v);
                 op.off = (int64_t)p.const_pool.size();
                 p.const_pool.insert(p.const_pool.end(), v.begin(), v.end());
-            } else if (x < NL && p.leaf_ev[x] >= 0) {
-                op.kind = kOneHot;
-                op.nev = 1;
-                op.ev_bit[0] = p.leaf_ev[x];
-            } else if (x < NL) {
+            } else {
                 op.kind = kSliced;
-                const std::vector<int> &raw = p.raw_labels[x];
-                std::vector<int> sl;
-                for (int l : raw) if (p.slice_bit_of_label[l] >= 0) sl.push_back(l);
-                if ((int)sl.size() > kMaxEv) return -2;
-                op.nev = (int)sl.size();
-                for (size_t j = 0; j < sl.size(); ++j) op.ev_bit[j] = p.slice_bit_of_label[sl[j]];
-                op.off = (int64_t)p.sliced_pool.size();
-                // one variant per assignment of the sliced labels
+                op.nev = (int)p.bits[x].size();
+                for (int j = 0; j < op.nev; ++j) op.ev_bit[j] = p.bits[x][j];
                 std::vector<std::vector<double>> vars;
                 double m = 0;
-                for (int v = 0; v < (1 << sl.size()); ++v) {
-                    // select the slice, then reduce/permute
-                    std::vector<int> keep;
-                    for (int l : raw) if (p.slice_bit_of_label[l] < 0) keep.push_back(l);
-                    std::vector<double> s(size_t(1) << keep.size(), 0.0);
-                    for (uint64_t xi = 0; xi < (uint64_t(1) << raw.size()); ++xi) {
-                        bool ok = true;
-                        uint64_t ki = 0;
-                        int kpos = 0;
-                        for (size_t q = 0; q < raw.size(); ++q) {
-                            int l = raw[q];
-                            int bit = (xi >> q) & 1;
-                            auto it = std::find(sl.begin(), sl.end(), l);
-                            if (it != sl.end()) {
-                                if (bit != ((v >> (it - sl.begin())) & 1)) { ok = false; break; }
-                            } else {
-                                ki |= uint64_t(bit) << kpos++;
-                            }
-                        }
-                        if (ok) s[ki] += p.raw_data[x][xi];
-                    }
-                    std::vector<double> r = host_reduce(s, keep, order);
-                    for (double q : r) m = std::max(m, q);
-                    vars.push_back(r);
+                for (const auto &vv : p.var[x]) {
+                    vars.push_back(host_reduce(vv, p.labels[x], order));
+                    for (double q : vars.back()) m = std::max(m, q);
                 }
                 op.cexp = (m > 0) ? std::ilogb(m) : 0;
+                op.off = (int64_t)p.sliced_pool.size();
                 for (auto &r : vars)
                     for (double q : r) p.sliced_pool.push_back(std::ldexp(q, -op.cexp));
-            } else {
-                op.kind = kInter;
             }
             return 0;
         };
         int ea = build(a, SA, AR, st.a);
         int eb = build(b, SB, BR, st.b);
         if (ea == -1 || eb == -1) { err = fmt("step %ld: too many index segments", i); return TQEC_EUNSUPPORTED; }
-        if (ea == -2 || eb == -2) { err = fmt("step %ld: a leaf carries more than %ld sliced labels", i, kMaxEv); return TQEC_EUNSUPPORTED; }
         rev.push_back(st);
     }
     // ---- memory plan (execution order) ------------------------------------

@@ -23,7 +23,11 @@ struct CompiledPlan {
 
     // ---- derived ---------------------------------------------------------
     std::vector<std::vector<int>> labels;       // effective labels per node
-    std::vector<char> dep;                      // syndrome-dependent node
+    std::vector<std::vector<int>> bits;         // syndrome bits a node depends on
+    std::vector<char> dep;                      // node contracted per syndrome on the device
+    std::vector<std::vector<std::vector<double>>> var;  // tabulated nodes: one tensor per bit pattern
+    int fold_limit = kMaxEv;                    // tabulate nodes with at most this many bits
+    int fold_bits = 0;                          // limit actually used
     std::vector<std::vector<double>> value;     // constant nodes (labels order)
     std::vector<std::vector<int>> mem;          // memory order of operands
 

@@ -63,36 +63,6 @@ struct Pools {
     const T *spool;
 };
 
-// Plain executor operand fetch: syndrome b of a chunk, workspace stride ldw.
-template <typename T>
-__device__ __forceinline__ T plain_load(const OperandDesc &op, uint32_t idx, int64_t b, const T *__restrict__ ws,
-                                        int64_t ldw, Pools<T> pools, const uint8_t *__restrict__ synT) {
-    switch (op.kind) {
-        case kInter:
-            return ws[(op.off + (int64_t)idx) * ldw + b];
-        case kConst:
-            return pools.cpool[op.off + idx];
-        case kSliced: {
-            uint32_t v = 0;
-            for (int j = 0; j < op.nev; ++j) v |= (uint32_t)synT[(int64_t)op.ev_bit[j] * ldw + b] << j;
-            return pools.spool[op.off + ((int64_t)v << op.rank) + idx];
-        }
-        default:
-            return (T)((idx & 1u) == (uint32_t)synT[(int64_t)op.ev_bit[0] * ldw + b]);
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ T plain_load_sum(const OperandDesc &op, uint32_t base, int nk, int64_t b,
-                                            const T *__restrict__ ws, int64_t ldw, Pools<T> pools,
-                                            const uint8_t *__restrict__ synT) {
-    if (op.nsingle == 0) return plain_load(op, base, b, ws, ldw, pools, synT);
-    T s = 0;
-    const uint32_t ns = 1u << op.nsingle;
-    for (uint32_t q = 0; q < ns; ++q) s += plain_load(op, base | (q << nk), b, ws, ldw, pools, synT);
-    return s;
-}
-
 // Transpose a chunk of syndromes [n][nbits] (row-major bytes) to [bit][ldw].
 __global__ void transpose_syndromes_kernel(const uint8_t *__restrict__ syn, int64_t n, int nbits,
                                            uint8_t *__restrict__ synT, int64_t ldw) {
@@ -102,27 +72,77 @@ __global__ void transpose_syndromes_kernel(const uint8_t *__restrict__ syn, int6
     for (int i = 0; i < nbits; ++i) synT[(int64_t)i * ldw + b] = row[i] & 1u;
 }
 
-// One plan step for a chunk: thread = (output entry c, syndrome b).
+// Element `idx` of an operand.  Workspace operands (kInter) live at
+// base[idx * stride]; table and constant operands at base[idx].
+template <typename T, bool WS>
+__device__ __forceinline__ T op_load(const T *base, uint32_t idx, int64_t stride) {
+    return WS ? base[(int64_t)idx * stride] : base[idx];
+}
+
+// Sum over the operand's single-side labels (only workspace operands carry
+// any; table and constant operands are pre-summed at plan creation).
+template <typename T, bool WS>
+__device__ __forceinline__ T op_load_sum(const T *base, uint32_t idx, int64_t stride, const OperandDesc &op) {
+    if (!WS || op.nsingle == 0) return op_load<T, WS>(base, idx, stride);
+    T s = 0;
+    const uint32_t ns = 1u << op.nsingle;
+    for (uint32_t q = 0; q < ns; ++q) s += op_load<T, WS>(base, idx | (q << op.sshift), stride);
+    return s;
+}
+
+// One output element: sum over the contracted labels.
+template <typename T, bool AW, bool BW>
+__device__ __forceinline__ T contract_one(const StepDesc &st, uint32_t c, const T *pa, const T *pb, int64_t stride) {
+    const uint32_t ar = gather_bits(c, st.a);
+    const uint32_t br = gather_bits(c, st.b);
+    const uint32_t nkk = 1u << st.nk;
+    T acc = 0;
+    for (uint32_t k = 0; k < nkk; ++k) {
+        const T va = op_load_sum<T, AW>(pa, ar | (k << st.a.kshift), stride, st.a);
+        const T vb = op_load_sum<T, BW>(pb, br | (k << st.b.kshift), stride, st.b);
+        acc = fma(va, vb, acc);
+    }
+    return acc;
+}
+
+// Base pointer of an operand for syndrome b of the plain executor.
+template <typename T>
+__device__ __forceinline__ const T *plain_base(const OperandDesc &op, int64_t b, const T *ws, int64_t ldw,
+                                               Pools<T> pools, const uint8_t *__restrict__ synT) {
+    if (op.kind == kInter) return ws + op.off * ldw + b;
+    if (op.kind == kConst) return pools.cpool + op.off;
+    uint32_t v = 0;
+    for (int j = 0; j < op.nev; ++j) v |= (uint32_t)synT[(int64_t)op.ev_bit[j] * ldw + b] << j;
+    return pools.spool + op.off + ((int64_t)v << op.rank);
+}
+
+template <typename T, bool AW, bool BW>
+__device__ __forceinline__ void plain_loop(const StepDesc &st, const T *pa, const T *pb, T *pc, int64_t ldw) {
+    const uint32_t nc = 1u << st.rc;
+    const int cexp = st.a.cexp + st.b.cexp;
+    for (uint32_t c = blockIdx.y; c < nc; c += gridDim.y) {
+        T acc = contract_one<T, AW, BW>(st, c, pa, pb, ldw);
+        if (cexp != 0) acc = ldexp(acc, cexp);
+        pc[(int64_t)c * ldw] = acc;
+    }
+}
+
+// One plan step for a chunk of syndromes: thread = (output entry c, syndrome
+// b); c is uniform across the block, so all index arithmetic is scalar and a
+// wavefront reads 64 consecutive syndromes of one entry.
 template <typename T>
 __global__ void __launch_bounds__(256) plain_step_kernel(StepDesc st, T *__restrict__ ws, int64_t ldw, int64_t nb,
                                                          Pools<T> pools, const uint8_t *__restrict__ synT) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    const uint32_t nc = 1u << st.rc;
-    const uint32_t nkk = 1u << st.nk;
-    const int cexp = st.a.cexp + st.b.cexp;
-    for (uint32_t c = blockIdx.y; c < nc; c += gridDim.y) {
-        const uint32_t ar = gather_bits(c, st.a);
-        const uint32_t br = gather_bits(c, st.b);
-        T acc = 0;
-        for (uint32_t k = 0; k < nkk; ++k) {
-            const T va = plain_load_sum(st.a, ar | k, st.nk, b, ws, ldw, pools, synT);
-            const T vb = plain_load_sum(st.b, br | k, st.nk, b, ws, ldw, pools, synT);
-            acc += va * vb;
-        }
-        if (cexp != 0) acc = ldexp(acc, cexp);
-        ws[(st.c_off + (int64_t)c) * ldw + b] = acc;
-    }
+    const T *pa = plain_base(st.a, b, ws, ldw, pools, synT);
+    const T *pb = plain_base(st.b, b, ws, ldw, pools, synT);
+    T *pc = ws + st.c_off * ldw + b;
+    const bool aw = st.a.kind == kInter, bw = st.b.kind == kInter;
+    if (aw && bw) plain_loop<T, true, true>(st, pa, pb, pc, ldw);
+    else if (aw) plain_loop<T, true, false>(st, pa, pb, pc, ldw);
+    else if (bw) plain_loop<T, false, true>(st, pa, pb, pc, ldw);
+    else plain_loop<T, false, false>(st, pa, pb, pc, ldw);
 }
 
 // First maximum (Julia findmax order) and top-2 gap of one syndrome.
@@ -177,40 +197,44 @@ struct FusedArgs {
     int32_t *recheck_count;
 };
 
+// Base pointer of an operand for tile slot g of the fused executor.
 template <typename T>
-__device__ __forceinline__ T fused_load(const OperandDesc &op, uint32_t idx, int g, int glog, const T *work,
-                                        const T *__restrict__ cpool, const T *__restrict__ spool,
-                                        const uint8_t *synl) {
-    switch (op.kind) {
-        case kInter:
-            return work[((op.off + idx) << glog) + g];
-        case kConst:
-            return cpool[op.off + idx];
-        case kSliced: {
-            uint32_t v = 0;
-            for (int j = 0; j < op.nev; ++j) v |= (uint32_t)synl[(op.ev_bit[j] << glog) + g] << j;
-            return spool[op.off + ((int64_t)v << op.rank) + idx];
-        }
-        default:
-            return (T)((idx & 1u) == (uint32_t)synl[(op.ev_bit[0] << glog) + g]);
-    }
+__device__ __forceinline__ const T *fused_base(const OperandDesc &op, int g, int glog, const T *work,
+                                               const T *__restrict__ cpool, const T *__restrict__ spool,
+                                               const uint8_t *synl) {
+    if (op.kind == kInter) return work + (op.off << glog) + g;
+    if (op.kind == kConst) return cpool + op.off;
+    uint32_t v = 0;
+    for (int j = 0; j < op.nev; ++j) v |= (uint32_t)synl[(op.ev_bit[j] << glog) + g] << j;
+    return spool + op.off + ((int64_t)v << op.rank);
 }
 
-template <typename T>
-__device__ __forceinline__ T fused_load_sum(const OperandDesc &op, uint32_t base, int nk, int g, int glog,
-                                            const T *work, const T *__restrict__ cpool,
-                                            const T *__restrict__ spool, const uint8_t *synl) {
-    if (op.nsingle == 0) return fused_load(op, base, g, glog, work, cpool, spool, synl);
-    T s = 0;
-    const uint32_t ns = 1u << op.nsingle;
-    for (uint32_t q = 0; q < ns; ++q) s += fused_load(op, base | (q << nk), g, glog, work, cpool, spool, synl);
-    return s;
+template <typename T, bool SCALE, bool AW, bool BW>
+__device__ __forceinline__ T fused_loop(const StepDesc &st, const T *pa, const T *pb, T *pc, int glog, int c0,
+                                       int cstep, int shift) {
+    const int nc = 1 << st.rc;
+    const int G = 1 << glog;
+    const int cexp = st.a.cexp + st.b.cexp;
+    T lmax = 0;
+    for (int c = c0; c < nc; c += cstep) {
+        T acc = contract_one<T, AW, BW>(st, (uint32_t)c, pa, pb, (int64_t)G);
+        if (SCALE) {
+            acc = ldexp(acc, shift);
+            lmax = acc > lmax ? acc : lmax;
+        } else if (cexp != 0) {
+            acc = ldexp(acc, cexp);
+        }
+        pc[c << glog] = acc;
+    }
+    return lmax;
 }
 
 __device__ __forceinline__ uint32_t max_bits(float v) { return __float_as_uint(v); }
 __device__ __forceinline__ uint32_t max_bits(double v) { return __float_as_uint((float)v); }
 
-// LDS layout: work[ws_entries][G] T | mx[nslots][G] u64 | ex[nslots][G] i32 | syn[nbits][G] u8
+// LDS layout: work[ws_entries][G] T | mx[nslots][G] u64 | ex[nslots][G] i32 |
+//             syn[nbits][G] u8 | lowf[G] i32
+// Thread t serves tile slot g = t mod G for the whole kernel.
 template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -227,13 +251,16 @@ __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
     const T *spool = reinterpret_cast<const T *>(args.spool);
 
     const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
     const int g = tid & (G - 1);
+    const int c0 = tid >> glog;
+    const int cstep = nthr >> glog;
     const int lane = tid & 63;
     const int64_t count = args.list ? (int64_t)(*args.list_count) : args.n;
     const int64_t ntiles = (count + G - 1) >> glog;
 
     if (SCALE) {
-        for (int i = tid; i < args.nslots * G; i += blockDim.x) mx[i] = 0ull;
+        for (int i = tid; i < args.nslots * G; i += nthr) mx[i] = 0ull;
     }
     // An operand whose largest entry is below 2^-100 means its step lost
     // more than 100 binary orders of magnitude: such syndromes are re-checked.
@@ -246,13 +273,13 @@ __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
             const int64_t rows = min((int64_t)G, count - b0);
             const uint8_t *src = args.syn + b0 * args.nbits;
             const int tot = G * args.nbits;
-            for (int i = tid; i < tot; i += blockDim.x) {
+            for (int i = tid; i < tot; i += nthr) {
                 const int r = i / args.nbits, bit = i - r * args.nbits;
                 synl[(bit << glog) + r] = (r < rows) ? (src[i] & 1u) : 0u;
             }
         } else {
             const int tot = G * args.nbits;
-            for (int i = tid; i < tot; i += blockDim.x) {
+            for (int i = tid; i < tot; i += nthr) {
                 const int r = i / args.nbits, bit = i - r * args.nbits;
                 const int64_t q = b0 + r;
                 synl[(bit << glog) + r] =
@@ -284,28 +311,15 @@ __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
                 shift = -(na + nb);
                 if ((st.a.kind == kInter && na < kLowExp) || (st.b.kind == kInter && nb < kLowExp)) lowf[g] = 1;
             }
-            const uint32_t nkk = 1u << st.nk;
-            const int total = (1 << st.rc) << glog;
-            T lmax = 0;
-            for (int w = tid; w < total; w += blockDim.x) {
-                const uint32_t c = (uint32_t)(w >> glog);
-                const uint32_t ar = gather_bits(c, st.a);
-                const uint32_t br = gather_bits(c, st.b);
-                T acc = 0;
-                for (uint32_t k = 0; k < nkk; ++k) {
-                    const T va = fused_load_sum<T>(st.a, ar | k, st.nk, g, glog, work, cpool, spool, synl);
-                    const T vb = fused_load_sum<T>(st.b, br | k, st.nk, g, glog, work, cpool, spool, synl);
-                    acc += va * vb;
-                }
-                if (SCALE) {
-                    acc = ldexp(acc, shift);
-                    lmax = acc > lmax ? acc : lmax;
-                } else {
-                    const int ce = st.a.cexp + st.b.cexp;
-                    if (ce != 0) acc = ldexp(acc, ce);
-                }
-                work[((st.c_off + c) << glog) + g] = acc;
-            }
+            const T *pa = fused_base(st.a, g, glog, work, cpool, spool, synl);
+            const T *pb = fused_base(st.b, g, glog, work, cpool, spool, synl);
+            T *pc = work + (st.c_off << glog) + g;
+            const bool aw = st.a.kind == kInter, bw = st.b.kind == kInter;
+            T lmax;
+            if (aw && bw) lmax = fused_loop<T, SCALE, true, true>(st, pa, pb, pc, glog, c0, cstep, shift);
+            else if (aw) lmax = fused_loop<T, SCALE, true, false>(st, pa, pb, pc, glog, c0, cstep, shift);
+            else if (bw) lmax = fused_loop<T, SCALE, false, true>(st, pa, pb, pc, glog, c0, cstep, shift);
+            else lmax = fused_loop<T, SCALE, false, false>(st, pa, pb, pc, glog, c0, cstep, shift);
             if (SCALE) {
                 for (int o = G; o < 64; o <<= 1) {
                     const float other = __shfl_xor((float)lmax, o);
@@ -325,12 +339,7 @@ __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
             const int64_t q = b0 + tid;
             if (q < count) {
                 const int64_t b = args.list ? (int64_t)args.list[q] : q;
-                int e = 0;
-                if (SCALE) {
-                    const float m = __uint_as_float((uint32_t)(mx[(args.root_slot << glog) + g] & 0xffffffffull));
-                    e = ex[(args.root_slot << glog) + g];
-                    (void)m;
-                }
+                const int e = SCALE ? ex[(args.root_slot << glog) + g] : 0;
                 T best = 0, second = -1;
                 int bi = 0;
                 for (int j = 0; j < args.nout; ++j) {
@@ -423,6 +432,8 @@ struct tqec_plan {
     FusedCfg f64, f32;
     bool cfg_done = false;
     bool timing = false;
+    size_t lds_limit = 0;
+    int n_cu = 0;
 };
 
 namespace {
@@ -466,27 +477,33 @@ size_t fused_lds_bytes(const CompiledPlan &cp, int glog, size_t elem, bool scale
     return b;
 }
 
-FusedCfg choose_fused(const CompiledPlan &cp, size_t elem, bool scale, size_t lds_limit, int n_cu) {
+// Workgroups per CU a configuration can keep resident (LDS and wave slots).
+int resident_per_cu(size_t lds, int threads, size_t lds_limit) {
+    const int by_lds = (int)(lds_limit / std::max<size_t>(lds, 1));
+    const int by_waves = 32 / std::max(1, threads / 64);
+    return std::max(1, std::min(by_lds, by_waves));
+}
+
+FusedCfg make_cfg(const CompiledPlan &cp, int glog, int threads, size_t elem, bool scale, size_t lds_limit,
+                  int n_cu) {
     FusedCfg c;
-    // Largest tile (up to 64 syndromes) whose LDS image leaves room for two
-    // workgroups per CU; fall back to one workgroup per CU.
-    for (int pass = 0; pass < 2 && c.glog < 0; ++pass) {
-        const size_t budget = pass == 0 ? lds_limit / 2 : lds_limit;
+    c.glog = glog;
+    c.threads = threads;
+    c.lds = fused_lds_bytes(cp, glog, elem, scale);
+    c.blocks = n_cu * resident_per_cu(c.lds, threads, lds_limit);
+    return c;
+}
+
+// Default before tuning: the largest tile whose LDS image lets at least four
+// workgroups share a CU (else two, else one), 128 threads.
+FusedCfg choose_fused(const CompiledPlan &cp, size_t elem, bool scale, size_t lds_limit, int n_cu) {
+    for (int div : {4, 2, 1}) {
         for (int gl = 6; gl >= 0; --gl) {
-            size_t b = fused_lds_bytes(cp, gl, elem, scale);
-            if (b <= budget) {
-                c.glog = gl;
-                c.lds = b;
-                break;
-            }
+            if (fused_lds_bytes(cp, gl, elem, scale) <= lds_limit / div)
+                return make_cfg(cp, gl, std::max(64, std::min(128, 1 << gl)), elem, scale, lds_limit, n_cu);
         }
     }
-    if (c.glog >= 0) {
-        c.threads = 256;
-        const int per_cu = std::max<int>(1, (int)(lds_limit / std::max<size_t>(c.lds, 1)));
-        c.blocks = n_cu * std::min(per_cu, 8);
-    }
-    return c;
+    return FusedCfg();
 }
 
 int configure(tqec_plan *p) {
@@ -497,17 +514,15 @@ int configure(tqec_plan *p) {
     TQ_HIP(hipGetDeviceProperties(&prop, dev));
     size_t lds = prop.sharedMemPerBlock;
     if (prop.maxSharedMemoryPerMultiProcessor > 0) lds = std::min(lds, (size_t)prop.maxSharedMemoryPerMultiProcessor);
-    const int ncu = prop.multiProcessorCount;
-    p->f64 = choose_fused(p->cp, sizeof(double), false, lds, ncu);
-    p->f32 = choose_fused(p->cp, sizeof(float), true, lds, ncu);
-    if (p->f64.glog >= 0) {
-        TQ_HIP(hipFuncSetAttribute((const void *)fused_kernel<double, false>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->f64.lds));
-    }
-    if (p->f32.glog >= 0) {
-        TQ_HIP(hipFuncSetAttribute((const void *)fused_kernel<float, true>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->f32.lds));
-    }
+    p->lds_limit = lds;
+    p->n_cu = prop.multiProcessorCount;
+    p->f64 = choose_fused(p->cp, sizeof(double), false, lds, p->n_cu);
+    p->f32 = choose_fused(p->cp, sizeof(float), true, lds, p->n_cu);
+    // Opt both kernels in to the whole LDS once; a launch uses what it asks for.
+    TQ_HIP(hipFuncSetAttribute((const void *)fused_kernel<double, false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    TQ_HIP(hipFuncSetAttribute((const void *)fused_kernel<float, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     p->cfg_done = true;
     return TQEC_OK;
 }
@@ -525,6 +540,11 @@ int grow(T **ptr, size_t *cap, size_t need) {
 
 // Record event i (0/1: main start/stop, 2/3: re-check start/stop) of the
 // current call; event 0 opens a new call slot.
+int record_or(hipError_t e) {
+    if (e != hipSuccess) return set_error(TQEC_EDEVICE, std::string("HIP: ") + hipGetErrorString(e));
+    return TQEC_OK;
+}
+
 int record(tqec_plan *p, Workspace &w, int i, hipStream_t stream) {
     if (!p->timing) return TQEC_OK;
     if (i == 0) {
@@ -747,6 +767,9 @@ int tqec_plan_stats(const tqec_plan *p, tqec_stats *s) {
     s->n_out = cp.n_out;
     s->fused_tile_f64 = p->f64.glog >= 0 && p->cfg_done ? (1 << p->f64.glog) : 0;
     s->fused_tile_f32 = p->f32.glog >= 0 && p->cfg_done ? (1 << p->f32.glog) : 0;
+    s->const_entries = (int64_t)cp.const_pool.size();
+    s->table_entries = (int64_t)cp.sliced_pool.size();
+    s->tabulate_bits = cp.fold_bits;
     return TQEC_OK;
 }
 
@@ -814,6 +837,89 @@ int tqec_last_recheck_count(tqec_plan *p, void *stream, int64_t *count) {
         return TQEC_OK;
     }
     *count = *it->second.host_count;
+    return TQEC_OK;
+}
+
+int tqec_plan_autotune(tqec_plan *p, int32_t mode, const uint8_t *d_syn, int64_t n, void *stream,
+                       int32_t *tile, int32_t *threads) {
+    if (!p || !d_syn) return set_error(TQEC_ENULL, "null argument");
+    if (n <= 0 || n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "trial batch size out of range");
+    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
+        return set_error(TQEC_EINVAL, "only the fused modes have tunable tiles");
+    std::lock_guard<std::mutex> lk(p->mu);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = configure(p))) return rc;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    DeviceData *dd;
+    if ((rc = ensure_device(p, dev, &dd))) return rc;
+    Workspace &w = p->ws[std::make_pair(dev, st)];
+    w.device = dev;
+    const int nout = 1 << p->cp.n_out;
+    const bool f32 = mode == TQEC_MODE_FUSED_F32;
+    FusedCfg &cfg = f32 ? p->f32 : p->f64;
+    if (cfg.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the fused executor");
+    if ((rc = grow(&w.d_mar, &w.d_mar_bytes, (size_t)n * nout * sizeof(double)))) return rc;
+    if ((rc = grow(&w.d_cls, &w.d_cls_bytes, (size_t)n * sizeof(int32_t)))) return rc;
+    if ((rc = grow(&w.d_tie, &w.d_tie_bytes, (size_t)n))) return rc;
+    hipEvent_t e0, e1;
+    TQ_HIP(hipEventCreate(&e0));
+    TQ_HIP(hipEventCreate(&e1));
+    const bool saved_timing = p->timing;
+    p->timing = false;
+    FusedCfg best = cfg;
+    float best_ms = 1e30f;
+    const size_t elem = f32 ? sizeof(float) : sizeof(double);
+    for (int gl = 0; gl <= 6; ++gl) {
+        if (fused_lds_bytes(p->cp, gl, elem, f32) > p->lds_limit) break;
+        for (int thr : {64, 128, 256}) {
+            if (thr < (1 << gl)) continue;
+            cfg = make_cfg(p->cp, gl, thr, elem, f32, p->lds_limit, p->n_cu);
+            float ms = 0;
+            for (int rep = 0; rep < 2 && rc == TQEC_OK; ++rep) {
+                if ((rc = record_or(hipEventRecord(e0, st)))) break;
+                rc = run_fused(p, dd, w, d_syn, n, mode, w.d_mar, w.d_cls, w.d_tie, st);
+                if (rc) break;
+                if ((rc = record_or(hipEventRecord(e1, st)))) break;
+                if ((rc = record_or(hipEventSynchronize(e1)))) break;
+                if ((rc = record_or(hipEventElapsedTime(&ms, e0, e1)))) break;
+            }
+            if (rc) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cfg;
+            }
+        }
+        if (rc) break;
+    }
+    cfg = best;
+    p->timing = saved_timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (tile) *tile = 1 << best.glog;
+    if (threads) *threads = best.threads;
+    return TQEC_OK;
+}
+
+int tqec_plan_set_tile(tqec_plan *p, int32_t mode, int32_t tile, int32_t threads) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
+        return set_error(TQEC_EINVAL, "only the fused modes have tunable tiles");
+    int gl = 0;
+    while (gl <= 6 && (1 << gl) != tile) ++gl;
+    if (gl > 6) return set_error(TQEC_EINVAL, "tile must be a power of two in [1, 64]");
+    if (threads != 64 && threads != 128 && threads != 256) return set_error(TQEC_EINVAL, "threads must be 64, 128 or 256");
+    if (threads < tile) return set_error(TQEC_EINVAL, "threads must be at least the tile");
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc;
+    if ((rc = configure(p))) return rc;
+    const bool f32 = mode == TQEC_MODE_FUSED_F32;
+    const size_t elem = f32 ? sizeof(float) : sizeof(double);
+    if (fused_lds_bytes(p->cp, gl, elem, f32) > p->lds_limit)
+        return set_error(TQEC_EUNSUPPORTED, "tile does not fit in LDS");
+    (f32 ? p->f32 : p->f64) = make_cfg(p->cp, gl, threads, elem, f32, p->lds_limit, p->n_cu);
     return TQEC_OK;
 }
 

@@ -91,6 +91,7 @@ class TNMMAP:
     factorize: bool = True
     mode: str = "fused_f32"
     evidence: str = "slices"
+    tabulate_bits: Optional[int] = None   # None: 0 for plain_f64, 4 otherwise
 
 
 @dataclass
@@ -180,7 +181,10 @@ def compile(decoder, tanner, em=None):
     lx, lz = logical_operator(tanner)
     graph = tnmmap_css_network(tanner, em, factorize=decoder.factorize, lxlz=(lx, lz))
     plan = build_plan(graph, decoder)
-    hip = _lib.HipPlan(plan)
+    tab = decoder.tabulate_bits
+    if tab is None:
+        tab = 0 if decoder.mode == "plain_f64" else 4
+    hip = _lib.HipPlan(plan, tabulate_bits=tab)
     n = tanner.nq
     ct = CompiledTNMMAP(tanner, lx, lz, graph, plan, hip, decoder.mode,
                         list(range(2 * n, 2 * n + tanner.ns)))

@@ -50,6 +50,7 @@ class PlanDesc(ctypes.Structure):
         ("n_sliced", ctypes.c_int32),
         ("sliced_label", _i32p),
         ("sliced_bit", _i32p),
+        ("tabulate_bits", ctypes.c_int32),
     ]
 
 
@@ -67,6 +68,9 @@ class Stats(ctypes.Structure):
         ("n_out", ctypes.c_int32),
         ("fused_tile_f64", ctypes.c_int32),
         ("fused_tile_f32", ctypes.c_int32),
+        ("const_entries", ctypes.c_int64),
+        ("table_entries", ctypes.c_int64),
+        ("tabulate_bits", ctypes.c_int32),
     ]
 
 
@@ -90,6 +94,9 @@ EXPORTS = [
       ctypes.c_void_p, ctypes.c_void_p]),
     ("tqec_plan_set_tolerances", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     ("tqec_last_recheck_count", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    ("tqec_plan_autotune", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, _i32p, _i32p]),
+    ("tqec_plan_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     ("tqec_plan_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("tqec_kernel_time_totals", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -137,7 +144,7 @@ def _ptr(a: np.ndarray, t):
 class HipPlan:
     """Owner of a ``tqec_plan*`` built from a :class:`planner.Plan`."""
 
-    def __init__(self, plan):
+    def __init__(self, plan, tabulate_bits: int = 4):
         lib = load()
         L = plan.n_leaves
         ranks = _i32([len(l) for l in plan.leaf_labels])
@@ -160,7 +167,7 @@ class HipPlan:
         desc = PlanDesc(nvars, L, _ptr(ranks, _i32p), _ptr(labels, _i32p), _ptr(data, _f64p), _ptr(ev, _i32p),
                         plan.n_steps, _ptr(lhs, _i32p), _ptr(rhs, _i32p), _ptr(orank, _i32p), _ptr(olab, _i32p),
                         len(plan.out_labels_final), _ptr(out, _i32p), plan.n_syndrome_bits, len(sl),
-                        _ptr(sl_l, _i32p), _ptr(sl_b, _i32p))
+                        _ptr(sl_l, _i32p), _ptr(sl_b, _i32p), int(tabulate_bits))
         h = ctypes.c_void_p()
         check(lib.tqec_plan_create(ctypes.byref(desc), ctypes.byref(h)))
         self._h = h
@@ -214,6 +221,20 @@ class HipPlan:
                                               ctypes.c_void_p(d_cls) if d_cls else None,
                                               ctypes.c_void_p(d_tie) if d_tie else None,
                                               ctypes.c_void_p(stream) if stream else None))
+
+    def autotune(self, mode, d_syn: int, n: int, stream: int = 0):
+        """Pick the fastest fused tile configuration on a device-resident
+        trial batch; returns (syndromes per workgroup, threads)."""
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        tile, thr = ctypes.c_int32(), ctypes.c_int32()
+        check(load().tqec_plan_autotune(self._h, m, ctypes.c_void_p(d_syn), int(n),
+                                        ctypes.c_void_p(stream) if stream else None,
+                                        ctypes.byref(tile), ctypes.byref(thr)))
+        return tile.value, thr.value
+
+    def set_tile(self, mode, tile: int, threads: int) -> None:
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        check(load().tqec_plan_set_tile(self._h, m, int(tile), int(threads)))
 
     def set_timing(self, enable: bool) -> None:
         check(load().tqec_plan_set_timing(self._h, 1 if enable else 0))

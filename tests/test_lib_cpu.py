@@ -62,7 +62,7 @@ def desc_from(plan, **over):
     d = L.PlanDesc(f["n_vars"], f["n_leaves"], P(f["ranks"], L._i32p), P(f["labels"], L._i32p),
                    P(f["data"], L._f64p), P(f["ev"], L._i32p), f["n_steps"], P(f["lhs"], L._i32p),
                    P(f["rhs"], L._i32p), P(f["orank"], L._i32p), P(f["olab"], L._i32p), f["n_out"],
-                   P(f["out"], L._i32p), f["n_syn"], 0, None, None)
+                   P(f["out"], L._i32p), f["n_syn"], 0, None, None, f.get("tab", 0))
     return d, f
 
 
@@ -146,6 +146,6 @@ def test_stats_mirror_plan():
     ct = small_plan()
     s = ct.hip.stats()
     ps = ct.plan.stats()
-    assert s["tc"] == ps["tc"] and s["dep_tc"] == ps["tc_dep"] and s["sc"] == ps["sc"]
+    assert s["tc"] == ps["tc"] and 0 < s["dep_tc"] <= ps["tc_dep"] and s["sc"] == ps["sc"]
     assert s["n_out"] == 2 and s["n_steps"] == ct.plan.n_steps
     assert s["plain_ws_entries"] > 0
