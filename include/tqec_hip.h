@@ -167,6 +167,53 @@ int tqec_plan_set_timing(tqec_plan *plan, int enable);
 int tqec_kernel_time_totals(tqec_plan *plan, void *stream, double *main_ms, double *recheck_ms,
                             int64_t *calls);
 
+/* ---------------------------------------------------------------------------
+ * On-device sampling.  Replaces the per-sample `random_error_pattern(em)` +
+ * `syndrome_extraction` loop (src/decoding/error_model.jl:69-71, 97-117,
+ * 131-146; the harness loop of test/decoding/tndecoder.jl:85-99).  Sample s,
+ * variable j draws u from Philox4x32-10 with counter (j, s mod 2^32, s >> 32,
+ * tag) and key (seed mod 2^32, seed >> 32): bit-identical to the host sampler.
+ *   CSS (tag 0x51D0): Y if u < py, X if u < px+py, Z if u < px+py+pz;
+ *     syndrome row [sx | sz] with sx = Hx ez, sz = Hz ex; class bit i =
+ *     lx_i . ez, bit k+i = lz_i . ex; errors row [ex | ez] (2 nq bytes).
+ *   flips (tag 0xF11B): mechanism j fires if u < p_j; syndrome = D e,
+ *     class bit l = L_l . e; errors row e (nm bytes).
+ * Matrices are dense row-major bytes.  Pointers passed to tqec_sample are
+ * device pointers; d_classes and d_errors may be NULL.
+ * ------------------------------------------------------------------------- */
+typedef struct tqec_sampler tqec_sampler;
+int tqec_sampler_create_css(int32_t nq, const double *px, const double *py, const double *pz, int32_t nsx,
+                            const uint8_t *Hx, int32_t nsz, const uint8_t *Hz, int32_t k, const uint8_t *lx,
+                            const uint8_t *lz, tqec_sampler **out);
+int tqec_sampler_create_flips(int32_t nm, const double *p, int32_t nd, const uint8_t *D, int32_t nl,
+                              const uint8_t *L, tqec_sampler **out);
+int tqec_sample(tqec_sampler *sampler, uint64_t seed, int64_t first, int64_t n, uint8_t *d_syndromes,
+                int32_t *d_classes, uint8_t *d_errors, void *stream);
+void tqec_sampler_destroy(tqec_sampler *sampler);
+
+/* ---------------------------------------------------------------------------
+ * On-device error-pattern recovery.  Replaces `error_pattern`
+ * (src/decoding/tndecoder.jl:167-174 CSS, :255-271 DEM) and its MIP pure
+ * error (src/decoding/ipdecoder.jl:150-169): e = R s with R a GF(2) right
+ * inverse of the check matrix, then the reference's logical fix driven by the
+ * decided class.
+ *   CSS: Rx (nq x nsz) maps sz to ex, Rz (nq x nsx) maps sx to ez; lx/lz are
+ *     k x nq.  ex += lx_i when lz_i . ex differs from class bit k+i; ez +=
+ *     lz_i when lx_i . ez differs from class bit i.  Output ex -> d_err_a,
+ *     ez -> d_err_b (n x nq bytes each).
+ *   DEM: R (nm x nd); l2q (nl x nm) marks the mechanisms touching logical l;
+ *     when their parity differs from class bit l, every one of them is
+ *     flipped (the reference's rule).  Output -> d_err_a (n x nm bytes).
+ * ------------------------------------------------------------------------- */
+typedef struct tqec_recovery tqec_recovery;
+int tqec_recovery_create_css(int32_t nq, int32_t nsx, int32_t nsz, const uint8_t *Rx, const uint8_t *Rz,
+                             int32_t k, const uint8_t *lx, const uint8_t *lz, tqec_recovery **out);
+int tqec_recovery_create_dem(int32_t nm, int32_t nd, const uint8_t *R, int32_t nl, const uint8_t *l2q,
+                             tqec_recovery **out);
+int tqec_recover(tqec_recovery *rec, const uint8_t *d_syndromes, const int32_t *d_classes, int64_t n,
+                 uint8_t *d_err_a, uint8_t *d_err_b, void *stream);
+void tqec_recovery_destroy(tqec_recovery *rec);
+
 /* Thread-local description of the last error. */
 const char *tqec_last_error(void);
 

@@ -378,6 +378,10 @@ __global__ void zero_i32_kernel(int32_t *p, int n) {
 
 }  // namespace
 
+namespace tqec {
+int set_error_shared(int code, const std::string &msg) { return set_error(code, msg); }
+}  // namespace tqec
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------

@@ -34,6 +34,7 @@ class TreeSA:
     sweeps: int = 300
     restarts: int = 4
     sc_target: Optional[int] = None
+    primary: Optional[tuple] = None   # variables of a time-ordered sweep candidate
 
 
 @dataclass(frozen=True)
@@ -57,7 +58,8 @@ def optimize_tree(ixs, iy, optimizer):
     if isinstance(optimizer, GreedyMethod):
         return greedy_tree(ixs, iy, seed=optimizer.seed)
     opt = optimizer if isinstance(optimizer, TreeSA) else TreeSA()
-    base = search_tree(ixs, iy, seed=opt.seed, ntrials=opt.ntrials, sc_target=opt.sc_target)
+    base = search_tree(ixs, iy, seed=opt.seed, ntrials=opt.ntrials, sc_target=opt.sc_target,
+                       primary=opt.primary)
     best, best_key = base, tree_cost(base)[0]
     for r in range(opt.restarts):
         tr = anneal_tree(base, iy, seed=opt.seed * 7919 + r, sweeps=opt.sweeps, sc_target=opt.sc_target)

@@ -101,6 +101,24 @@ EXPORTS = [
     ("tqec_kernel_time_totals", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
       ctypes.POINTER(ctypes.c_int64)]),
+    ("tqec_sampler_create_css", ctypes.c_int,
+     [ctypes.c_int32, _f64p, _f64p, _f64p, ctypes.c_int32, _u8p, ctypes.c_int32, _u8p, ctypes.c_int32, _u8p, _u8p,
+      ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_sampler_create_flips", ctypes.c_int,
+     [ctypes.c_int32, _f64p, ctypes.c_int32, _u8p, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_sample", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p]),
+    ("tqec_sampler_destroy", None, [ctypes.c_void_p]),
+    ("tqec_recovery_create_css", ctypes.c_int,
+     [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _u8p, _u8p, ctypes.c_int32, _u8p, _u8p,
+      ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_recovery_create_dem", ctypes.c_int,
+     [ctypes.c_int32, ctypes.c_int32, _u8p, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_recover", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p]),
+    ("tqec_recovery_destroy", None, [ctypes.c_void_p]),
     ("tqec_last_error", ctypes.c_char_p, []),
 ]
 
@@ -251,3 +269,98 @@ class HipPlan:
         c = ctypes.c_int64()
         check(load().tqec_last_recheck_count(self._h, ctypes.c_void_p(stream) if stream else None, ctypes.byref(c)))
         return c.value
+
+
+def _u8(a) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+    return a if a.size else np.zeros(1, np.uint8)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _vp(x):
+    return ctypes.c_void_p(x) if x else None
+
+
+class HipSampler:
+    """Device sampler (``tqec_sampler``): bit-identical to
+    :func:`tensorqec.errors.sample_depolarizing` / :func:`sample_flips`."""
+
+    def __init__(self, handle, nsyn, nvar, kind):
+        self._h, self.nsyn, self.nvar, self.kind = handle, nsyn, nvar, kind
+
+    @classmethod
+    def css(cls, tanner, em, lx, lz):
+        lib = load()
+        h = ctypes.c_void_p()
+        px, py, pz = _f64(em.px), _f64(em.py), _f64(em.pz)
+        Hx, Hz, LX, LZ = _u8(tanner.stgx.H), _u8(tanner.stgz.H), _u8(lx), _u8(lz)
+        check(lib.tqec_sampler_create_css(tanner.nq, _ptr(px, _f64p), _ptr(py, _f64p), _ptr(pz, _f64p),
+                                          tanner.stgx.ns, _ptr(Hx, _u8p), tanner.stgz.ns, _ptr(Hz, _u8p),
+                                          int(lx.shape[0]), _ptr(LX, _u8p), _ptr(LZ, _u8p), ctypes.byref(h)))
+        return cls(h, tanner.ns, tanner.nq, "css")
+
+    @classmethod
+    def flips(cls, p, D, Lm):
+        lib = load()
+        h = ctypes.c_void_p()
+        p, D, Lm = _f64(p), _u8(D), _u8(Lm)
+        nd = D.shape[0] if D.ndim == 2 else 0
+        nl = Lm.shape[0] if Lm.ndim == 2 else 0
+        check(lib.tqec_sampler_create_flips(len(p), _ptr(p, _f64p), nd, _ptr(D, _u8p), nl, _ptr(Lm, _u8p),
+                                            ctypes.byref(h)))
+        return cls(h, nd, len(p), "flips")
+
+    def sample(self, seed: int, first: int, n: int, d_syn: int, d_cls: int = 0, d_err: int = 0, stream: int = 0):
+        check(load().tqec_sample(self._h, int(seed), int(first), int(n), _vp(d_syn), _vp(d_cls), _vp(d_err),
+                                 _vp(stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().tqec_sampler_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class HipRecovery:
+    """Device error-pattern recovery (``tqec_recovery``)."""
+
+    def __init__(self, handle, nvar, kind):
+        self._h, self.nvar, self.kind = handle, nvar, kind
+
+    @classmethod
+    def css(cls, tanner, Rx, Rz, lx, lz):
+        lib = load()
+        h = ctypes.c_void_p()
+        Rx, Rz, LX, LZ = _u8(Rx), _u8(Rz), _u8(lx), _u8(lz)
+        check(lib.tqec_recovery_create_css(tanner.nq, tanner.stgx.ns, tanner.stgz.ns, _ptr(Rx, _u8p),
+                                           _ptr(Rz, _u8p), int(lx.shape[0]), _ptr(LX, _u8p), _ptr(LZ, _u8p),
+                                           ctypes.byref(h)))
+        return cls(h, tanner.nq, "css")
+
+    @classmethod
+    def dem(cls, R, l2q_mask):
+        lib = load()
+        h = ctypes.c_void_p()
+        R, M = _u8(R), _u8(l2q_mask)
+        nl = M.shape[0] if M.ndim == 2 else 0
+        check(lib.tqec_recovery_create_dem(R.shape[0], R.shape[1], _ptr(R, _u8p), nl, _ptr(M, _u8p),
+                                           ctypes.byref(h)))
+        return cls(h, R.shape[0], "dem")
+
+    def recover(self, d_syn: int, d_cls: int, n: int, d_a: int, d_b: int = 0, stream: int = 0):
+        check(load().tqec_recover(self._h, _vp(d_syn), _vp(d_cls), int(n), _vp(d_a), _vp(d_b), _vp(stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().tqec_recovery_destroy(h)
+            except Exception:
+                pass
+            self._h = None

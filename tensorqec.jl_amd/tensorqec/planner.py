@@ -384,14 +384,44 @@ def anneal_tree(tree: ContractionTree, iy: Sequence[int], seed: int = 0, sweeps:
     return ContractionTree(L, children, masks)
 
 
+def sweep_order(ixs: Sequence[Sequence[int]], iy: Sequence[int], primary: Sequence[int]) -> List[int]:
+    """Eliminate the ``primary`` variables in the given order; every other
+    variable is eliminated as soon as all primary variables of the tensors
+    holding it are gone.  For a detector error model with mechanisms sorted
+    by their detectors (i.e. by time) this is a sweep through the circuit
+    whose state is one partial-parity bit per open detector chain."""
+    holders: Dict[int, List[int]] = {}
+    for t, ix in enumerate(ixs):
+        for l in ix:
+            holders.setdefault(l, []).append(t)
+    prim = list(primary)
+    pset = set(prim)
+    out = set(iy)
+    rest = [l for l in holders if l not in pset and l not in out]
+    need = {a: set(m for t in holders[a] for m in ixs[t] if m in pset) for a in rest}
+    waiting: Dict[int, List[int]] = {}
+    pos = {m: i for i, m in enumerate(prim)}
+    for a, ms in need.items():
+        last = max((pos[m] for m in ms), default=-1)
+        waiting.setdefault(last, []).append(a)
+    order = list(sorted(waiting.get(-1, [])))
+    for i, m in enumerate(prim):
+        order.append(m)
+        order.extend(sorted(waiting.get(i, [])))
+    return order
+
+
 def search_tree(ixs: Sequence[Sequence[int]], iy: Sequence[int], seed: int = 0, ntrials: int = 8,
                 sa_sweeps: int = 0, sc_target: Optional[int] = None,
-                objective: str = "tc") -> ContractionTree:
+                objective: str = "tc", primary: Optional[Sequence[int]] = None,
+                greedy_limit: int = 1200) -> ContractionTree:
     """Best of several seeded greedy and min-fill elimination trees,
     optionally refined by :func:`anneal_tree`.  ``objective`` ranks trees by
     multiply-adds ("tc") or element traffic ("rw")."""
     cands: List[ContractionTree] = []
-    for t in range(ntrials):
+    if primary is not None:
+        cands.append(elimination_tree(ixs, iy, sweep_order(ixs, iy, primary)))
+    for t in range(ntrials if len(ixs) <= greedy_limit else 0):
         cands.append(greedy_tree(ixs, iy, seed=seed * 1000 + t, alpha=0.0, temperature=0.0 if t == 0 else 0.3))
         cands.append(greedy_tree(ixs, iy, seed=seed * 1000 + t, alpha=1.0, temperature=0.0 if t == 0 else 0.3))
         cands.append(elimination_tree(ixs, iy, min_fill_order(ixs, iy, seed=seed * 1000 + t)))
