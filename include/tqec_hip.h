@@ -51,6 +51,9 @@ extern "C" {
                                  intermediates in HBM                     */
 #define TQEC_MODE_FUSED_F64 1 /* whole plan per workgroup tile, FP64,
                                  intermediates in LDS                     */
+#define TQEC_SEMIRING_SUM 0   /* sum-product: marginals (TNMMAP)            */
+#define TQEC_SEMIRING_MAX 1   /* max-product with arg-max backtrace (TNMAP)  */
+
 #define TQEC_MODE_FUSED_F32 2 /* fused, FP32 with per-tensor power-of-two
                                  scaling; syndromes whose top-2 class
                                  margin is below the re-check threshold are
@@ -88,6 +91,9 @@ typedef struct tqec_plan_desc {
                                        this many syndrome bits are also
                                        contracted there, once per bit
                                        pattern, and looked up per syndrome */
+    int32_t semiring;               /* TQEC_SEMIRING_SUM or _MAX; a max-
+                                       product plan must have n_out = 0 and
+                                       is decoded by tqec_map_decode_batch */
 } tqec_plan_desc;
 
 typedef struct tqec_plan tqec_plan;
@@ -137,6 +143,23 @@ int tqec_decode_batch(tqec_plan *plan, const uint8_t *syndromes, int64_t n, int3
 int tqec_decode_batch_device(tqec_plan *plan, const uint8_t *d_syndromes, int64_t n, int32_t mode,
                              double *d_marginals, int32_t *d_classes, uint8_t *d_tie_flags,
                              void *stream);
+
+/*
+ * Most probable configuration (TNMAP) for a max-product plan: replaces
+ * TensorInference's `most_probable_config` after `update_evidence!`
+ * (src/decoding/tndecoder.jl:53-57).  The contraction keeps, per output
+ * entry, the maximum over the labels each step eliminates (first maximum in
+ * the order: contracted labels, then single-side labels of A, then of B) and
+ * records it; a per-syndrome backtrace from the scalar root assigns every
+ * label.  Writes the values of labels 0..n_config-1 (n x n_config bytes) and,
+ * when logp is not NULL, the natural log of the maximum (n doubles).
+ * Host pointers for tqec_map_decode_batch (synchronises `stream`), device
+ * pointers for the _device variant (enqueues only).
+ */
+int tqec_map_decode_batch(tqec_plan *plan, const uint8_t *syndromes, int64_t n, int32_t n_config,
+                          uint8_t *config, double *logp, void *stream);
+int tqec_map_decode_batch_device(tqec_plan *plan, const uint8_t *d_syndromes, int64_t n, int32_t n_config,
+                                 uint8_t *d_config, double *d_logp, void *stream);
 
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);

@@ -331,3 +331,36 @@ def dem_marginals(rates, flipped, detector_list, logical_list, syndromes, factor
     t = contract(ixs, ts, ["b"] + iy)
     nout = len(iy)
     return np.transpose(t, [0] + list(range(nout, 0, -1))).reshape(B, 1 << nout)
+
+
+def brute_force_map_css(Hx, Hz, px, py, pz):
+    """Exact maximum of P(ex, ez) over error patterns with a given syndrome
+    (the quantity TNMAP maximises, tndecoder.jl:53-57 on the network of
+    stg2uaimodel, :33-40): {syndrome tuple: max probability}."""
+    Hx, Hz = np.asarray(Hx, dtype=np.int64), np.asarray(Hz, dtype=np.int64)
+    n = Hx.shape[1]
+    if n > 10:
+        raise ValueError("brute force limited to n <= 10")
+    paulis = np.array(list(itertools.product(range(4), repeat=n)), dtype=np.int64)
+    ex = ((paulis == 1) | (paulis == 2)).astype(np.int64)
+    ez = ((paulis == 2) | (paulis == 3)).astype(np.int64)
+    P = np.stack([1 - np.asarray(px) - np.asarray(py) - np.asarray(pz), px, py, pz], axis=1)
+    probs = np.ones(len(paulis))
+    for q in range(n):
+        probs *= P[q][paulis[:, q]]
+    keys = np.concatenate([(ez @ Hx.T) & 1, (ex @ Hz.T) & 1], axis=1)
+    out = {}
+    for key, pr in zip(map(tuple, keys), probs):
+        if pr > out.get(key, -1.0):
+            out[key] = pr
+    return out
+
+
+def css_config_probability(ex, ez, px, py, pz):
+    """P(ex, ez) under independent depolarizing noise (rows are patterns)."""
+    ex = np.atleast_2d(ex).astype(bool)
+    ez = np.atleast_2d(ez).astype(bool)
+    px, py, pz = (np.asarray(a, dtype=np.float64) for a in (px, py, pz))
+    p_i = 1 - px - py - pz
+    val = np.where(ex & ez, py, np.where(ex, px, np.where(ez, pz, p_i)))
+    return np.prod(val, axis=1)

@@ -151,6 +151,12 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
     if (d->tabulate_bits < 0 || d->tabulate_bits > kMaxEv) { err = fmt("tabulate_bits must be in [0, %ld]", kMaxEv); return TQEC_EINVAL; }
     p.n_vars = NV; p.n_leaves = NL; p.n_steps = NS; p.n_out = d->n_out; p.n_syn = d->n_syndrome_bits;
     p.fold_limit = d->tabulate_bits;
+    if (d->semiring != TQEC_SEMIRING_SUM && d->semiring != TQEC_SEMIRING_MAX) { err = "unknown semiring"; return TQEC_EINVAL; }
+    p.semiring = d->semiring;
+    if (p.semiring == TQEC_SEMIRING_MAX) {
+        p.fold_limit = 0;
+        if (d->n_out != 0) { err = "a max-product plan must contract to a scalar (n_out = 0)"; return TQEC_EINVAL; }
+    }
     p.slice_bit_of_label.assign(NV, -1);
     for (int i = 0; i < d->n_sliced; ++i) {
         int l = d->sliced_label[i], b = d->sliced_bit[i];
@@ -289,6 +295,20 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
     p.fold_bits = std::min<int>(p.fold_limit, (int)p.bits[root].size() - 1);
     p.dep.assign(NL + NS, 0);
     for (int x = NL; x < NL + NS; ++x) p.dep[x] = (int)p.bits[x].size() > p.fold_bits;
+    if (p.semiring == TQEC_SEMIRING_MAX) {
+        // The backtrace needs the arg-max of every eliminated label, so a step
+        // that eliminates one (or has such a step below it) runs on the device
+        // even when it does not depend on the syndrome.
+        std::vector<char> elim(NL + NS, 0);
+        for (int i = 0; i < NS; ++i) {
+            const int node = NL + i, a = p.lhs[i], b = p.rhs[i];
+            bool e = elim[a] || elim[b];
+            for (int l : p.labels[a]) e = e || !contains(p.labels[node], l);
+            for (int l : p.labels[b]) e = e || !contains(p.labels[node], l);
+            elim[node] = e;
+            p.dep[node] = p.dep[node] || e;
+        }
+    }
 
     // ---- statistics, constant folding and tabulation ----------------------
     p.value.resize(NL + NS);
@@ -376,6 +396,7 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
     p.mem.assign(NL + NS, {});
     p.mem[root] = p.out_labels;
     std::vector<StepDesc> rev;  // built in reverse order
+    std::vector<MapStep> rev_map;
     for (int i = NS - 1; i >= 0; --i) {
         const int node = NL + i;
         if (!p.dep[node]) continue;
@@ -399,13 +420,16 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
             op.slot = -1;
             const bool inter = p.dep[x];
             // memory order: kept labels in C's order (low bits), then labels
-            // summed on this side only, then the contracted labels (high bits)
+            // summed on this side only, then the contracted labels (high bits).
+            // Sum-product sums the single-side labels of host data here;
+            // max-product keeps them (their arg-max is needed on the device).
+            const bool keep_single = inter || p.semiring == TQEC_SEMIRING_MAX;
             std::vector<int> order = R;
-            if (inter) order.insert(order.end(), S.begin(), S.end());
+            if (keep_single) order.insert(order.end(), S.begin(), S.end());
             order.insert(order.end(), K.begin(), K.end());
             p.mem[x] = order;
             op.rank = (int)order.size();
-            op.nsingle = inter ? (int)S.size() : 0;
+            op.nsingle = keep_single ? (int)S.size() : 0;
             op.sshift = (int)R.size();
             op.kshift = (int)R.size() + op.nsingle;
             int dst = 0;
@@ -452,12 +476,39 @@ int compile_plan(const tqec_plan_desc *d, CompiledPlan &p, std::string &err) {
         int eb = build(b, SB, BR, st.b);
         if (ea == -1 || eb == -1) { err = fmt("step %ld: too many index segments", i); return TQEC_EUNSUPPORTED; }
         rev.push_back(st);
+        // labels of the backtrace: C in memory order, then K, SA, SB
+        MapStep ms;
+        ms.off = (int32_t)p.map_labels.size();
+        ms.nc = (int32_t)C.size();
+        ms.nk = (int32_t)K.size();
+        ms.nsa = st.a.nsingle;
+        ms.nsb = st.b.nsingle;
+        for (int l : C) p.map_labels.push_back(l);
+        for (int l : K) p.map_labels.push_back(l);
+        if (st.a.nsingle) for (int l : SA) p.map_labels.push_back(l);
+        if (st.b.nsingle) for (int l : SB) p.map_labels.push_back(l);
+        rev_map.push_back(ms);
     }
     // ---- memory plan (execution order) ------------------------------------
     std::vector<int> dep_steps;
     for (int i = 0; i < NS; ++i) if (p.dep[NL + i]) dep_steps.push_back(i);
     std::reverse(rev.begin(), rev.end());
     p.steps = rev;
+    std::reverse(rev_map.begin(), rev_map.end());
+    p.map_steps = rev_map;
+    if (p.semiring == TQEC_SEMIRING_MAX) {
+        int64_t ao = 0;
+        for (size_t j = 0; j < p.map_steps.size(); ++j) {
+            MapStep &ms = p.map_steps[j];
+            if (ms.nk + ms.nsa + ms.nsb > 16) {
+                err = "a max-product step eliminates more than 16 labels at once";
+                return TQEC_EUNSUPPORTED;
+            }
+            ms.arg_off = ao;
+            ao += int64_t(1) << ms.nc;
+        }
+        p.arg_entries = ao;
+    }
     Allocator al;
     SlotAllocator sa;
     std::vector<int64_t> off(NL + NS, -1);

@@ -11,6 +11,15 @@
 
 namespace tqec {
 
+// Backtrace record of one device step of a max-product plan: the labels of
+// its output (memory order), then the contracted and single-side labels in
+// the bit order of the step's arg-max record.
+struct MapStep {
+    int32_t off = 0;      // into CompiledPlan::map_labels
+    int32_t nc = 0, nk = 0, nsa = 0, nsb = 0;
+    int64_t arg_off = 0;  // arg-max table offset per syndrome (entries)
+};
+
 struct CompiledPlan {
     // ---- descriptor copy -------------------------------------------------
     int n_vars = 0, n_leaves = 0, n_steps = 0, n_out = 0, n_syn = 0;
@@ -31,6 +40,10 @@ struct CompiledPlan {
     std::vector<std::vector<double>> value;     // constant nodes (labels order)
     std::vector<std::vector<int>> mem;          // memory order of operands
 
+    int semiring = TQEC_SEMIRING_SUM;
+    std::vector<MapStep> map_steps;             // max-product backtrace, execution order
+    std::vector<int32_t> map_labels;
+    int64_t arg_entries = 0;                    // arg-max records per syndrome
     std::vector<StepDesc> steps;                // dependent steps, execution order
     std::vector<double> const_pool;             // kConst data (mem order, scaled)
     std::vector<double> sliced_pool;            // kSliced variants

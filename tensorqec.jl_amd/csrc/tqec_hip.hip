@@ -371,6 +371,121 @@ __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Max-product executor (TNMAP)
+// ---------------------------------------------------------------------------
+
+struct MapArgs {
+    const StepDesc *steps;
+    const MapStep *msteps;
+    const int32_t *labels;
+    int nsteps;
+    const double *cpool;
+    const double *spool;
+    const uint8_t *syn;
+    int64_t n;
+    int nbits;
+    int glog;
+    int64_t ws_entries;
+    int64_t arg_entries;
+    int asg_words;
+    int64_t root_off;
+    int n_config;
+    uint8_t *config;   // [n][n_config]
+    double *logp;      // [n] natural log of the maximum, or null
+};
+
+// Element of a max-product operand; single-side labels are enumerated by the
+// caller (they are eliminated here and their arg-max is recorded).
+template <bool AW>
+__device__ __forceinline__ double map_load(const double *base, uint32_t idx, int glog) {
+    return AW ? base[idx << glog] : base[idx];
+}
+
+// LDS: work[ws][G] f64 | arg[arg_entries][G] u16 | syn[nbits][G] u8 | asg[G][words] u32
+// Forward: each output keeps the maximum over its contracted and single-side
+// labels (first maximum in the order k, then sa, then sb) and records which.
+// Backtrace (one thread per syndrome): from the scalar root down, each step's
+// record fixes the labels it eliminated; the error bits are read at the end.
+__global__ void __launch_bounds__(256) map_kernel(MapArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int glog = a.glog, G = 1 << glog;
+    double *work = reinterpret_cast<double *>(smem);
+    size_t off = (size_t)a.ws_entries * G * sizeof(double);
+    uint16_t *arg = reinterpret_cast<uint16_t *>(smem + off);
+    off += (((size_t)a.arg_entries * G * sizeof(uint16_t)) + 15) & ~size_t(15);
+    uint8_t *synl = smem + off;
+    off += (((size_t)a.nbits * G) + 15) & ~size_t(15);
+    uint32_t *asg = reinterpret_cast<uint32_t *>(smem + off);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int g = tid & (G - 1), c0 = tid >> glog, cstep = nthr >> glog;
+    const int64_t ntiles = (a.n + G - 1) >> glog;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t b0 = t << glog;
+        const int64_t rows = min((int64_t)G, a.n - b0);
+        const uint8_t *src = a.syn + b0 * a.nbits;
+        for (int i = tid; i < G * a.nbits; i += nthr) {
+            const int r = i / a.nbits, bit = i - r * a.nbits;
+            synl[(bit << glog) + r] = (r < rows) ? (src[i] & 1u) : 0u;
+        }
+        for (int i = tid; i < G * a.asg_words; i += nthr) asg[i] = 0u;
+        __syncthreads();
+        for (int s = 0; s < a.nsteps; ++s) {
+            const StepDesc &st = a.steps[s];
+            const MapStep &ms = a.msteps[s];
+            const double *pa = fused_base(st.a, g, glog, work, a.cpool, a.spool, synl);
+            const double *pb = fused_base(st.b, g, glog, work, a.cpool, a.spool, synl);
+            const bool aw = st.a.kind == kInter, bw = st.b.kind == kInter;
+            const int cexp = st.a.cexp + st.b.cexp;
+            const uint32_t nkk = 1u << st.nk, nsa = 1u << st.a.nsingle, nsb = 1u << st.b.nsingle;
+            for (int c = c0; c < (1 << st.rc); c += cstep) {
+                const uint32_t ar = gather_bits((uint32_t)c, st.a);
+                const uint32_t br = gather_bits((uint32_t)c, st.b);
+                double best = -1.0;
+                uint32_t bi = 0;
+                for (uint32_t k = 0; k < nkk; ++k) {
+                    for (uint32_t qa = 0; qa < nsa; ++qa) {
+                        const uint32_t ia = ar | (qa << st.a.sshift) | (k << st.a.kshift);
+                        const double va = aw ? map_load<true>(pa, ia, glog) : map_load<false>(pa, ia, glog);
+                        for (uint32_t qb = 0; qb < nsb; ++qb) {
+                            const uint32_t ib = br | (qb << st.b.sshift) | (k << st.b.kshift);
+                            const double vb = bw ? map_load<true>(pb, ib, glog) : map_load<false>(pb, ib, glog);
+                            const double v = va * vb;
+                            if (v > best) {
+                                best = v;
+                                bi = k | (qa << st.nk) | (qb << (st.nk + st.a.nsingle));
+                            }
+                        }
+                    }
+                }
+                if (cexp != 0) best = ldexp(best, cexp);
+                work[((st.c_off + c) << glog) + g] = best;
+                arg[((ms.arg_off + c) << glog) + g] = (uint16_t)bi;
+            }
+            __syncthreads();
+        }
+        if (tid < G && b0 + tid < a.n) {
+            uint32_t *as = asg + tid * a.asg_words;
+            for (int s = a.nsteps - 1; s >= 0; --s) {
+                const MapStep &ms = a.msteps[s];
+                const int32_t *L = a.labels + ms.off;
+                uint32_t c = 0;
+                for (int j = 0; j < ms.nc; ++j) c |= ((as[L[j] >> 5] >> (L[j] & 31)) & 1u) << j;
+                const uint32_t rec = arg[((ms.arg_off + c) << glog) + tid];
+                const int ne = ms.nk + ms.nsa + ms.nsb;
+                for (int j = 0; j < ne; ++j) {
+                    const int l = L[ms.nc + j];
+                    as[l >> 5] |= ((rec >> j) & 1u) << (l & 31);
+                }
+            }
+            const int64_t b = b0 + tid;
+            for (int v = 0; v < a.n_config; ++v) a.config[b * a.n_config + v] = (uint8_t)((as[v >> 5] >> (v & 31)) & 1u);
+            if (a.logp) a.logp[b] = log(work[(a.root_off << glog) + tid]);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void zero_i32_kernel(int32_t *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -389,6 +504,8 @@ int set_error_shared(int code, const std::string &msg) { return set_error(code, 
 namespace {
 
 struct DeviceData {
+    MapStep *msteps = nullptr;
+    int32_t *mlabels = nullptr;
     StepDesc *steps = nullptr;
     double *cpool64 = nullptr, *spool64 = nullptr;
     float *cpool32 = nullptr, *spool32 = nullptr;
@@ -466,6 +583,12 @@ int ensure_device(tqec_plan *p, int device, DeviceData **out) {
     if ((rc = upload<double>(cp.sliced_pool, &dd.spool64))) return rc;
     if ((rc = upload<float>(cp.const_pool, &dd.cpool32))) return rc;
     if ((rc = upload<float>(cp.sliced_pool, &dd.spool32))) return rc;
+    if (cp.semiring == TQEC_SEMIRING_MAX) {
+        TQ_HIP(hipMalloc((void **)&dd.msteps, std::max<size_t>(1, cp.map_steps.size()) * sizeof(MapStep)));
+        TQ_HIP(hipMemcpy(dd.msteps, cp.map_steps.data(), cp.map_steps.size() * sizeof(MapStep), hipMemcpyHostToDevice));
+        TQ_HIP(hipMalloc((void **)&dd.mlabels, std::max<size_t>(1, cp.map_labels.size()) * sizeof(int32_t)));
+        TQ_HIP(hipMemcpy(dd.mlabels, cp.map_labels.data(), cp.map_labels.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     p->dev[device] = dd;
     *out = &p->dev[device];
     return TQEC_OK;
@@ -747,6 +870,8 @@ void tqec_plan_destroy(tqec_plan *p) {
         if (hipSetDevice(kv.first) != hipSuccess) continue;
         DeviceData &d = kv.second;
         (void)hipFree(d.steps);
+        if (d.msteps) (void)hipFree(d.msteps);
+        if (d.mlabels) (void)hipFree(d.mlabels);
         (void)hipFree(d.cpool64);
         (void)hipFree(d.spool64);
         (void)hipFree(d.cpool32);
@@ -793,6 +918,8 @@ int tqec_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int3
     if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
     if (n > 0 && !d_syn) return set_error(TQEC_ENULL, "null syndrome pointer");
     if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
+    if (p->cp.semiring != TQEC_SEMIRING_SUM)
+        return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);
     return decode_device(p, d_syn, n, mode, d_mar, d_cls, d_tie, (hipStream_t)stream, nullptr);
 }
@@ -804,6 +931,8 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (n > 0 && !syn) return set_error(TQEC_ENULL, "null syndrome pointer");
     if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
     if (mode < 0 || mode > 2) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+    if (p->cp.semiring != TQEC_SEMIRING_SUM)
+        return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);
     hipStream_t st = (hipStream_t)stream;
     int dev;
@@ -905,6 +1034,102 @@ int tqec_plan_autotune(tqec_plan *p, int32_t mode, const uint8_t *d_syn, int64_t
     if (tile) *tile = 1 << best.glog;
     if (threads) *threads = best.threads;
     return TQEC_OK;
+}
+
+int tqec_map_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int32_t n_config,
+                                 uint8_t *d_config, double *d_logp, void *stream) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (p->cp.semiring != TQEC_SEMIRING_MAX)
+        return set_error(TQEC_EINVAL, "tqec_map_decode_batch needs a max-product plan");
+    if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
+    if (n > 0 && (!d_syn || (n_config > 0 && !d_config))) return set_error(TQEC_ENULL, "null pointer");
+    if (n_config < 0 || n_config > p->cp.n_vars) return set_error(TQEC_EINVAL, "n_config out of range");
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc;
+    if ((rc = configure(p))) return rc;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    DeviceData *dd;
+    if ((rc = ensure_device(p, dev, &dd))) return rc;
+    if (n == 0) return TQEC_OK;
+    const CompiledPlan &cp = p->cp;
+    const int words = (cp.n_vars + 31) / 32;
+    auto lds_of = [&](int gl) {
+        const size_t G = size_t(1) << gl;
+        size_t b = (size_t)cp.ws_entries * G * 8;
+        b += ((size_t)cp.arg_entries * G * 2 + 15) & ~size_t(15);
+        b += ((size_t)cp.n_syn * G + 15) & ~size_t(15);
+        b += G * words * 4;
+        return b;
+    };
+    int gl = -1;
+    for (int div : {4, 2, 1}) {
+        for (int x = 6; x >= 0 && gl < 0; --x)
+            if (lds_of(x) <= p->lds_limit / div) gl = x;
+        if (gl >= 0) break;
+    }
+    if (gl < 0) return set_error(TQEC_EUNSUPPORTED, "max-product plan needs more LDS per syndrome than a workgroup holds");
+    const size_t lds = lds_of(gl);
+    const int threads = std::max(64, std::min(256, 1 << (gl + 4)));
+    MapArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.steps = dd->steps;
+    a.msteps = dd->msteps;
+    a.labels = dd->mlabels;
+    a.nsteps = (int)cp.steps.size();
+    a.cpool = dd->cpool64;
+    a.spool = dd->spool64;
+    a.syn = d_syn;
+    a.n = n;
+    a.nbits = cp.n_syn;
+    a.glog = gl;
+    a.ws_entries = cp.ws_entries;
+    a.arg_entries = cp.arg_entries;
+    a.asg_words = words;
+    a.root_off = cp.root_off;
+    a.n_config = n_config;
+    a.config = d_config;
+    a.logp = d_logp;
+    TQ_HIP(hipFuncSetAttribute((const void *)map_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_limit));
+    const int64_t ntiles = (n + (1 << gl) - 1) >> gl;
+    const int per_cu = resident_per_cu(lds, threads, p->lds_limit);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)p->n_cu * per_cu));
+    hipLaunchKernelGGL(map_kernel, dim3(grid), dim3(threads), lds, (hipStream_t)stream, a);
+    TQ_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+int tqec_map_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t n_config, uint8_t *config,
+                          double *logp, void *stream) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
+    if (n > 0 && (!syn || (n_config > 0 && !config))) return set_error(TQEC_ENULL, "null pointer");
+    if (n == 0) return TQEC_OK;
+    hipStream_t st = (hipStream_t)stream;
+    uint8_t *dsyn = nullptr, *dcfg = nullptr;
+    double *dlp = nullptr;
+    const size_t sb = (size_t)n * std::max(1, p->cp.n_syn), cb = (size_t)n * std::max(1, n_config);
+    TQ_HIP(hipMalloc((void **)&dsyn, sb));
+    int rc = TQEC_OK;
+    hipError_t e = hipMalloc((void **)&dcfg, cb);
+    if (e == hipSuccess && logp) e = hipMalloc((void **)&dlp, (size_t)n * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(dsyn, syn, (size_t)n * p->cp.n_syn, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) rc = set_error(TQEC_EDEVICE, hipGetErrorString(e));
+    if (rc == TQEC_OK) rc = tqec_map_decode_batch_device(p, dsyn, n, n_config, dcfg, dlp, stream);
+    if (rc == TQEC_OK && n_config > 0) {
+        e = hipMemcpyAsync(config, dcfg, (size_t)n * n_config, hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) rc = set_error(TQEC_EDEVICE, hipGetErrorString(e));
+    }
+    if (rc == TQEC_OK && logp) {
+        e = hipMemcpyAsync(logp, dlp, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) rc = set_error(TQEC_EDEVICE, hipGetErrorString(e));
+    }
+    e = hipStreamSynchronize(st);
+    if (rc == TQEC_OK && e != hipSuccess) rc = set_error(TQEC_EDEVICE, hipGetErrorString(e));
+    (void)hipFree(dsyn);
+    (void)hipFree(dcfg);
+    if (dlp) (void)hipFree(dlp);
+    return rc;
 }
 
 int tqec_plan_set_tile(tqec_plan *p, int32_t mode, int32_t tile, int32_t threads) {

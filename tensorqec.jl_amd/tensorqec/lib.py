@@ -51,6 +51,7 @@ class PlanDesc(ctypes.Structure):
         ("sliced_label", _i32p),
         ("sliced_bit", _i32p),
         ("tabulate_bits", ctypes.c_int32),
+        ("semiring", ctypes.c_int32),
     ]
 
 
@@ -93,6 +94,11 @@ EXPORTS = [
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p]),
     ("tqec_plan_set_tolerances", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    ("tqec_map_decode_batch", ctypes.c_int,
+     [ctypes.c_void_p, _u8p, ctypes.c_int64, ctypes.c_int32, _u8p, _f64p, ctypes.c_void_p]),
+    ("tqec_map_decode_batch_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p]),
     ("tqec_last_recheck_count", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     ("tqec_plan_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, _i32p, _i32p]),
@@ -162,7 +168,7 @@ def _ptr(a: np.ndarray, t):
 class HipPlan:
     """Owner of a ``tqec_plan*`` built from a :class:`planner.Plan`."""
 
-    def __init__(self, plan, tabulate_bits: int = 4):
+    def __init__(self, plan, tabulate_bits: int = 4, semiring: int = 0):
         lib = load()
         L = plan.n_leaves
         ranks = _i32([len(l) for l in plan.leaf_labels])
@@ -185,7 +191,7 @@ class HipPlan:
         desc = PlanDesc(nvars, L, _ptr(ranks, _i32p), _ptr(labels, _i32p), _ptr(data, _f64p), _ptr(ev, _i32p),
                         plan.n_steps, _ptr(lhs, _i32p), _ptr(rhs, _i32p), _ptr(orank, _i32p), _ptr(olab, _i32p),
                         len(plan.out_labels_final), _ptr(out, _i32p), plan.n_syndrome_bits, len(sl),
-                        _ptr(sl_l, _i32p), _ptr(sl_b, _i32p), int(tabulate_bits))
+                        _ptr(sl_l, _i32p), _ptr(sl_b, _i32p), int(tabulate_bits), int(semiring))
         h = ctypes.c_void_p()
         check(lib.tqec_plan_create(ctypes.byref(desc), ctypes.byref(h)))
         self._h = h
@@ -209,6 +215,18 @@ class HipPlan:
         s = Stats()
         check(load().tqec_plan_stats(self._h, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def map_decode(self, syndromes: np.ndarray, n_config: int):
+        """Max-product plan: (configurations (B, n_config) uint8, log max (B,))."""
+        s = np.ascontiguousarray(np.atleast_2d(np.asarray(syndromes, dtype=np.uint8)))
+        if s.shape[1] != self.n_syn:
+            raise ValueError(f"syndromes must be (B, {self.n_syn}); got {s.shape}")
+        B = s.shape[0]
+        cfg = np.empty((B, n_config), dtype=np.uint8)
+        lp = np.empty(B, dtype=np.float64)
+        check(load().tqec_map_decode_batch(self._h, _ptr(s, _u8p), B, int(n_config),
+                                           _ptr(cfg, _u8p) if cfg.size else None, _ptr(lp, _f64p), None))
+        return cfg, lp
 
     def set_tolerances(self, tie_tol: float, recheck_tol: float) -> None:
         check(load().tqec_plan_set_tolerances(self._h, float(tie_tol), float(recheck_tol)))

@@ -62,7 +62,7 @@ def desc_from(plan, **over):
     d = L.PlanDesc(f["n_vars"], f["n_leaves"], P(f["ranks"], L._i32p), P(f["labels"], L._i32p),
                    P(f["data"], L._f64p), P(f["ev"], L._i32p), f["n_steps"], P(f["lhs"], L._i32p),
                    P(f["rhs"], L._i32p), P(f["orank"], L._i32p), P(f["olab"], L._i32p), f["n_out"],
-                   P(f["out"], L._i32p), f["n_syn"], 0, None, None, f.get("tab", 0))
+                   P(f["out"], L._i32p), f["n_syn"], 0, None, None, f.get("tab", 0), f.get("semiring", 0))
     return d, f
 
 
