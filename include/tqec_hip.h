@@ -161,6 +161,50 @@ int tqec_map_decode_batch(tqec_plan *plan, const uint8_t *syndromes, int64_t n, 
 int tqec_map_decode_batch_device(tqec_plan *plan, const uint8_t *d_syndromes, int64_t n, int32_t n_config,
                                  uint8_t *d_config, double *d_logp, void *stream);
 
+/* Decode only the syndromes whose row numbers are listed in d_list; the
+ * list length is read on the device from *d_count (at most n_max), so the
+ * call can follow a kernel that builds the list without a host round trip.
+ * Outputs are written at the listed rows.  Fused modes only. */
+int tqec_decode_list_device(tqec_plan *plan, const uint8_t *d_syndromes, const int32_t *d_list,
+                            const int32_t *d_count, int64_t n_max, int32_t mode, double *d_marginals,
+                            int32_t *d_classes, uint8_t *d_tie_flags, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned front-end.  No counterpart in the reference: an MLP from syndrome
+ * bits to logical-class logits (3 hidden ReLU layers of `hidden` units, BF16
+ * weights and activations, FP32 accumulation on MFMA), trained on class
+ * posteriors of the TN decoder, with a confidence gate whose rejected
+ * syndromes go to tqec_decode_list_device.
+ * Parameters (FP32, flat): W1[kin][hidden] b1 W2[hidden][hidden] b2
+ * W3[hidden][hidden] b3 W4[hidden][n_out] b4, kin = n_in rounded up to a
+ * multiple of 32 (rows k >= n_in of W1 are kept at zero).
+ * All data pointers are device pointers except tqec_mlp_set/get_params and
+ * the host loss of tqec_mlp_gradients / tqec_mlp_train_step (NULL to skip;
+ * when given, the call synchronises `stream`).
+ * ------------------------------------------------------------------------- */
+typedef struct tqec_mlp tqec_mlp;
+int tqec_mlp_create(int32_t n_in, int32_t hidden, int32_t n_out, tqec_mlp **out);
+int64_t tqec_mlp_num_params(const tqec_mlp *mlp);
+int tqec_mlp_set_params(tqec_mlp *mlp, const float *host_params);
+int tqec_mlp_get_params(tqec_mlp *mlp, float *host_params);
+int tqec_mlp_forward(tqec_mlp *mlp, const uint8_t *d_syndromes, int64_t n, float *d_logits, void *stream);
+/* Mean soft cross-entropy against d_targets (n x n_out class probabilities):
+ * gradients (flat, FP32) into d_grad; n must be a multiple of 32. */
+int tqec_mlp_gradients(tqec_mlp *mlp, const uint8_t *d_syndromes, const float *d_targets, int64_t n,
+                       float *d_grad, float *h_loss, void *stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss. */
+int tqec_mlp_train_step(tqec_mlp *mlp, const uint8_t *d_syndromes, const float *d_targets, int64_t n, float lr,
+                        float *h_loss, void *stream);
+/* Classes (argmax), confidences (largest posterior) and the list of rows
+ * whose confidence is below `threshold` (*d_reject_count is reset first). */
+int tqec_mlp_gate(tqec_mlp *mlp, const uint8_t *d_syndromes, int64_t n, float threshold, int32_t *d_classes,
+                  float *d_confidence, int32_t *d_reject, int32_t *d_reject_count, void *stream);
+void tqec_mlp_destroy(tqec_mlp *mlp);
+/* Training targets from TN marginals: q = marginal / sum over classes (a row
+ * of zeros gives the uniform distribution). */
+int tqec_posteriors_from_marginals(const double *d_marginals, int64_t n, int32_t n_classes, float *d_q,
+                                   void *stream);
+
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);
 

@@ -1,0 +1,601 @@
+// mlp_hip.hip — learned front-end: a syndrome -> logical-class MLP with
+// hand-written HIP training and inference kernels for gfx950.
+//
+// Network: x (n_in bits, zero-padded to a multiple of 32) -> 3 hidden layers
+// (ReLU) -> n_out logits.  Weights are kept in FP32 (master copy, Adam) and
+// in BF16 (the copy every GEMM reads); activations are BF16; every product is
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16.  Training minimises the
+// soft cross-entropy against class posteriors produced by the tensor-network
+// decoder.  A confidence gate accepts the network's class when its posterior
+// reaches a threshold and queues the other syndromes for the TN path.
+//
+// This component has no counterpart in the reference (TensorQEC.jl has no
+// learned decoder); it sits in front of the TNMMAP path
+// (src/decoding/tndecoder.jl:160-165) and falls back to it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tqec_hip.h"
+
+namespace tqec {
+int set_error_shared(int code, const std::string &msg);
+}
+
+namespace {
+
+int fail(int code, const std::string &m) { return tqec::set_error_shared(code, m); }
+
+#define TQM_HIP(call)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess) return fail(TQEC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+typedef uint16_t bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ bf16 f2bf(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (bf16)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);   // round to nearest even
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: C[M][N] = op(A)[M][K] * op(B)[K][N], BF16 in, FP32 accumulate.
+//   A: AT ? A[k*lda + m] : A[m*lda + k];   B: BT ? B[n*ldb + k] : B[k*ldb + n]
+// Block tile 128x128, K step 32, 4 waves each owning 64x64 = 2x2 MFMA tiles
+// of 32x32.  Tiles are staged in LDS as [row][k] (k contiguous) so each lane
+// reads its 8 consecutive k values with one 16-byte LDS load.  Rows/columns
+// beyond M/N are zero-filled on load and skipped on store; K must be a
+// multiple of 32.
+// ---------------------------------------------------------------------------
+
+enum Epi : int {
+    kEpiBiasReluBf16 = 0,  // out_bf16 = bf16(relu(acc + bias[col]))
+    kEpiMaskBf16 = 1,      // out_bf16 = bf16(acc) where mask_bf16 > 0, else 0
+    kEpiF32 = 2,           // out_f32 = acc
+};
+
+struct GemmArgs {
+    const bf16 *A;
+    const bf16 *B;
+    int M, N, K, lda, ldb, ldc;
+    const float *bias;
+    const bf16 *mask;
+    bf16 *outb;
+    float *outf;
+};
+
+constexpr int BM = 128, BN = 128, BK = 32, LDSK = BK + 8;
+
+template <bool T>
+__device__ __forceinline__ void load_tile(bf16 *S, const bf16 *G, int rows0, int R, int k0, int K, int ld, int tid) {
+    // S[r][k] for r in [0,128), k in [0,32)
+    if (!T) {
+        // G[r*ld + k]: 128 rows x 32 k = 512 chunks of 8
+        for (int c = tid; c < 512; c += 256) {
+            const int r = c >> 2, kc = (c & 3) * 8;
+            const int gr = rows0 + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gr < R) v = *reinterpret_cast<const uint4 *>(G + (size_t)gr * ld + k0 + kc);
+            *reinterpret_cast<uint4 *>(S + r * LDSK + kc) = v;
+        }
+    } else {
+        // G[k*ld + r]: 32 k x 128 r = 512 chunks of 8 along r, transposed into S
+        for (int c = tid; c < 512; c += 256) {
+            const int k = c >> 4, rc = (c & 15) * 8;
+            const int gr = rows0 + rc;
+            bf16 v[8];
+            if (gr + 7 < R) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(G + (size_t)(k0 + k) * ld + gr);
+                memcpy(v, &u, 16);
+            } else {
+                for (int j = 0; j < 8; ++j) v[j] = (gr + j < R) ? G[(size_t)(k0 + k) * ld + gr + j] : (bf16)0;
+            }
+            for (int j = 0; j < 8; ++j) S[(rc + j) * LDSK + k] = v[j];
+        }
+    }
+}
+
+template <bool AT, bool BT, int EPI>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+    __shared__ __align__(16) bf16 As[BM * LDSK];
+    __shared__ __align__(16) bf16 Bs[BN * LDSK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x16 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        load_tile<AT>(As, g.A, m0, g.M, k0, g.K, g.lda, tid);
+        load_tile<BT ? false : true>(Bs, g.B, n0, g.N, k0, g.K, g.ldb, tid);
+        __syncthreads();
+        for (int kk = 0; kk < BK; kk += 16) {
+            bf16x8 a[2], b[2];
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(As + (wm + i * 32 + (lane & 31)) * LDSK + kk + 8 * (lane >> 5));
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8 *>(Bs + (wn + j * 32 + (lane & 31)) * LDSK + kk + 8 * (lane >> 5));
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout of the 32x32 tile: lane l holds column l%32, value v is row
+    // 8*(v/4) + 4*(l/32) + v%4.
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + j * 32 + (lane & 31);
+            if (col >= g.N) continue;
+            for (int v = 0; v < 16; ++v) {
+                const int row = m0 + wm + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (row >= g.M) continue;
+                const float x = acc[i][j][v];
+                const size_t o = (size_t)row * g.ldc + col;
+                if (EPI == kEpiBiasReluBf16) {
+                    g.outb[o] = f2bf(fmaxf(x + g.bias[col], 0.f));
+                } else if (EPI == kEpiMaskBf16) {
+                    g.outb[o] = bf2f(g.mask[o]) > 0.f ? f2bf(x) : (bf16)0;
+                } else {
+                    g.outf[o] = x;
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels
+// ---------------------------------------------------------------------------
+
+// Syndrome bytes [B][n_in] -> BF16 0/1 features [B][kin] (zero padded).
+__global__ void embed_kernel(const uint8_t *__restrict__ syn, int64_t B, int n_in, int kin, bf16 *__restrict__ X) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * kin) return;
+    const int64_t b = i / kin;
+    const int f = (int)(i - b * kin);
+    X[i] = (f < n_in && (syn[b * n_in + f] & 1)) ? (bf16)0x3f80 : (bf16)0;
+}
+
+// Output layer: z[b][c] = sum_k H[b][k] W[k][c] + bias[c]; one wave per row.
+__global__ void out_fwd_kernel(const bf16 *__restrict__ H, int64_t B, int hid, const bf16 *__restrict__ W,
+                               const float *__restrict__ bias, int nout, float *__restrict__ z) {
+    const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= B) return;
+    float part[16];
+    for (int c = 0; c < nout; ++c) part[c] = 0.f;
+    for (int k = lane; k < hid; k += 64) {
+        const float h = bf2f(H[b * hid + k]);
+        for (int c = 0; c < nout; ++c) part[c] += h * bf2f(W[k * nout + c]);
+    }
+    for (int c = 0; c < nout; ++c) {
+        float s = part[c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) z[b * nout + c] = s + bias[c];
+    }
+}
+
+// Softmax + soft cross-entropy: dz = (softmax(z) - q) / B (FP32), loss summed
+// per block into loss_parts.
+__global__ void xent_kernel(const float *__restrict__ z, const float *__restrict__ q, int64_t B, int nout,
+                            float *__restrict__ dz, float *__restrict__ loss_parts) {
+    __shared__ float red[256];
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (b < B) {
+        float m = -INFINITY;
+        for (int c = 0; c < nout; ++c) m = fmaxf(m, z[b * nout + c]);
+        float s = 0.f;
+        for (int c = 0; c < nout; ++c) s += expf(z[b * nout + c] - m);
+        const float ls = logf(s);
+        for (int c = 0; c < nout; ++c) {
+            const float lp = z[b * nout + c] - m - ls;
+            const float qc = q[b * nout + c];
+            l -= qc * lp;
+            dz[b * nout + c] = (expf(lp) - qc) / (float)B;
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0];
+}
+
+// dA[b][k] = (sum_c dz[b][c] W[k][c]) masked by H[b][k] > 0, as BF16.
+__global__ void out_bwd_dx_kernel(const float *__restrict__ dz, const bf16 *__restrict__ W, const bf16 *__restrict__ H,
+                                  int64_t B, int hid, int nout, bf16 *__restrict__ dA) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * hid) return;
+    const int64_t b = i / hid;
+    const int k = (int)(i - b * hid);
+    float s = 0.f;
+    for (int c = 0; c < nout; ++c) s += dz[b * nout + c] * bf2f(W[k * nout + c]);
+    dA[i] = bf2f(H[i]) > 0.f ? f2bf(s) : (bf16)0;
+}
+
+// dW[k][c] = sum_b H[b][k] dz[b][c] (one thread per (k, c), loop over b).
+__global__ void out_bwd_dw_kernel(const bf16 *__restrict__ H, const float *__restrict__ dz, int64_t B, int hid,
+                                  int nout, float *__restrict__ dW) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hid * nout) return;
+    const int k = i / nout, c = i - k * nout;
+    float s = 0.f;
+    for (int64_t b = 0; b < B; ++b) s += bf2f(H[b * hid + k]) * dz[b * nout + c];
+    dW[i] = s;
+}
+
+// Column sums: db[c] = sum_b D[b][c] (BF16 or FP32 input).
+template <typename T>
+__global__ void colsum_kernel(const T *__restrict__ D, int64_t B, int n, float *__restrict__ db) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    float s = 0.f;
+    for (int64_t b = 0; b < B; ++b) {
+        if constexpr (sizeof(T) == 2) s += bf2f(D[b * n + c]);
+        else s += D[b * n + c];
+    }
+    db[c] = s;
+}
+
+// Adam on the FP32 master parameters; refreshes the BF16 copy.
+__global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, bf16 *__restrict__ wb, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    wb[i] = f2bf(wi);
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ w, bf16 *__restrict__ wb, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wb[i] = f2bf(w[i]);
+}
+
+// Confidence gate: the class is accepted when the largest posterior is at
+// least `threshold`; otherwise the syndrome id is appended to `reject`.
+__global__ void gate_kernel(const float *__restrict__ z, int64_t B, int nout, float threshold,
+                            int32_t *__restrict__ classes, float *__restrict__ conf, int32_t *__restrict__ reject,
+                            int32_t *__restrict__ reject_count) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float m = -INFINITY;
+    int bi = 0;
+    for (int c = 0; c < nout; ++c) {
+        const float x = z[b * nout + c];
+        if (x > m) {
+            m = x;
+            bi = c;
+        }
+    }
+    float s = 0.f;
+    for (int c = 0; c < nout; ++c) s += expf(z[b * nout + c] - m);
+    const float pmax = 1.f / s;
+    if (conf) conf[b] = pmax;
+    if (classes) classes[b] = bi;
+    if (pmax < threshold && reject) {
+        const int32_t slot = atomicAdd(reject_count, 1);
+        reject[slot] = (int32_t)b;
+    }
+}
+
+__global__ void zero_count_kernel(int32_t *p) { *p = 0; }
+
+// Class posteriors from TN marginals: q[b][c] = mar[b][c] / sum_c mar[b][c].
+__global__ void normalize_kernel(const double *__restrict__ mar, int64_t B, int nout, float *__restrict__ q) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double s = 0;
+    for (int c = 0; c < nout; ++c) s += mar[b * nout + c];
+    for (int c = 0; c < nout; ++c) q[b * nout + c] = s > 0 ? (float)(mar[b * nout + c] / s) : 1.f / nout;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct tqec_mlp {
+    int n_in = 0, kin = 0, hid = 0, nout = 0;
+    int64_t nparams = 0;
+    int64_t off[8] = {0};          // W1 b1 W2 b2 W3 b3 W4 b4 in the flat parameter vector
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    bf16 *wb = nullptr;
+    int64_t step = 0;
+    // activations / scratch for up to `cap` rows
+    int64_t cap = 0;
+    bf16 *X = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *dA = nullptr, *dB = nullptr;
+    float *z = nullptr, *dz = nullptr, *loss_parts = nullptr;
+    int64_t loss_cap = 0;
+};
+
+namespace {
+
+int64_t param_count(int kin, int hid, int nout, int64_t off[8]) {
+    int64_t o = 0;
+    const int64_t sizes[8] = {(int64_t)kin * hid, hid, (int64_t)hid * hid, hid, (int64_t)hid * hid, hid,
+                              (int64_t)hid * nout, nout};
+    for (int i = 0; i < 8; ++i) {
+        off[i] = o;
+        o += sizes[i];
+    }
+    return o;
+}
+
+int ensure_cap(tqec_mlp *p, int64_t B) {
+    if (p->cap >= B) return TQEC_OK;
+    bf16 **bufs[6] = {&p->X, &p->H1, &p->H2, &p->H3, &p->dA, &p->dB};
+    for (auto b : bufs)
+        if (*b) (void)hipFree(*b);
+    if (p->z) (void)hipFree(p->z);
+    if (p->dz) (void)hipFree(p->dz);
+    p->cap = 0;
+    TQM_HIP(hipMalloc((void **)&p->X, (size_t)B * p->kin * sizeof(bf16)));
+    for (int i = 1; i < 6; ++i) TQM_HIP(hipMalloc((void **)bufs[i], (size_t)B * p->hid * sizeof(bf16)));
+    TQM_HIP(hipMalloc((void **)&p->z, (size_t)B * p->nout * sizeof(float)));
+    TQM_HIP(hipMalloc((void **)&p->dz, (size_t)B * p->nout * sizeof(float)));
+    const int64_t nb = (B + 255) / 256;
+    if (p->loss_cap < nb) {
+        if (p->loss_parts) (void)hipFree(p->loss_parts);
+        TQM_HIP(hipMalloc((void **)&p->loss_parts, (size_t)nb * sizeof(float)));
+        p->loss_cap = nb;
+    }
+    p->cap = B;
+    return TQEC_OK;
+}
+
+template <bool AT, bool BT, int EPI>
+void gemm(const GemmArgs &a, hipStream_t st) {
+    dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM);
+    hipLaunchKernelGGL((gemm_bf16_kernel<AT, BT, EPI>), grid, dim3(256), 0, st, a);
+}
+
+int forward(tqec_mlp *p, const uint8_t *d_syn, int64_t B, hipStream_t st) {
+    int rc;
+    if ((rc = ensure_cap(p, B))) return rc;
+    const int64_t nx = B * p->kin;
+    hipLaunchKernelGGL(embed_kernel, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, d_syn, B, p->n_in, p->kin,
+                       p->X);
+    const bf16 *in[3] = {p->X, p->H1, p->H2};
+    bf16 *out[3] = {p->H1, p->H2, p->H3};
+    const int kdim[3] = {p->kin, p->hid, p->hid};
+    for (int l = 0; l < 3; ++l) {
+        GemmArgs a{};
+        a.A = in[l];
+        a.B = p->wb + p->off[2 * l];
+        a.M = (int)B;
+        a.N = p->hid;
+        a.K = kdim[l];
+        a.lda = kdim[l];
+        a.ldb = p->hid;
+        a.ldc = p->hid;
+        a.bias = p->w + p->off[2 * l + 1];
+        a.outb = out[l];
+        gemm<false, false, kEpiBiasReluBf16>(a, st);
+    }
+    const unsigned rows_per_block = 4;
+    hipLaunchKernelGGL(out_fwd_kernel, dim3((unsigned)((B + rows_per_block - 1) / rows_per_block)), dim3(256), 0, st,
+                       p->H3, B, p->hid, p->wb + p->off[6], p->w + p->off[7], p->nout, p->z);
+    TQM_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+// Gradients of the mean soft cross-entropy into p->g (FP32, flat layout).
+int backward(tqec_mlp *p, const float *d_q, int64_t B, hipStream_t st) {
+    const int64_t nb = (B + 255) / 256;
+    hipLaunchKernelGGL(xent_kernel, dim3((unsigned)nb), dim3(256), 0, st, p->z, d_q, B, p->nout, p->dz, p->loss_parts);
+    // output layer
+    hipLaunchKernelGGL(out_bwd_dw_kernel, dim3((unsigned)((p->hid * p->nout + 255) / 256)), dim3(256), 0, st, p->H3,
+                       p->dz, B, p->hid, p->nout, p->g + p->off[6]);
+    hipLaunchKernelGGL((colsum_kernel<float>), dim3(1), dim3(64), 0, st, p->dz, B, p->nout, p->g + p->off[7]);
+    const int64_t nh = B * p->hid;
+    hipLaunchKernelGGL(out_bwd_dx_kernel, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, p->dz,
+                       p->wb + p->off[6], p->H3, B, p->hid, p->nout, p->dA);
+    // hidden layers 3, 2, 1: dA holds dL/d(pre-activation) of layer l
+    const bf16 *hin[3] = {p->X, p->H1, p->H2};
+    const int kdim[3] = {p->kin, p->hid, p->hid};
+    bf16 *cur = p->dA, *nxt = p->dB;
+    for (int l = 2; l >= 0; --l) {
+        // dW_l[K][N] = in_l^T [K][B] * dA [B][N]
+        GemmArgs w{};
+        w.A = hin[l];
+        w.B = cur;
+        w.M = kdim[l];
+        w.N = p->hid;
+        w.K = (int)B;
+        w.lda = kdim[l];
+        w.ldb = p->hid;
+        w.ldc = p->hid;
+        w.outf = p->g + p->off[2 * l];
+        gemm<true, false, kEpiF32>(w, st);
+        hipLaunchKernelGGL((colsum_kernel<bf16>), dim3((unsigned)((p->hid + 255) / 256)), dim3(256), 0, st, cur, B,
+                           p->hid, p->g + p->off[2 * l + 1]);
+        if (l > 0) {
+            // d in_l [B][K] = dA [B][N] * W_l^T (W_l stored [K][N]), masked by in_l > 0
+            GemmArgs x{};
+            x.A = cur;
+            x.B = p->wb + p->off[2 * l];
+            x.M = (int)B;
+            x.N = kdim[l];
+            x.K = p->hid;
+            x.lda = p->hid;
+            x.ldb = p->hid;
+            x.ldc = kdim[l];
+            x.mask = hin[l];
+            x.outb = nxt;
+            gemm<false, true, kEpiMaskBf16>(x, st);
+            std::swap(cur, nxt);
+        }
+    }
+    TQM_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+// Training reduces over the batch inside the weight-gradient GEMM, whose K
+// loop needs a multiple of 32 rows; inference takes any batch size.
+int check_batch(tqec_mlp *p, int64_t B, bool training) {
+    if (!p) return fail(TQEC_ENULL, "null network");
+    if (B <= 0 || B > (int64_t(1) << 30)) return fail(TQEC_EINVAL, "batch size out of range");
+    if (training && B % 32 != 0) return fail(TQEC_EINVAL, "training batch size must be a multiple of 32");
+    return TQEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tqec_mlp_create(int32_t n_in, int32_t hidden, int32_t n_out, tqec_mlp **out) {
+    if (!out) return fail(TQEC_ENULL, "null output handle");
+    *out = nullptr;
+    if (n_in <= 0 || n_in > 4096 || hidden <= 0 || hidden % 128 != 0 || n_out <= 0 || n_out > 16)
+        return fail(TQEC_EINVAL, "n_in in [1, 4096], hidden a multiple of 128, n_out in [1, 16]");
+    auto *p = new (std::nothrow) tqec_mlp();
+    if (!p) return fail(TQEC_ENOMEM, "out of host memory");
+    p->n_in = n_in;
+    p->kin = (n_in + 31) / 32 * 32;
+    p->hid = hidden;
+    p->nout = n_out;
+    p->nparams = param_count(p->kin, hidden, n_out, p->off);
+    const size_t nb = (size_t)p->nparams;
+    hipError_t e = hipMalloc((void **)&p->w, nb * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->g, nb * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->m, nb * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->v, nb * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->wb, nb * 2);
+    if (e == hipSuccess) e = hipMemset(p->w, 0, nb * 4);
+    if (e == hipSuccess) e = hipMemset(p->m, 0, nb * 4);
+    if (e == hipSuccess) e = hipMemset(p->v, 0, nb * 4);
+    if (e == hipSuccess) e = hipMemset(p->wb, 0, nb * 2);
+    if (e != hipSuccess) {
+        tqec_mlp_destroy(p);
+        return fail(TQEC_EDEVICE, hipGetErrorString(e));
+    }
+    *out = p;
+    return TQEC_OK;
+}
+
+int64_t tqec_mlp_num_params(const tqec_mlp *p) { return p ? p->nparams : -1; }
+
+int tqec_mlp_set_params(tqec_mlp *p, const float *host) {
+    if (!p || !host) return fail(TQEC_ENULL, "null argument");
+    // padded input rows of W1 (k >= n_in) must stay zero
+    std::vector<float> w(host, host + p->nparams);
+    for (int k = p->n_in; k < p->kin; ++k)
+        for (int c = 0; c < p->hid; ++c) w[p->off[0] + (int64_t)k * p->hid + c] = 0.f;
+    TQM_HIP(hipMemcpy(p->w, w.data(), (size_t)p->nparams * 4, hipMemcpyHostToDevice));
+    TQM_HIP(hipMemset(p->m, 0, (size_t)p->nparams * 4));
+    TQM_HIP(hipMemset(p->v, 0, (size_t)p->nparams * 4));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((p->nparams + 255) / 256)), dim3(256), 0, 0, p->w, p->wb,
+                       p->nparams);
+    TQM_HIP(hipDeviceSynchronize());
+    p->step = 0;
+    return TQEC_OK;
+}
+
+int tqec_mlp_get_params(tqec_mlp *p, float *host) {
+    if (!p || !host) return fail(TQEC_ENULL, "null argument");
+    TQM_HIP(hipDeviceSynchronize());
+    TQM_HIP(hipMemcpy(host, p->w, (size_t)p->nparams * 4, hipMemcpyDeviceToHost));
+    return TQEC_OK;
+}
+
+int tqec_mlp_forward(tqec_mlp *p, const uint8_t *d_syn, int64_t B, float *d_logits, void *stream) {
+    int rc;
+    if ((rc = check_batch(p, B, false))) return rc;
+    if (!d_syn || !d_logits) return fail(TQEC_ENULL, "null device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = forward(p, d_syn, B, st))) return rc;
+    TQM_HIP(hipMemcpyAsync(d_logits, p->z, (size_t)B * p->nout * 4, hipMemcpyDeviceToDevice, st));
+    return TQEC_OK;
+}
+
+int tqec_mlp_gradients(tqec_mlp *p, const uint8_t *d_syn, const float *d_targets, int64_t B, float *d_grad,
+                       float *h_loss, void *stream) {
+    int rc;
+    if ((rc = check_batch(p, B, true))) return rc;
+    if (!d_syn || !d_targets) return fail(TQEC_ENULL, "null device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = forward(p, d_syn, B, st))) return rc;
+    if ((rc = backward(p, d_targets, B, st))) return rc;
+    if (d_grad) TQM_HIP(hipMemcpyAsync(d_grad, p->g, (size_t)p->nparams * 4, hipMemcpyDeviceToDevice, st));
+    if (h_loss) {
+        const int64_t nb = (B + 255) / 256;
+        std::vector<float> parts(nb);
+        TQM_HIP(hipMemcpyAsync(parts.data(), p->loss_parts, nb * 4, hipMemcpyDeviceToHost, st));
+        TQM_HIP(hipStreamSynchronize(st));
+        double s = 0;
+        for (float x : parts) s += x;
+        *h_loss = (float)(s / (double)B);
+    }
+    return TQEC_OK;
+}
+
+int tqec_mlp_train_step(tqec_mlp *p, const uint8_t *d_syn, const float *d_targets, int64_t B, float lr,
+                        float *h_loss, void *stream) {
+    int rc;
+    if ((rc = tqec_mlp_gradients(p, d_syn, d_targets, B, nullptr, h_loss, stream))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    p->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)p->step), c2 = 1.f - std::pow(b2, (float)p->step);
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((p->nparams + 255) / 256)), dim3(256), 0, st, p->w, p->g, p->m,
+                       p->v, p->wb, p->nparams, lr, b1, b2, 1e-8f, c1, c2);
+    // padded rows of W1 get zero gradient (zero inputs) and so stay zero
+    TQM_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+int tqec_mlp_gate(tqec_mlp *p, const uint8_t *d_syn, int64_t B, float threshold, int32_t *d_classes,
+                  float *d_confidence, int32_t *d_reject, int32_t *d_reject_count, void *stream) {
+    int rc;
+    if ((rc = check_batch(p, B, false))) return rc;
+    if (!d_syn || (d_reject && !d_reject_count)) return fail(TQEC_ENULL, "null device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = forward(p, d_syn, B, st))) return rc;
+    if (d_reject_count) hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, d_reject_count);
+    hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, p->z, B, p->nout, threshold,
+                       d_classes, d_confidence, d_reject, d_reject_count);
+    TQM_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+int tqec_posteriors_from_marginals(const double *d_marginals, int64_t n, int32_t n_classes, float *d_q,
+                                   void *stream) {
+    if (n < 0 || n_classes <= 0) return fail(TQEC_EINVAL, "bad sizes");
+    if (n > 0 && (!d_marginals || !d_q)) return fail(TQEC_ENULL, "null device pointer");
+    if (n == 0) return TQEC_OK;
+    hipLaunchKernelGGL(normalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       d_marginals, n, n_classes, d_q);
+    TQM_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+void tqec_mlp_destroy(tqec_mlp *p) {
+    if (!p) return;
+    void *bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->X, p->H1, p->H2, p->H3, p->dA, p->dB, p->z, p->dz, p->loss_parts};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+}  // extern "C"

@@ -744,13 +744,18 @@ FusedArgs base_args(tqec_plan *p, DeviceData *dd, const uint8_t *d_syn, int64_t 
     return a;
 }
 
+// `in_list` / `in_count` (device): decode only the syndromes listed (at most
+// n of them), writing their outputs at their own row.
 int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, int mode, double *d_mar,
-              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream, const int32_t *in_list = nullptr,
+              const int32_t *in_count = nullptr) {
     const FusedCfg &c64 = p->f64;
     if (c64.glog < 0)
         return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP64)");
     if (mode == TQEC_MODE_FUSED_F64) {
         FusedArgs a = base_args(p, dd, d_syn, n, c64.glog, false);
+        a.list = in_list;
+        a.list_count = in_count;
         a.marginals = d_mar;
         a.classes = d_cls;
         a.ties = d_tie;
@@ -780,6 +785,8 @@ int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, 
     }
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
     FusedArgs a = base_args(p, dd, d_syn, n, c32.glog, true);
+    a.list = in_list;
+    a.list_count = in_count;
     a.marginals = d_mar;
     a.classes = d_cls;
     a.ties = d_tie;
@@ -1034,6 +1041,29 @@ int tqec_plan_autotune(tqec_plan *p, int32_t mode, const uint8_t *d_syn, int64_t
     if (tile) *tile = 1 << best.glog;
     if (threads) *threads = best.threads;
     return TQEC_OK;
+}
+
+int tqec_decode_list_device(tqec_plan *p, const uint8_t *d_syn, const int32_t *d_list, const int32_t *d_count,
+                            int64_t n_max, int32_t mode, double *d_mar, int32_t *d_cls, uint8_t *d_tie,
+                            void *stream) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (p->cp.semiring != TQEC_SEMIRING_SUM)
+        return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
+    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
+        return set_error(TQEC_EINVAL, "list decoding runs on the fused executors");
+    if (n_max < 0 || n_max > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "n_max out of range");
+    if (n_max > 0 && (!d_syn || !d_list || !d_count)) return set_error(TQEC_ENULL, "null device pointer");
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc;
+    if ((rc = configure(p))) return rc;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    DeviceData *dd;
+    if ((rc = ensure_device(p, dev, &dd))) return rc;
+    Workspace &w = p->ws[std::make_pair(dev, (hipStream_t)stream)];
+    w.device = dev;
+    if (n_max == 0) return TQEC_OK;
+    return run_fused(p, dd, w, d_syn, n_max, mode, d_mar, d_cls, d_tie, (hipStream_t)stream, d_list, d_count);
 }
 
 int tqec_map_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int32_t n_config,

@@ -125,6 +125,25 @@ EXPORTS = [
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p]),
     ("tqec_recovery_destroy", None, [ctypes.c_void_p]),
+    ("tqec_decode_list_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("tqec_mlp_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_mlp_num_params", ctypes.c_int64, [ctypes.c_void_p]),
+    ("tqec_mlp_set_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("tqec_mlp_get_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("tqec_mlp_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    ("tqec_mlp_gradients", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                           ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
+    ("tqec_mlp_train_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
+    ("tqec_mlp_gate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
+    ("tqec_mlp_destroy", None, [ctypes.c_void_p]),
+    ("tqec_posteriors_from_marginals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                                       ctypes.c_void_p, ctypes.c_void_p]),
     ("tqec_last_error", ctypes.c_char_p, []),
 ]
 
@@ -282,6 +301,16 @@ class HipPlan:
         check(load().tqec_kernel_time_totals(self._h, ctypes.c_void_p(stream) if stream else None,
                                              ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def decode_list_device(self, d_syn: int, d_list: int, d_count: int, n_max: int, mode, d_mar: int,
+                           d_cls: int, d_tie: int, stream: int = 0) -> None:
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        check(load().tqec_decode_list_device(self._h, ctypes.c_void_p(d_syn), ctypes.c_void_p(d_list),
+                                             ctypes.c_void_p(d_count), int(n_max), m,
+                                             ctypes.c_void_p(d_mar) if d_mar else None,
+                                             ctypes.c_void_p(d_cls) if d_cls else None,
+                                             ctypes.c_void_p(d_tie) if d_tie else None,
+                                             ctypes.c_void_p(stream) if stream else None))
 
     def last_recheck_count(self, stream: int = 0) -> int:
         c = ctypes.c_int64()
