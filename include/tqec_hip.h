@@ -51,6 +51,12 @@ extern "C" {
                                  intermediates in HBM                     */
 #define TQEC_MODE_FUSED_F64 1 /* whole plan per workgroup tile, FP64,
                                  intermediates in LDS                     */
+#define TQEC_MODE_WIDE_F32 3  /* plain layout (intermediates in HBM) in FP32
+                                 with per-tensor power-of-two scaling;
+                                 low-margin syndromes re-contracted by the
+                                 plain FP64 executor (for plans too wide
+                                 for LDS; synchronises the stream once)  */
+
 #define TQEC_SEMIRING_SUM 0   /* sum-product: marginals (TNMMAP)            */
 #define TQEC_SEMIRING_MAX 1   /* max-product with arg-max backtrace (TNMAP)  */
 

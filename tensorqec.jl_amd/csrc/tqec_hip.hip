@@ -116,33 +116,136 @@ __device__ __forceinline__ const T *plain_base(const OperandDesc &op, int64_t b,
     return pools.spool + op.off + ((int64_t)v << op.rank);
 }
 
-template <typename T, bool AW, bool BW>
-__device__ __forceinline__ void plain_loop(const StepDesc &st, const T *pa, const T *pb, T *pc, int64_t ldw) {
+// Per-slot, per-syndrome scale of the wide FP32 executor: the largest
+// entry's bits (atomic max of non-negative floats) and the exponent, laid out
+// [slot][ldw] like the workspace.
+struct PlainScale {
+    uint32_t *mx;
+    int32_t *ex;
+};
+
+template <typename T, bool SCALE, bool AW, bool BW>
+__device__ __forceinline__ T plain_loop(const StepDesc &st, const T *pa, const T *pb, T *pc, int64_t ldw,
+                                        int shift) {
     const uint32_t nc = 1u << st.rc;
     const int cexp = st.a.cexp + st.b.cexp;
+    T lmax = 0;
     for (uint32_t c = blockIdx.y; c < nc; c += gridDim.y) {
         T acc = contract_one<T, AW, BW>(st, c, pa, pb, ldw);
-        if (cexp != 0) acc = ldexp(acc, cexp);
+        if (SCALE) {
+            acc = ldexp(acc, shift);
+            lmax = acc > lmax ? acc : lmax;
+        } else if (cexp != 0) {
+            acc = ldexp(acc, cexp);
+        }
         pc[(int64_t)c * ldw] = acc;
     }
+    return lmax;
 }
 
 // One plan step for a chunk of syndromes: thread = (output entry c, syndrome
 // b); c is uniform across the block, so all index arithmetic is scalar and a
-// wavefront reads 64 consecutive syndromes of one entry.
-template <typename T>
+// wavefront reads 64 consecutive syndromes of one entry.  With SCALE (FP32),
+// every per-syndrome tensor carries a power-of-two exponent as in the fused
+// executor; the producer's maximum is gathered with a global atomic max.
+template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) plain_step_kernel(StepDesc st, T *__restrict__ ws, int64_t ldw, int64_t nb,
-                                                         Pools<T> pools, const uint8_t *__restrict__ synT) {
+                                                         Pools<T> pools, const uint8_t *__restrict__ synT,
+                                                         PlainScale sc) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     const T *pa = plain_base(st.a, b, ws, ldw, pools, synT);
     const T *pb = plain_base(st.b, b, ws, ldw, pools, synT);
     T *pc = ws + st.c_off * ldw + b;
+    int shift = 0, ea = 0, eb = 0;
+    if (SCALE) {
+        int na = 0, nbx = 0;
+        if (st.a.kind == kInter) {
+            const float m = __uint_as_float(sc.mx[(int64_t)st.a.slot * ldw + b]);
+            na = m > 0.f ? ilogbf(m) : 0;
+            ea = sc.ex[(int64_t)st.a.slot * ldw + b] + na;
+        } else {
+            ea = st.a.cexp;
+        }
+        if (st.b.kind == kInter) {
+            const float m = __uint_as_float(sc.mx[(int64_t)st.b.slot * ldw + b]);
+            nbx = m > 0.f ? ilogbf(m) : 0;
+            eb = sc.ex[(int64_t)st.b.slot * ldw + b] + nbx;
+        } else {
+            eb = st.b.cexp;
+        }
+        shift = -(na + nbx);
+    }
     const bool aw = st.a.kind == kInter, bw = st.b.kind == kInter;
-    if (aw && bw) plain_loop<T, true, true>(st, pa, pb, pc, ldw);
-    else if (aw) plain_loop<T, true, false>(st, pa, pb, pc, ldw);
-    else if (bw) plain_loop<T, false, true>(st, pa, pb, pc, ldw);
-    else plain_loop<T, false, false>(st, pa, pb, pc, ldw);
+    T lmax;
+    if (aw && bw) lmax = plain_loop<T, SCALE, true, true>(st, pa, pb, pc, ldw, shift);
+    else if (aw) lmax = plain_loop<T, SCALE, true, false>(st, pa, pb, pc, ldw, shift);
+    else if (bw) lmax = plain_loop<T, SCALE, false, true>(st, pa, pb, pc, ldw, shift);
+    else lmax = plain_loop<T, SCALE, false, false>(st, pa, pb, pc, ldw, shift);
+    if (SCALE) {
+        atomicMax(&sc.mx[(int64_t)st.c_slot * ldw + b], __float_as_uint((float)lmax));
+        if (blockIdx.y == 0) sc.ex[(int64_t)st.c_slot * ldw + b] = ea + eb;
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Wide FP32 result: marginals with the exponent applied, first arg-max, tie
+// flag, and low-margin syndromes appended to the FP64 re-check list.
+__global__ void plain_finalize32_kernel(const float *__restrict__ ws, int64_t ldw, int64_t nb, int64_t root_off,
+                                        const int32_t *__restrict__ ex_root, int nout, double tie_tol,
+                                        double recheck_tol, int64_t out_base, double *__restrict__ marginals,
+                                        int32_t *__restrict__ classes, uint8_t *__restrict__ ties,
+                                        int32_t *__restrict__ list, int32_t *__restrict__ count) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int e = ex_root[b];
+    float best = -1, second = -1;
+    int bi = 0;
+    for (int j = 0; j < nout; ++j) {
+        const float x = ws[(root_off + j) * ldw + b];
+        if (marginals) marginals[(out_base + b) * nout + j] = ldexp((double)x, e);
+        if (j == 0 || x > best) {
+            if (j > 0) second = best;
+            best = x;
+            bi = j;
+        } else if (x > second) {
+            second = x;
+        }
+    }
+    if (classes) classes[out_base + b] = bi;
+    const bool low = nout > 1 && second >= 0 && (double)(best - second) <= tie_tol * (double)best;
+    if (ties) ties[out_base + b] = low ? 1 : 0;
+    const bool rc = !(best > 0) ||
+                    (nout > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);
+    if (rc) {
+        const int32_t slot = atomicAdd(count, 1);
+        list[slot] = (int32_t)(out_base + b);
+    }
+}
+
+__global__ void gather_rows_kernel(const uint8_t *__restrict__ syn, int nbits, const int32_t *__restrict__ list,
+                                   int64_t n, uint8_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * nbits) return;
+    const int64_t r = i / nbits;
+    out[i] = syn[(int64_t)list[r] * nbits + (i - r * nbits)];
+}
+
+__global__ void scatter_results_kernel(const int32_t *__restrict__ list, int64_t n, int nout,
+                                       const double *__restrict__ mar, const int32_t *__restrict__ cls,
+                                       const uint8_t *__restrict__ tie, double *__restrict__ d_mar,
+                                       int32_t *__restrict__ d_cls, uint8_t *__restrict__ d_tie) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t b = list[r];
+    if (d_mar)
+        for (int j = 0; j < nout; ++j) d_mar[b * nout + j] = mar[r * nout + j];
+    if (d_cls) d_cls[b] = cls[r];
+    if (d_tie) d_tie[b] = tie[r];
 }
 
 // First maximum (Julia findmax order) and top-2 gap of one syndrome.
@@ -526,6 +629,12 @@ struct Workspace {
     int32_t *d_cls = nullptr; size_t d_cls_bytes = 0;
     uint8_t *d_tie = nullptr; size_t d_tie_bytes = 0;
     int64_t last_recheck = -1;
+    void *smx = nullptr; size_t smx_bytes = 0;       // wide FP32 scales
+    void *sex = nullptr; size_t sex_bytes = 0;
+    uint8_t *g_syn = nullptr; size_t g_syn_bytes = 0; // wide FP32 re-check staging
+    double *g_mar = nullptr; size_t g_mar_bytes = 0;
+    int32_t *g_cls = nullptr; size_t g_cls_bytes = 0;
+    uint8_t *g_tie = nullptr; size_t g_tie_bytes = 0;
     // Per-call HIP events (main start/stop, re-check start/stop), accumulated
     // while timing is enabled and consumed by tqec_kernel_time_totals.
     std::vector<std::array<hipEvent_t, 4>> ev;
@@ -711,13 +820,87 @@ int run_plain(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, 
                            w.synT, chunk);
         for (const StepDesc &st : cp.steps) {
             const unsigned ny = (unsigned)std::min<int64_t>(int64_t(1) << st.rc, 65535);
-            hipLaunchKernelGGL(plain_step_kernel<double>, dim3(grd.x, ny), blk, 0, stream, st, (double *)w.ws,
-                               chunk, nb, pools, (const uint8_t *)w.synT);
+            hipLaunchKernelGGL((plain_step_kernel<double, false>), dim3(grd.x, ny), blk, 0, stream, st,
+                               (double *)w.ws, chunk, nb, pools, (const uint8_t *)w.synT, PlainScale{nullptr, nullptr});
         }
         hipLaunchKernelGGL(plain_finalize_kernel, grd, blk, 0, stream, (const double *)w.ws, chunk, nb, cp.root_off,
                            nout, p->tie_tol, s0, d_mar, d_cls, d_tie);
     }
     if ((rc = record(p, w, 1, stream))) return rc;
+    TQ_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+// Wide FP32 executor: the plain executor's layout and launches in FP32 with
+// per-tensor, per-syndrome power-of-two scaling; syndromes whose top-2 margin
+// is below the re-check threshold are gathered and re-contracted by the plain
+// FP64 executor (one host synchronisation for the list length).
+int run_wide(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+             int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+    const CompiledPlan &cp = p->cp;
+    const int64_t per = std::max<int64_t>(1, cp.ws_entries) * (int64_t)sizeof(float);
+    int64_t chunk = std::max<int64_t>(256, p->plain_chunk_cap_bytes / per);
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(n, 1));
+    chunk = (chunk + 255) / 256 * 256;
+    int rc;
+    if ((rc = grow((char **)&w.ws, &w.ws_bytes, (size_t)(per * chunk)))) return rc;
+    if ((rc = grow(&w.synT, &w.synT_bytes, (size_t)std::max(1, cp.n_syn) * chunk))) return rc;
+    const size_t scale_bytes = (size_t)std::max(1, cp.n_slots) * chunk * 4;
+    if ((rc = grow((char **)&w.smx, &w.smx_bytes, scale_bytes))) return rc;
+    if ((rc = grow((char **)&w.sex, &w.sex_bytes, scale_bytes))) return rc;
+    if (w.list_cap < n) {
+        if (w.list) TQ_HIP(hipFree(w.list));
+        w.list = nullptr;
+        w.list_cap = 0;
+        TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+        w.list_cap = n;
+    }
+    if (!w.count) {
+        TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
+        TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
+    }
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
+    Pools<float> pools{dd->cpool32, dd->spool32};
+    PlainScale sc{(uint32_t *)w.smx, (int32_t *)w.sex};
+    const int nout = 1 << cp.n_out;
+    if ((rc = record(p, w, 0, stream))) return rc;
+    for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+        const int64_t nb = std::min(chunk, n - s0);
+        const dim3 blk(256);
+        const dim3 grd((unsigned)((nb + 255) / 256));
+        hipLaunchKernelGGL(transpose_syndromes_kernel, grd, blk, 0, stream, d_syn + s0 * cp.n_syn, nb, cp.n_syn,
+                           w.synT, chunk);
+        for (const StepDesc &st : cp.steps) {
+            hipLaunchKernelGGL(zero_u32_kernel, grd, blk, 0, stream, sc.mx + (int64_t)st.c_slot * chunk, nb);
+            const unsigned ny = (unsigned)std::min<int64_t>(int64_t(1) << st.rc, 65535);
+            hipLaunchKernelGGL((plain_step_kernel<float, true>), dim3(grd.x, ny), blk, 0, stream, st,
+                               (float *)w.ws, chunk, nb, pools, (const uint8_t *)w.synT, sc);
+        }
+        hipLaunchKernelGGL(plain_finalize32_kernel, grd, blk, 0, stream, (const float *)w.ws, chunk, nb, cp.root_off,
+                           (const int32_t *)sc.ex + (int64_t)cp.root_slot * chunk, nout, p->tie_tol, p->recheck_tol,
+                           s0, d_mar, d_cls, d_tie, w.list, w.count);
+    }
+    if ((rc = record(p, w, 1, stream))) return rc;
+    TQ_HIP(hipMemcpyAsync(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    TQ_HIP(hipStreamSynchronize(stream));
+    const int64_t m = *w.host_count;
+    w.last_recheck = m;
+    if (m == 0) return TQEC_OK;
+    // FP64 re-check of the listed rows
+    if ((rc = grow(&w.g_syn, &w.g_syn_bytes, (size_t)m * std::max(1, cp.n_syn)))) return rc;
+    if ((rc = grow(&w.g_mar, &w.g_mar_bytes, (size_t)m * nout * sizeof(double)))) return rc;
+    if ((rc = grow(&w.g_cls, &w.g_cls_bytes, (size_t)m * sizeof(int32_t)))) return rc;
+    if ((rc = grow(&w.g_tie, &w.g_tie_bytes, (size_t)m))) return rc;
+    const int64_t tot = m * cp.n_syn;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, d_syn, cp.n_syn,
+                       w.list, m, w.g_syn);
+    const bool saved = p->timing;
+    p->timing = false;
+    rc = run_plain(p, dd, w, w.g_syn, m, w.g_mar, w.g_cls, w.g_tie, stream);
+    p->timing = saved;
+    if (rc) return rc;
+    hipLaunchKernelGGL(scatter_results_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, w.list, m, nout,
+                       w.g_mar, w.g_cls, w.g_tie, d_mar, d_cls, d_tie);
     TQ_HIP(hipGetLastError());
     return TQEC_OK;
 }
@@ -827,6 +1010,7 @@ int decode_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int mode, doubl
     if (wout) *wout = &w;
     if (n == 0) return TQEC_OK;
     if (mode == TQEC_MODE_PLAIN_F64) return run_plain(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+    if (mode == TQEC_MODE_WIDE_F32) return run_wide(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
     if (mode == TQEC_MODE_FUSED_F64 || mode == TQEC_MODE_FUSED_F32)
         return run_fused(p, dd, w, d_syn, n, mode, d_mar, d_cls, d_tie, stream);
     return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
@@ -870,6 +1054,8 @@ void tqec_plan_destroy(tqec_plan *p) {
         if (w.d_mar) (void)hipFree(w.d_mar);
         if (w.d_cls) (void)hipFree(w.d_cls);
         if (w.d_tie) (void)hipFree(w.d_tie);
+        for (void *q : {w.smx, w.sex, (void *)w.g_syn, (void *)w.g_mar, (void *)w.g_cls, (void *)w.g_tie})
+            if (q) (void)hipFree(q);
         for (auto &a : w.ev)
             for (hipEvent_t e : a) (void)hipEventDestroy(e);
     }
@@ -937,7 +1123,7 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
     if (n > 0 && !syn) return set_error(TQEC_ENULL, "null syndrome pointer");
     if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
-    if (mode < 0 || mode > 2) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+    if (mode < 0 || mode > 3) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
     if (p->cp.semiring != TQEC_SEMIRING_SUM)
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);

@@ -24,7 +24,9 @@ TQEC_ENULL = -5
 MODE_PLAIN_F64 = 0
 MODE_FUSED_F64 = 1
 MODE_FUSED_F32 = 2
-MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32}
+MODE_WIDE_F32 = 3
+MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32,
+         "wide_f32": MODE_WIDE_F32}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)

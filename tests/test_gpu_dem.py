@@ -63,3 +63,8 @@ def test_circuit_level_d3_dem_vs_oracle():
     nz = ref > 0
     assert np.all(r.marginals[~nz] == 0)
     assert np.max(np.abs(r.marginals[nz] - ref[nz]) / ref[nz]) < 1e-12
+    # the wide FP32 executor (too wide for LDS) with its FP64 re-check
+    w = D.decode_batch(ct, S, mode="wide_f32", error_patterns=False)
+    assert np.max(np.abs(w.marginals - ref) / ref.max(axis=1, keepdims=True)) < 2e-5
+    ok = np.abs(np.diff(np.sort(ref, axis=1)[:, -2:], axis=1))[:, 0] > 1e-12 * ref.max(axis=1)
+    assert np.array_equal(w.classes[ok], np.argmax(ref, axis=1)[ok])

@@ -26,7 +26,7 @@ from tensorqec.gf2 import logical_operator
 pytestmark = pytest.mark.gpu
 
 GOLDEN = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
-MODES = ["plain_f64", "fused_f64", "fused_f32"]
+MODES = ["plain_f64", "fused_f64", "fused_f32", "wide_f32"]
 NETS = [dict(factorize=False, evidence="leaves"), dict(factorize=True, evidence="slices")]
 
 
@@ -37,7 +37,7 @@ def say(*a):
 
 def check_marginals(m, ref, mode):
     ref = np.asarray(ref)
-    if mode == "fused_f32":
+    if mode.endswith("f32"):
         valid = ref.max(axis=1) > 0          # impossible syndromes: every class exactly 0
         assert np.all(m[~valid] == 0.0)
         err = np.max(np.abs(m[valid] - ref[valid]) / ref[valid].max(axis=1, keepdims=True))
@@ -74,7 +74,7 @@ def test_golden_all_syndromes(key, code, noise, mode, net):
     # impossible syndromes (all-zero rows) have no defined class; compare the rest
     valid = ref.max(axis=1) > 0
     check_classes(r.classes[valid], r.ties[valid], ref[valid])
-    if mode != "fused_f32":
+    if not mode.endswith("f32"):
         assert np.array_equal(r.ties[valid], O.tie_flags(ref[valid], 1e-12))
 
 
@@ -115,7 +115,7 @@ def test_seeded_batches_vs_c_oracle(code, p, net):
     ref, _, _ = OraclePlan(ct.plan).decode(S)
     lds_f64 = ct.hip.stats()["fused_lds_entries"] * 8
     for mode in MODES:
-        if mode != "plain_f64" and lds_f64 > 160 * 1024:
+        if mode.startswith("fused") and lds_f64 > 160 * 1024:
             # a tensor this wide cannot stay in one workgroup's LDS: the fused
             # executors refuse the plan instead of spilling
             with pytest.raises(L.TqecError) as e:
