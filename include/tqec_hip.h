@@ -211,6 +211,47 @@ void tqec_mlp_destroy(tqec_mlp *mlp);
 int tqec_posteriors_from_marginals(const double *d_marginals, int64_t n, int32_t n_classes, float *d_q,
                                    void *stream);
 
+/* ---- Mixed-code executor ------------------------------------------------
+ * Batches of syndromes of different codes (different plans) decoded by one
+ * persistent FP32 launch with the FP64 re-check behind it, as
+ * `decode(ct, syndrome)` (src/decoding/interfaces.jl:116-160) for a stream
+ * of compiled decoders of uneven size; each batch's results are those of
+ * tqec_decode_batch_device(plan, ..., TQEC_MODE_FUSED_F32, ...).  Plans whose
+ * intermediates exceed the LDS run from a per-workgroup slab of device
+ * memory.  Per-call tables are staged in pinned memory and, on a non-null
+ * stream, the call (table copy, counter reset, both kernels) is recorded
+ * once into a HIP graph and replayed.  Two calls may be in flight; a third
+ * waits for the first.  No reference counterpart: the reference decodes one
+ * syndrome of one code per call. */
+typedef struct tqec_mixed tqec_mixed;
+
+typedef struct {
+    int32_t plan;              /* index into the plans given to tqec_mixed_create */
+    int32_t reserved;
+    int64_t n;                 /* syndromes in this batch (0 allowed) */
+    const uint8_t *syndromes;  /* device [n][n_syndrome_bits of the plan] */
+    double *marginals;         /* device [n][2^n_out] or NULL */
+    int32_t *classes;          /* device [n] or NULL */
+    uint8_t *ties;             /* device [n] or NULL */
+} tqec_mixed_batch;
+
+/* Binds the plans to the current device; at most `max_batches` batches and
+ * `max_rows` syndromes in total per call.  The plans must outlive it. */
+int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batches, int64_t max_rows,
+                      tqec_mixed **out);
+/* Graph capture on (default) or off. */
+int tqec_mixed_set_graph(tqec_mixed *m, int32_t enable);
+/* Tile (syndromes per workgroup) chosen for a plan and where its
+ * intermediates live: bit 0 FP32 pass in a slab, bit 1 FP64 re-check in a slab. */
+int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int32_t *slab);
+/* Enqueue one call on `stream` (device pointers in the table). */
+int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int32_t n_batches, void *stream);
+/* After the last call: per batch, microseconds from the first tile's start
+ * to the batch's last result write on the device clock (0 for empty
+ * batches), and the number of FP64 re-checks.  Waits for the call. */
+int tqec_mixed_last_call(tqec_mixed *m, double *batch_us, int32_t n_batches, int64_t *recheck_count);
+void tqec_mixed_destroy(tqec_mixed *m);
+
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);
 

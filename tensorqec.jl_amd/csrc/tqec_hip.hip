@@ -298,6 +298,9 @@ struct FusedArgs {
     double recheck_tol;
     int32_t *recheck_list;    // FP32 pass: low-margin syndromes are appended here
     int32_t *recheck_count;
+    int32_t *recheck_batch;   // mixed-code executor: batch of each listed row
+    int64_t recheck_cap;      // capacity of the re-check list
+    int32_t batch_id;
 };
 
 // Base pointer of an operand for tile slot g of the fused executor.
@@ -337,14 +340,21 @@ __device__ __forceinline__ uint32_t max_bits(double v) { return __float_as_uint(
 
 // LDS layout: work[ws_entries][G] T | mx[nslots][G] u64 | ex[nslots][G] i32 |
 //             syn[nbits][G] u8 | lowf[G] i32
-// Thread t serves tile slot g = t mod G for the whole kernel.
-template <typename T, bool SCALE>
-__global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
-    extern __shared__ __align__(16) unsigned char smem[];
+// Thread t serves tile slot g = t mod G.  With GW (slab mode) the work array
+// lives instead in the workgroup's private slab of global memory `gwork`
+// (same [entry][G] layout) and the LDS holds only the scales and syndromes;
+// this serves plans whose intermediates exceed the LDS.
+//
+// One tile of G syndromes (rows b0.. of args.syn, or list entries b0.. of
+// args.list) through the whole step program.  Starts and ends with a
+// workgroup barrier, so consecutive tiles may reuse the LDS freely.
+template <typename T, bool SCALE, bool GW = false>
+__device__ void fused_tile(const FusedArgs &args, int64_t b0, int64_t count, uint32_t tag_base,
+                           unsigned char *smem, T *gwork = nullptr) {
     const int G = 1 << args.glog;
     const int glog = args.glog;
-    T *work = reinterpret_cast<T *>(smem);
-    size_t off = (size_t)args.ws_entries * G * sizeof(T);
+    T *work = GW ? gwork : reinterpret_cast<T *>(smem);
+    size_t off = GW ? 0 : (size_t)args.ws_entries * G * sizeof(T);
     off = (off + 15) & ~size_t(15);
     unsigned long long *mx = reinterpret_cast<unsigned long long *>(smem + off);
     int32_t *ex = reinterpret_cast<int32_t *>(smem + off + (SCALE ? (size_t)args.nslots * G * 8 : 0));
@@ -352,126 +362,297 @@ __global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
     int32_t *lowf = reinterpret_cast<int32_t *>(synl + (((size_t)args.nbits * G + 3) & ~size_t(3)));
     const T *cpool = reinterpret_cast<const T *>(args.cpool);
     const T *spool = reinterpret_cast<const T *>(args.spool);
-
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int g = tid & (G - 1);
     const int c0 = tid >> glog;
     const int cstep = nthr >> glog;
     const int lane = tid & 63;
-    const int64_t count = args.list ? (int64_t)(*args.list_count) : args.n;
-    const int64_t ntiles = (count + G - 1) >> glog;
-
-    if (SCALE) {
-        for (int i = tid; i < args.nslots * G; i += nthr) mx[i] = 0ull;
-    }
     // An operand whose largest entry is below 2^-100 means its step lost
     // more than 100 binary orders of magnitude: such syndromes are re-checked.
     constexpr int kLowExp = -100;
-    uint32_t iter = 0;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter) {
-        const int64_t b0 = t << glog;
-        // ---- syndrome bits of the tile -> LDS [bit][g] ----
-        if (!args.list) {
-            const int64_t rows = min((int64_t)G, count - b0);
-            const uint8_t *src = args.syn + b0 * args.nbits;
-            const int tot = G * args.nbits;
-            for (int i = tid; i < tot; i += nthr) {
-                const int r = i / args.nbits, bit = i - r * args.nbits;
-                synl[(bit << glog) + r] = (r < rows) ? (src[i] & 1u) : 0u;
-            }
-        } else {
-            const int tot = G * args.nbits;
-            for (int i = tid; i < tot; i += nthr) {
-                const int r = i / args.nbits, bit = i - r * args.nbits;
-                const int64_t q = b0 + r;
-                synl[(bit << glog) + r] =
-                    (q < count) ? (args.syn[(int64_t)args.list[q] * args.nbits + bit] & 1u) : 0u;
-            }
+    // ---- syndrome bits of the tile -> LDS [bit][g]; clear the scales ----
+    if (!args.list) {
+        const int64_t rows = min((int64_t)G, count - b0);
+        const uint8_t *src = args.syn + b0 * args.nbits;
+        const int tot = G * args.nbits;
+        for (int i = tid; i < tot; i += nthr) {
+            const int r = i / args.nbits, bit = i - r * args.nbits;
+            synl[(bit << glog) + r] = (r < rows) ? (src[i] & 1u) : 0u;
         }
-        if (SCALE && tid < G) lowf[tid] = 0;
-        __syncthreads();
-        const uint32_t tag_base = iter * (uint32_t)args.nsteps + 1u;
-        for (int s = 0; s < args.nsteps; ++s) {
-            const StepDesc &st = args.steps[s];
-            int ea = 0, eb = 0, shift = 0;
-            if (SCALE) {
-                int na = 0, nb = 0;
-                if (st.a.kind == kInter) {
-                    const float m = __uint_as_float((uint32_t)(mx[(st.a.slot << glog) + g] & 0xffffffffull));
-                    na = (m > 0.f) ? ilogbf(m) : 0;
-                    ea = ex[(st.a.slot << glog) + g] + na;
-                } else {
-                    ea = st.a.cexp;
-                }
-                if (st.b.kind == kInter) {
-                    const float m = __uint_as_float((uint32_t)(mx[(st.b.slot << glog) + g] & 0xffffffffull));
-                    nb = (m > 0.f) ? ilogbf(m) : 0;
-                    eb = ex[(st.b.slot << glog) + g] + nb;
-                } else {
-                    eb = st.b.cexp;
-                }
-                shift = -(na + nb);
-                if ((st.a.kind == kInter && na < kLowExp) || (st.b.kind == kInter && nb < kLowExp)) lowf[g] = 1;
-            }
-            const T *pa = fused_base(st.a, g, glog, work, cpool, spool, synl);
-            const T *pb = fused_base(st.b, g, glog, work, cpool, spool, synl);
-            T *pc = work + (st.c_off << glog) + g;
-            const bool aw = st.a.kind == kInter, bw = st.b.kind == kInter;
-            T lmax;
-            if (aw && bw) lmax = fused_loop<T, SCALE, true, true>(st, pa, pb, pc, glog, c0, cstep, shift);
-            else if (aw) lmax = fused_loop<T, SCALE, true, false>(st, pa, pb, pc, glog, c0, cstep, shift);
-            else if (bw) lmax = fused_loop<T, SCALE, false, true>(st, pa, pb, pc, glog, c0, cstep, shift);
-            else lmax = fused_loop<T, SCALE, false, false>(st, pa, pb, pc, glog, c0, cstep, shift);
-            if (SCALE) {
-                for (int o = G; o < 64; o <<= 1) {
-                    const float other = __shfl_xor((float)lmax, o);
-                    lmax = other > (float)lmax ? (T)other : lmax;
-                }
-                if (lane < G) {
-                    const unsigned long long tagged =
-                        ((unsigned long long)(tag_base + (uint32_t)s) << 32) | (unsigned long long)max_bits(lmax);
-                    atomicMax(&mx[(st.c_slot << glog) + g], tagged);
-                }
-                if (tid < G) ex[(st.c_slot << glog) + g] = ea + eb;
-            }
-            __syncthreads();
+    } else {
+        const int tot = G * args.nbits;
+        for (int i = tid; i < tot; i += nthr) {
+            const int r = i / args.nbits, bit = i - r * args.nbits;
+            const int64_t q = b0 + r;
+            synl[(bit << glog) + r] = (q < count) ? (args.syn[(int64_t)args.list[q] * args.nbits + bit] & 1u) : 0u;
         }
-        // ---- per-syndrome result ----
-        if (tid < G) {
-            const int64_t q = b0 + tid;
-            if (q < count) {
-                const int64_t b = args.list ? (int64_t)args.list[q] : q;
-                const int e = SCALE ? ex[(args.root_slot << glog) + g] : 0;
-                T best = 0, second = -1;
-                int bi = 0;
-                for (int j = 0; j < args.nout; ++j) {
-                    const T x = work[((args.root_off + j) << glog) + g];
-                    if (args.marginals) args.marginals[b * args.nout + j] = ldexp((double)x, e);
-                    if (j == 0 || x > best) {
-                        if (j > 0) second = best;
-                        best = x;
-                        bi = j;
-                    } else if (x > second) {
-                        second = x;
-                    }
-                }
-                if (args.classes) args.classes[b] = bi;
-                const bool low = args.nout > 1 && second >= 0 && (double)(best - second) <= args.tie_tol * (double)best;
-                if (args.ties) args.ties[b] = low ? 1 : 0;
-                if (SCALE && args.recheck_list) {
-                    const bool rc = !(best > 0) || lowf[tid] != 0 ||
-                                    (args.nout > 1 && second >= 0 &&
-                                     (double)(best - second) <= args.recheck_tol * (double)best);
-                    if (rc) {
-                        const int32_t slot = atomicAdd(args.recheck_count, 1);
-                        args.recheck_list[slot] = (int32_t)b;
-                    }
-                }
+    }
+    if (SCALE) {
+        for (int i = tid; i < args.nslots * G; i += nthr) mx[i] = 0ull;
+        if (tid < G) lowf[tid] = 0;
+    }
+    __syncthreads();
+    for (int s = 0; s < args.nsteps; ++s) {
+        const StepDesc &st = args.steps[s];
+        int ea = 0, eb = 0, shift = 0;
+        if (SCALE) {
+            int na = 0, nb = 0;
+            if (st.a.kind == kInter) {
+                const float m = __uint_as_float((uint32_t)(mx[(st.a.slot << glog) + g] & 0xffffffffull));
+                na = (m > 0.f) ? ilogbf(m) : 0;
+                ea = ex[(st.a.slot << glog) + g] + na;
+            } else {
+                ea = st.a.cexp;
             }
+            if (st.b.kind == kInter) {
+                const float m = __uint_as_float((uint32_t)(mx[(st.b.slot << glog) + g] & 0xffffffffull));
+                nb = (m > 0.f) ? ilogbf(m) : 0;
+                eb = ex[(st.b.slot << glog) + g] + nb;
+            } else {
+                eb = st.b.cexp;
+            }
+            shift = -(na + nb);
+            if ((st.a.kind == kInter && na < kLowExp) || (st.b.kind == kInter && nb < kLowExp)) lowf[g] = 1;
+        }
+        const T *pa = fused_base(st.a, g, glog, work, cpool, spool, synl);
+        const T *pb = fused_base(st.b, g, glog, work, cpool, spool, synl);
+        T *pc = work + (st.c_off << glog) + g;
+        const bool aw = st.a.kind == kInter, bw = st.b.kind == kInter;
+        T lmax;
+        if (aw && bw) lmax = fused_loop<T, SCALE, true, true>(st, pa, pb, pc, glog, c0, cstep, shift);
+        else if (aw) lmax = fused_loop<T, SCALE, true, false>(st, pa, pb, pc, glog, c0, cstep, shift);
+        else if (bw) lmax = fused_loop<T, SCALE, false, true>(st, pa, pb, pc, glog, c0, cstep, shift);
+        else lmax = fused_loop<T, SCALE, false, false>(st, pa, pb, pc, glog, c0, cstep, shift);
+        if (SCALE) {
+            for (int o = G; o < 64; o <<= 1) {
+                const float other = __shfl_xor((float)lmax, o);
+                lmax = other > (float)lmax ? (T)other : lmax;
+            }
+            if (lane < G) {
+                const unsigned long long tagged =
+                    ((unsigned long long)(tag_base + (uint32_t)s) << 32) | (unsigned long long)max_bits(lmax);
+                atomicMax(&mx[(st.c_slot << glog) + g], tagged);
+            }
+            if (tid < G) ex[(st.c_slot << glog) + g] = ea + eb;
         }
         __syncthreads();
     }
+    // ---- per-syndrome result ----
+    if (tid < G) {
+        const int64_t q = b0 + tid;
+        if (q < count) {
+            const int64_t b = args.list ? (int64_t)args.list[q] : q;
+            const int e = SCALE ? ex[(args.root_slot << glog) + g] : 0;
+            T best = 0, second = -1;
+            int bi = 0;
+            for (int j = 0; j < args.nout; ++j) {
+                const T x = work[((args.root_off + j) << glog) + g];
+                if (args.marginals) args.marginals[b * args.nout + j] = ldexp((double)x, e);
+                if (j == 0 || x > best) {
+                    if (j > 0) second = best;
+                    best = x;
+                    bi = j;
+                } else if (x > second) {
+                    second = x;
+                }
+            }
+            if (args.classes) args.classes[b] = bi;
+            const bool low = args.nout > 1 && second >= 0 && (double)(best - second) <= args.tie_tol * (double)best;
+            if (args.ties) args.ties[b] = low ? 1 : 0;
+            if (SCALE && args.recheck_list) {
+                const bool rc = !(best > 0) || lowf[tid] != 0 ||
+                                (args.nout > 1 && second >= 0 &&
+                                 (double)(best - second) <= args.recheck_tol * (double)best);
+                if (rc) {
+                    const int32_t slot = atomicAdd(args.recheck_count, 1);
+                    if (slot < args.recheck_cap) {
+                        args.recheck_list[slot] = (int32_t)b;
+                        if (args.recheck_batch) args.recheck_batch[slot] = args.batch_id;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Persistent single-plan kernel: workgroups stride over the tiles.
+template <typename T, bool SCALE>
+__global__ void __launch_bounds__(256) fused_kernel(FusedArgs args) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t count = args.list ? (int64_t)(*args.list_count) : args.n;
+    const int64_t ntiles = (count + (1 << args.glog) - 1) >> args.glog;
+    uint32_t iter = 0;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter)
+        fused_tile<T, SCALE>(args, t << args.glog, count, iter * (uint32_t)args.nsteps + 1u, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-code executor: batches of different codes in one persistent launch
+// ---------------------------------------------------------------------------
+
+struct MixedPlan {
+    const StepDesc *steps;
+    const void *cpool32, *spool32, *cpool64, *spool64;
+    int64_t ws_entries, root_off;
+    int nsteps, nbits, nout, nslots, root_slot;
+    int glog32;          // largest FP32 tile of the plan (a batch may use a smaller one)
+    int gw32, gw64;      // 1: intermediates in the workgroup's global slab (FP32 pass / FP64 re-check)
+    int pad;
+    double tie_tol, recheck_tol;
+};
+
+struct MixedBatch {
+    const uint8_t *syn;  // [n][nbits] on the device
+    double *mar;
+    int32_t *cls;
+    uint8_t *tie;
+    int64_t n;
+    int64_t tile0;       // first global tile of this batch
+    int plan;
+    int glog;            // FP32 tile of this batch
+};
+
+// Device-resident call header: written by the host before each call (a
+// captured graph replays the same kernel arguments, so everything that
+// varies between calls lives here).
+struct MixedHeader {
+    int32_t nbatches;
+    int32_t pad;
+    int64_t ntiles;
+};
+
+// Per-call counters, cleared by mixed_reset_kernel at the start of a call.
+struct MixedCounters {
+    unsigned long long next_tile;       // FP32 work queue
+    unsigned long long next_recheck;    // FP64 work queue
+    int32_t rl_count;                   // re-check list length
+    int32_t pad;
+    unsigned long long t_start;         // device wall clock at the first tile
+};
+
+struct MixedArgs {
+    const MixedPlan *plans;
+    const MixedBatch *batches;
+    const MixedHeader *hdr;
+    MixedCounters *ctr;
+    int32_t *rl_row, *rl_batch;          // FP64 re-check list (row, batch), capacity rl_cap
+    int64_t rl_cap;
+    unsigned long long *t_done;          // per batch: device wall clock of its last write
+    char *slab32, *slab64;               // per-workgroup global slabs
+    int64_t slab32_bytes, slab64_bytes;  // bytes per workgroup
+};
+
+__device__ __forceinline__ FusedArgs mixed_fused_args(const MixedArgs &m, const MixedPlan &p, const MixedBatch &bt,
+                                                      bool f32, int glog) {
+    FusedArgs a;
+    a.steps = p.steps;
+    a.nsteps = p.nsteps;
+    a.cpool = f32 ? p.cpool32 : p.cpool64;
+    a.spool = f32 ? p.spool32 : p.spool64;
+    a.syn = bt.syn;
+    a.n = bt.n;
+    a.nbits = p.nbits;
+    a.glog = glog;
+    a.nout = p.nout;
+    a.ws_entries = p.ws_entries;
+    a.nslots = p.nslots;
+    a.root_off = p.root_off;
+    a.root_slot = p.root_slot;
+    a.list = nullptr;
+    a.list_count = nullptr;
+    a.marginals = bt.mar;
+    a.classes = bt.cls;
+    a.ties = bt.tie;
+    a.tie_tol = p.tie_tol;
+    a.recheck_tol = p.recheck_tol;
+    a.recheck_list = nullptr;
+    a.recheck_count = nullptr;
+    a.recheck_batch = nullptr;
+    a.recheck_cap = 0;
+    a.batch_id = 0;
+    return a;
+}
+
+// Next item of a work queue, broadcast to the workgroup through LDS.
+__device__ __forceinline__ int64_t mixed_grab(unsigned long long *queue, int64_t *slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = (int64_t)atomicAdd(queue, 1ull);
+    __syncthreads();
+    return *slot;
+}
+
+// FP32 pass.  Workgroups pull global tiles from a queue; tile t belongs to
+// the batch whose [tile0, next tile0) range holds it.  All plans share one
+// LDS allocation; slab plans use the workgroup's slab instead.
+__global__ void __launch_bounds__(256) mixed_kernel_f32(MixedArgs m) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int64_t grab;
+    const int nb = m.hdr->nbatches;
+    const int64_t ntiles = m.hdr->ntiles;
+    uint32_t tag = 1;
+    bool first = true;
+    for (;;) {
+        const int64_t t = mixed_grab(&m.ctr->next_tile, &grab);
+        if (t >= ntiles) break;
+        if (first && threadIdx.x == 0) atomicMin(&m.ctr->t_start, (unsigned long long)wall_clock64());
+        first = false;
+        int lo = 0, hi = nb - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (m.batches[mid].tile0 <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const MixedBatch bt = m.batches[lo];
+        const MixedPlan &p = m.plans[bt.plan];
+        FusedArgs a = mixed_fused_args(m, p, bt, true, bt.glog);
+        a.recheck_list = m.rl_row;
+        a.recheck_count = &m.ctr->rl_count;
+        a.recheck_batch = m.rl_batch;
+        a.recheck_cap = m.rl_cap;
+        a.batch_id = lo;
+        const int64_t b0 = (t - bt.tile0) << bt.glog;
+        if (p.gw32)
+            fused_tile<float, true, true>(a, b0, bt.n, tag, smem,
+                                          reinterpret_cast<float *>(m.slab32 + (int64_t)blockIdx.x * m.slab32_bytes));
+        else
+            fused_tile<float, true, false>(a, b0, bt.n, tag, smem);
+        tag += (uint32_t)p.nsteps;
+        if (threadIdx.x == 0) atomicMax(&m.t_done[lo], (unsigned long long)wall_clock64());
+    }
+}
+
+// FP64 re-check of the listed (batch, row) pairs, one syndrome per tile.
+__global__ void __launch_bounds__(256) mixed_kernel_f64(MixedArgs m) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int64_t grab;
+    const int64_t count = min((int64_t)m.ctr->rl_count, m.rl_cap);
+    for (;;) {
+        const int64_t e = mixed_grab(&m.ctr->next_recheck, &grab);
+        if (e >= count) break;
+        const int bi = m.rl_batch[e];
+        const MixedBatch bt = m.batches[bi];
+        const MixedPlan &p = m.plans[bt.plan];
+        FusedArgs a = mixed_fused_args(m, p, bt, false, 0);
+        if (p.gw64)
+            fused_tile<double, false, true>(a, m.rl_row[e], bt.n, 1u, smem,
+                                            reinterpret_cast<double *>(m.slab64 + (int64_t)blockIdx.x * m.slab64_bytes));
+        else
+            fused_tile<double, false, false>(a, m.rl_row[e], bt.n, 1u, smem);
+        if (threadIdx.x == 0) atomicMax(&m.t_done[bi], (unsigned long long)wall_clock64());
+    }
+}
+
+__global__ void mixed_reset_kernel(MixedCounters *ctr, unsigned long long *t_done, int cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctr->next_tile = 0ull;
+        ctr->next_recheck = 0ull;
+        ctr->rl_count = 0;
+        ctr->t_start = ~0ull;
+    }
+    if (i < cap) t_done[i] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -703,14 +884,21 @@ int ensure_device(tqec_plan *p, int device, DeviceData **out) {
     return TQEC_OK;
 }
 
-size_t fused_lds_bytes(const CompiledPlan &cp, int glog, size_t elem, bool scale) {
+// LDS of a fused tile besides the work array: scales, syndrome bits, flags.
+size_t fused_lds_aux(const CompiledPlan &cp, int glog, bool scale) {
     const size_t G = size_t(1) << glog;
-    size_t b = (size_t)cp.ws_entries * G * elem;
-    b = (b + 15) & ~size_t(15);
+    size_t b = 0;
     if (scale) b += (size_t)cp.n_slots * G * 12;
     b += ((size_t)cp.n_syn * G + 3) & ~size_t(3);
     if (scale) b += G * sizeof(int32_t);
     return b;
+}
+
+size_t fused_lds_bytes(const CompiledPlan &cp, int glog, size_t elem, bool scale) {
+    const size_t G = size_t(1) << glog;
+    size_t b = (size_t)cp.ws_entries * G * elem;
+    b = (b + 15) & ~size_t(15);
+    return b + fused_lds_aux(cp, glog, scale);
 }
 
 // Workgroups per CU a configuration can keep resident (LDS and wave slots).
@@ -975,6 +1163,7 @@ int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, 
     a.ties = d_tie;
     a.recheck_list = w.list;
     a.recheck_count = w.count;
+    a.recheck_cap = w.list_cap;
     const int64_t ntiles = (n + (1 << c32.glog) - 1) >> c32.glog;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, c32.blocks));
     if ((rc = record(p, w, 0, stream))) return rc;
@@ -1014,6 +1203,110 @@ int decode_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int mode, doubl
     if (mode == TQEC_MODE_FUSED_F64 || mode == TQEC_MODE_FUSED_F32)
         return run_fused(p, dd, w, d_syn, n, mode, d_mar, d_cls, d_tie, stream);
     return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+}
+
+// Mixed-code executor state.  Each of two call slots owns the per-call
+// tables (header, batch table, counters, batch clocks, re-check list) and a
+// pinned host staging copy of header + batch table, so a call can be staged
+// while the previous one still runs; the slabs are shared because calls on
+// one stream execute in order.
+struct MixedSlot {
+    void *h_stage = nullptr;            // pinned: MixedHeader | MixedBatch[max_batches]
+    void *d_table = nullptr;            // device copy of the same
+    MixedCounters *ctr = nullptr;
+    unsigned long long *t_done = nullptr;
+    int32_t *rl_row = nullptr, *rl_batch = nullptr;
+    hipEvent_t staged = nullptr;        // the device has consumed h_stage
+    hipStream_t graph_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t nbatches = 0;
+    bool used = false;
+};
+
+struct MixedPlanCfg {
+    int glog32 = 0;
+    bool gw32 = false, gw64 = false;
+    size_t lds32 = 0, lds64 = 0;        // dynamic LDS of the plan's tiles
+    size_t slab32 = 0, slab64 = 0;      // slab bytes per workgroup (slab plans)
+};
+
+constexpr int kMixedThreads = 256;
+constexpr size_t kMixedSlabBudget = size_t(4) << 20;   // FP32 slab per workgroup before shrinking the tile
+
+}  // namespace
+
+struct tqec_mixed {
+    std::vector<tqec_plan *> plans;
+    std::vector<MixedPlanCfg> cfg;
+    int device = -1;
+    int32_t max_batches = 0;
+    int64_t max_rows = 0;
+    size_t lds32 = 0, lds64 = 0;
+    int blocks32 = 0, blocks64 = 0;
+    MixedPlan *d_plans = nullptr;
+    char *slab32 = nullptr, *slab64 = nullptr;
+    size_t slab32_bytes = 0, slab64_bytes = 0;
+    std::array<MixedSlot, 2> slot;
+    int cur = 0;                        // slot of the next call
+    int last = -1;                      // slot of the last call
+    bool use_graph = true;
+    double clock_khz = 0;
+};
+
+namespace {
+
+size_t mixed_table_bytes(int32_t max_batches) {
+    return sizeof(MixedHeader) + (size_t)std::max(1, max_batches) * sizeof(MixedBatch);
+}
+
+MixedArgs mixed_args(tqec_mixed *m, MixedSlot &sl) {
+    MixedArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.plans = m->d_plans;
+    a.hdr = reinterpret_cast<const MixedHeader *>(sl.d_table);
+    a.batches = reinterpret_cast<const MixedBatch *>(reinterpret_cast<const char *>(sl.d_table) + sizeof(MixedHeader));
+    a.ctr = sl.ctr;
+    a.rl_row = sl.rl_row;
+    a.rl_batch = sl.rl_batch;
+    a.rl_cap = m->max_rows;
+    a.t_done = sl.t_done;
+    a.slab32 = m->slab32;
+    a.slab64 = m->slab64;
+    a.slab32_bytes = (int64_t)m->slab32_bytes;
+    a.slab64_bytes = (int64_t)m->slab64_bytes;
+    return a;
+}
+
+// The whole call: stage the tables, clear the counters, FP32 pass, FP64
+// re-check.  Enqueued directly or recorded once into a graph per slot.
+int mixed_enqueue(tqec_mixed *m, MixedSlot &sl, hipStream_t stream) {
+    const size_t tb = mixed_table_bytes(m->max_batches);
+    TQ_HIP(hipMemcpyAsync(sl.d_table, sl.h_stage, tb, hipMemcpyHostToDevice, stream));
+    const int cap = std::max(1, m->max_batches);
+    hipLaunchKernelGGL(mixed_reset_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, stream, sl.ctr, sl.t_done,
+                       cap);
+    MixedArgs a = mixed_args(m, sl);
+    hipLaunchKernelGGL(mixed_kernel_f32, dim3((unsigned)m->blocks32), dim3(kMixedThreads), m->lds32, stream, a);
+    hipLaunchKernelGGL(mixed_kernel_f64, dim3((unsigned)m->blocks64), dim3(kMixedThreads), m->lds64, stream, a);
+    TQ_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+void mixed_free(tqec_mixed *m) {
+    if (m->device >= 0) (void)hipSetDevice(m->device);
+    for (MixedSlot &sl : m->slot) {
+        if (sl.staged) (void)hipEventSynchronize(sl.staged);
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        if (sl.staged) (void)hipEventDestroy(sl.staged);
+        if (sl.h_stage) (void)hipHostFree(sl.h_stage);
+        for (void *q : {sl.d_table, (void *)sl.ctr, (void *)sl.t_done, (void *)sl.rl_row, (void *)sl.rl_batch})
+            if (q) (void)hipFree(q);
+    }
+    if (m->d_plans) (void)hipFree(m->d_plans);
+    if (m->slab32) (void)hipFree(m->slab32);
+    if (m->slab64) (void)hipFree(m->slab64);
 }
 
 }  // namespace
@@ -1405,5 +1698,252 @@ int tqec_kernel_time_totals(tqec_plan *p, void *stream, double *main_ms, double 
 }
 
 const char *tqec_last_error(void) { return g_last_error.c_str(); }
+
+
+int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batches, int64_t max_rows,
+                      tqec_mixed **out) {
+    if (!out || !plans) return set_error(TQEC_ENULL, "null argument");
+    *out = nullptr;
+    if (n_plans < 1) return set_error(TQEC_EINVAL, "need at least one plan");
+    if (max_batches < 1) return set_error(TQEC_EINVAL, "max_batches must be positive");
+    if (max_rows < 1 || max_rows > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "max_rows out of range");
+    for (int i = 0; i < n_plans; ++i) {
+        if (!plans[i]) return set_error(TQEC_ENULL, "null plan " + std::to_string(i));
+        if (plans[i]->cp.semiring != TQEC_SEMIRING_SUM)
+            return set_error(TQEC_EINVAL, "plan " + std::to_string(i) + " is a max-product plan");
+    }
+    std::unique_ptr<tqec_mixed> m(new (std::nothrow) tqec_mixed());
+    if (!m) return set_error(TQEC_ENOMEM, "out of host memory");
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    m->device = dev;
+    m->plans.assign(plans, plans + n_plans);
+    m->max_batches = max_batches;
+    m->max_rows = max_rows;
+    int khz = 0;
+    TQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    m->clock_khz = khz > 0 ? (double)khz : 100000.0;
+    size_t lds_limit = 0;
+    int n_cu = 0;
+    std::vector<MixedPlan> hp(n_plans);
+    int rc;
+    for (int i = 0; i < n_plans; ++i) {
+        tqec_plan *p = plans[i];
+        std::lock_guard<std::mutex> lk(p->mu);
+        if ((rc = configure(p))) return rc;
+        DeviceData *dd;
+        if ((rc = ensure_device(p, dev, &dd))) return rc;
+        lds_limit = p->lds_limit;
+        n_cu = p->n_cu;
+        const CompiledPlan &cp = p->cp;
+        const size_t budget = lds_limit / 2;
+        MixedPlanCfg c;
+        int gl = p->f32.glog >= 0 ? std::min(p->f32.glog, 6) : 6;
+        while (gl >= 0 && fused_lds_bytes(cp, gl, sizeof(float), true) > budget) --gl;
+        if (gl >= 0) {
+            c.glog32 = gl;
+            c.lds32 = fused_lds_bytes(cp, gl, sizeof(float), true);
+        } else {
+            // slab mode: the LDS keeps the scales and syndrome bits only
+            c.gw32 = true;
+            const size_t ws = (size_t)std::max<int64_t>(1, cp.ws_entries);
+            gl = 6;
+            while (gl > 0 && ((ws * sizeof(float)) << gl > kMixedSlabBudget || fused_lds_aux(cp, gl, true) > budget))
+                --gl;
+            c.glog32 = gl;
+            c.lds32 = fused_lds_aux(cp, gl, true);
+            c.slab32 = (((ws << gl) * sizeof(float)) + 255) & ~size_t(255);
+        }
+        const size_t l64 = fused_lds_bytes(cp, 0, sizeof(double), false);
+        if (l64 <= budget) {
+            c.lds64 = l64;
+        } else {
+            c.gw64 = true;
+            const size_t ws = (size_t)std::max<int64_t>(1, cp.ws_entries);
+            c.lds64 = fused_lds_aux(cp, 0, false);
+            c.slab64 = (ws * sizeof(double) + 255) & ~size_t(255);
+        }
+        m->cfg.push_back(c);
+        MixedPlan &q = hp[i];
+        std::memset(&q, 0, sizeof(q));
+        q.steps = dd->steps;
+        q.cpool32 = dd->cpool32;
+        q.spool32 = dd->spool32;
+        q.cpool64 = dd->cpool64;
+        q.spool64 = dd->spool64;
+        q.ws_entries = cp.ws_entries;
+        q.root_off = cp.root_off;
+        q.nsteps = (int)cp.steps.size();
+        q.nbits = cp.n_syn;
+        q.nout = 1 << cp.n_out;
+        q.nslots = cp.n_slots;
+        q.root_slot = cp.root_slot;
+        q.glog32 = c.glog32;
+        q.gw32 = c.gw32;
+        q.gw64 = c.gw64;
+        q.tie_tol = p->tie_tol;
+        q.recheck_tol = p->recheck_tol;
+        m->lds32 = std::max(m->lds32, c.lds32);
+        m->lds64 = std::max(m->lds64, c.lds64);
+        m->slab32_bytes = std::max(m->slab32_bytes, c.slab32);
+        m->slab64_bytes = std::max(m->slab64_bytes, c.slab64);
+    }
+    if (m->lds32 > lds_limit || m->lds64 > lds_limit)
+        return set_error(TQEC_EUNSUPPORTED, "a plan's scales and syndrome bits exceed the LDS");
+    TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel_f32, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_limit));
+    TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel_f64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_limit));
+    m->blocks32 = n_cu * resident_per_cu(m->lds32 + 64, kMixedThreads, lds_limit);
+    m->blocks64 = n_cu * resident_per_cu(m->lds64 + 64, kMixedThreads, lds_limit);
+    tqec_mixed *mp = m.get();
+    auto fail = [&](int code) {
+        mixed_free(mp);
+        m.reset();
+        return code;
+    };
+#define TQ_MX(call)                                                                               \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(set_error(TQEC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    TQ_MX(hipMalloc((void **)&mp->d_plans, hp.size() * sizeof(MixedPlan)));
+    TQ_MX(hipMemcpy(mp->d_plans, hp.data(), hp.size() * sizeof(MixedPlan), hipMemcpyHostToDevice));
+    if (mp->slab32_bytes) TQ_MX(hipMalloc((void **)&mp->slab32, mp->slab32_bytes * (size_t)mp->blocks32));
+    if (mp->slab64_bytes) TQ_MX(hipMalloc((void **)&mp->slab64, mp->slab64_bytes * (size_t)mp->blocks64));
+    const size_t tb = mixed_table_bytes(max_batches);
+    for (MixedSlot &sl : mp->slot) {
+        TQ_MX(hipHostMalloc(&sl.h_stage, tb, hipHostMallocDefault));
+        std::memset(sl.h_stage, 0, tb);
+        TQ_MX(hipMalloc(&sl.d_table, tb));
+        TQ_MX(hipMalloc((void **)&sl.ctr, sizeof(MixedCounters)));
+        TQ_MX(hipMalloc((void **)&sl.t_done, (size_t)max_batches * sizeof(unsigned long long)));
+        TQ_MX(hipMalloc((void **)&sl.rl_row, (size_t)max_rows * sizeof(int32_t)));
+        TQ_MX(hipMalloc((void **)&sl.rl_batch, (size_t)max_rows * sizeof(int32_t)));
+        TQ_MX(hipEventCreateWithFlags(&sl.staged, hipEventDisableTiming));
+    }
+#undef TQ_MX
+    *out = m.release();
+    return TQEC_OK;
+}
+
+int tqec_mixed_set_graph(tqec_mixed *m, int32_t enable) {
+    if (!m) return set_error(TQEC_ENULL, "null executor");
+    m->use_graph = enable != 0;
+    return TQEC_OK;
+}
+
+int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int32_t *slab) {
+    if (!m) return set_error(TQEC_ENULL, "null executor");
+    if (plan < 0 || plan >= (int32_t)m->cfg.size()) return set_error(TQEC_EINVAL, "plan index out of range");
+    if (tile) *tile = 1 << m->cfg[plan].glog32;
+    if (slab) *slab = (m->cfg[plan].gw32 ? 1 : 0) | (m->cfg[plan].gw64 ? 2 : 0);
+    return TQEC_OK;
+}
+
+int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int32_t n_batches, void *stream) {
+    if (!m) return set_error(TQEC_ENULL, "null executor");
+    if (n_batches < 0 || n_batches > m->max_batches)
+        return set_error(TQEC_EINVAL, "n_batches out of range [0, " + std::to_string(m->max_batches) + "]");
+    if (n_batches > 0 && !batches) return set_error(TQEC_ENULL, "null batch table");
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    if (dev != m->device) return set_error(TQEC_EINVAL, "executor was created on another device");
+    int64_t rows = 0;
+    for (int32_t i = 0; i < n_batches; ++i) {
+        const tqec_mixed_batch &b = batches[i];
+        if (b.plan < 0 || b.plan >= (int32_t)m->plans.size())
+            return set_error(TQEC_EINVAL, "batch " + std::to_string(i) + ": plan index out of range");
+        if (b.n < 0) return set_error(TQEC_EINVAL, "batch " + std::to_string(i) + ": negative size");
+        if (b.n > 0 && !b.syndromes) return set_error(TQEC_ENULL, "batch " + std::to_string(i) + ": null syndromes");
+        rows += b.n;
+        if (rows > m->max_rows)
+            return set_error(TQEC_EINVAL, "batches hold more than max_rows = " + std::to_string(m->max_rows) + " syndromes");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    MixedSlot &sl = m->slot[m->cur];
+    if (sl.used) TQ_HIP(hipEventSynchronize(sl.staged));
+    char *h = reinterpret_cast<char *>(sl.h_stage);
+    MixedHeader *hdr = reinterpret_cast<MixedHeader *>(h);
+    MixedBatch *tb = reinterpret_cast<MixedBatch *>(h + sizeof(MixedHeader));
+    int64_t tiles = 0;
+    for (int32_t i = 0; i < n_batches; ++i) {
+        const tqec_mixed_batch &b = batches[i];
+        int gl = 0;
+        while (gl < m->cfg[b.plan].glog32 && (int64_t(1) << gl) < b.n) ++gl;
+        MixedBatch &q = tb[i];
+        q.syn = b.syndromes;
+        q.mar = b.marginals;
+        q.cls = b.classes;
+        q.tie = b.ties;
+        q.n = b.n;
+        q.tile0 = tiles;
+        q.plan = b.plan;
+        q.glog = gl;
+        tiles += (b.n + (int64_t(1) << gl) - 1) >> gl;
+    }
+    hdr->nbatches = n_batches;
+    hdr->pad = 0;
+    hdr->ntiles = tiles;
+    int rc;
+    if (m->use_graph && st != nullptr) {
+        if (!sl.exec || sl.graph_stream != st) {
+            if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+            if (sl.graph) (void)hipGraphDestroy(sl.graph);
+            sl.exec = nullptr;
+            sl.graph = nullptr;
+            TQ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = mixed_enqueue(m, sl, st);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return set_error(TQEC_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            sl.graph = g;
+            TQ_HIP(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+            sl.graph_stream = st;
+        }
+        TQ_HIP(hipGraphLaunch(sl.exec, st));
+    } else {
+        if ((rc = mixed_enqueue(m, sl, st))) return rc;
+    }
+    TQ_HIP(hipEventRecord(sl.staged, st));
+    sl.used = true;
+    sl.nbatches = n_batches;
+    m->last = m->cur;
+    m->cur ^= 1;
+    return TQEC_OK;
+}
+
+int tqec_mixed_last_call(tqec_mixed *m, double *batch_us, int32_t n_batches, int64_t *recheck_count) {
+    if (!m) return set_error(TQEC_ENULL, "null executor");
+    if (m->last < 0) return set_error(TQEC_EINVAL, "no call yet");
+    MixedSlot &sl = m->slot[m->last];
+    if (batch_us && n_batches != sl.nbatches)
+        return set_error(TQEC_EINVAL, "the last call had " + std::to_string(sl.nbatches) + " batches");
+    TQ_HIP(hipSetDevice(m->device));
+    TQ_HIP(hipEventSynchronize(sl.staged));
+    MixedCounters c;
+    TQ_HIP(hipMemcpy(&c, sl.ctr, sizeof(c), hipMemcpyDeviceToHost));
+    if (recheck_count) *recheck_count = c.rl_count;
+    if (batch_us && n_batches > 0) {
+        std::vector<unsigned long long> t(n_batches);
+        TQ_HIP(hipMemcpy(t.data(), sl.t_done, (size_t)n_batches * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < n_batches; ++i)
+            batch_us[i] = (t[i] == 0 || c.t_start == ~0ull || t[i] < c.t_start)
+                              ? 0.0
+                              : (double)(t[i] - c.t_start) * 1000.0 / m->clock_khz;
+    }
+    return TQEC_OK;
+}
+
+void tqec_mixed_destroy(tqec_mixed *m) {
+    if (!m) return;
+    mixed_free(m);
+    delete m;
+}
 
 }  // extern "C"

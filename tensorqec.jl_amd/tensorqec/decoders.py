@@ -182,6 +182,13 @@ def compile(decoder, tanner, em=None):
         em = iid_error(0.05, 0.05, 0.05, tanner.nq)
     lx, lz = logical_operator(tanner)
     graph = tnmmap_css_network(tanner, em, factorize=decoder.factorize, lxlz=(lx, lz))
+    opt = decoder.optimizer
+    if isinstance(opt, TreeSA) and opt.primary is None:
+        # extra candidate: sweep the qubits in index order, x and z error of a
+        # qubit together (a row-by-row sweep of a planar layout)
+        from dataclasses import replace
+        n = tanner.nq
+        decoder = replace(decoder, optimizer=replace(opt, primary=tuple(v for q in range(n) for v in (q, q + n))))
     plan = build_plan(graph, decoder)
     tab = decoder.tabulate_bits
     if tab is None:

@@ -77,6 +77,18 @@ class Stats(ctypes.Structure):
     ]
 
 
+class MixedBatch(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+        ("syndromes", ctypes.c_void_p),
+        ("marginals", ctypes.c_void_p),
+        ("classes", ctypes.c_void_p),
+        ("ties", ctypes.c_void_p),
+    ]
+
+
 class TqecError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"tqec error {code}: {msg}")
@@ -146,6 +158,15 @@ EXPORTS = [
     ("tqec_mlp_destroy", None, [ctypes.c_void_p]),
     ("tqec_posteriors_from_marginals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                                        ctypes.c_void_p, ctypes.c_void_p]),
+    ("tqec_mixed_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_mixed_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("tqec_mixed_plan_config", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p]),
+    ("tqec_mixed_decode_device", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MixedBatch), ctypes.c_int32,
+                                                 ctypes.c_void_p]),
+    ("tqec_mixed_last_call", ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int32,
+                                             ctypes.POINTER(ctypes.c_int64)]),
+    ("tqec_mixed_destroy", None, [ctypes.c_void_p]),
     ("tqec_last_error", ctypes.c_char_p, []),
 ]
 
@@ -413,3 +434,55 @@ class HipRecovery:
             except Exception:
                 pass
             self._h = None
+
+
+class HipMixed:
+    """Owner of a ``tqec_mixed*``: batches of several compiled plans decoded
+    by one persistent launch (FP32 with FP64 re-check), optionally replayed
+    from a captured HIP graph."""
+
+    def __init__(self, plans, max_batches: int, max_rows: int, graph: bool = True):
+        lib = load()
+        self._plans = list(plans)                 # keep the plans alive
+        arr = (ctypes.c_void_p * len(self._plans))(*[p.handle.value for p in self._plans])
+        h = ctypes.c_void_p()
+        check(lib.tqec_mixed_create(arr, len(self._plans), int(max_batches), int(max_rows), ctypes.byref(h)))
+        self._h = h
+        self.max_batches = int(max_batches)
+        self.n_last = 0
+        self.set_graph(graph)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().tqec_mixed_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_graph(self, enable: bool) -> None:
+        check(load().tqec_mixed_set_graph(self._h, 1 if enable else 0))
+
+    def plan_config(self, i: int):
+        """(tile, slab flags) of plan i: bit 0 FP32 slab, bit 1 FP64 slab."""
+        t, sl = ctypes.c_int32(), ctypes.c_int32()
+        check(load().tqec_mixed_plan_config(self._h, int(i), ctypes.byref(t), ctypes.byref(sl)))
+        return t.value, sl.value
+
+    def decode_device(self, batches, stream: int = 0) -> None:
+        """``batches``: sequence of (plan index, n, d_syn, d_mar, d_cls, d_tie)
+        with device addresses (0 for an absent output).  Enqueues only."""
+        tab = (MixedBatch * max(1, len(batches)))()
+        for i, (pi, n, syn, mar, cls, tie) in enumerate(batches):
+            tab[i] = MixedBatch(int(pi), 0, int(n), syn or None, mar or None, cls or None, tie or None)
+        check(load().tqec_mixed_decode_device(self._h, tab, len(batches), ctypes.c_void_p(stream) if stream else None))
+        self.n_last = len(batches)
+
+    def last_call(self):
+        """(per-batch latency in microseconds, FP64 re-check count) of the
+        last call; waits for it."""
+        us = np.zeros(max(1, self.n_last), dtype=np.float64)
+        rc = ctypes.c_int64()
+        check(load().tqec_mixed_last_call(self._h, _ptr(us, _f64p), self.n_last, ctypes.byref(rc)))
+        return us[:self.n_last], rc.value
