@@ -67,6 +67,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--config", default=None,
                     help="JSON line of an earlier run: reuse its plan and tile choice (no autotuning)")
+    ap.add_argument("--plan", default=None,
+                    help="SEED,SC_TARGET,MODE,TILE,THREADS[,TABLES_IN_LDS,WAVES]: pin the plan, executor and tile "
+                         "(no autotuning); SC_TARGET may be none; MODE fused_f32 or spec_f32")
+    ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB copied once by a plain device copy before the timed steps (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (syndrome batches shard "
@@ -94,36 +100,69 @@ def main():
     plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
     # Candidate contraction trees for the fused executor (seed, width target);
     # each is autotuned on the first batch and the fastest is kept.
+    pinned = None
     if args.config:
         pinned = json.loads(open(args.config).read().strip().splitlines()[-1])["config"]["plan"]
+    elif args.plan:
+        f = args.plan.split(",")
+        pinned = {"seed": int(f[0]), "sc_target": None if f[1].lower() == "none" else int(f[1]), "mode": f[2],
+                  "tile": int(f[3]), "threads": int(f[4])}
+        if len(f) > 5:
+            pinned["tables_in_lds"] = bool(int(f[5]))
+            pinned["waves"] = int(f[6])
+    if pinned:
         cands = [(pinned["seed"], pinned["sc_target"])]
     else:
-        cands = [(1, 10), (2, 10), (3, 10), (1, 9), (3, None), (2, 9)]
-    best = None
+        cands = [(1, 10), (2, 10), (3, 10), (1, 9), (3, None), (2, 9), (1, None)]
+    outs = [torch.empty((B, 4), dtype=torch.float64, device=dev), torch.empty(B, dtype=torch.int32, device=dev),
+            torch.empty(B, dtype=torch.uint8, device=dev)]
+
+    def time_mode(ct, mode):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(3):
+            ct.hip.decode_device(S_dev[0].data_ptr(), B, mode, outs[0].data_ptr(), outs[1].data_ptr(),
+                                 outs[2].data_ptr(), stream)
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / 3
+
+    # Candidate contraction trees (seed, width target): each is autotuned with
+    # the generic fused kernel; the two fastest are also autotuned with the
+    # plan-specialised kernel; the fastest (plan, executor, tile) is kept.
+    tried = []
     for seed, sct in cands:
         ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.TreeSA(seed=seed, sc_target=sct)), t, em)
-        if args.config:
-            ct.hip.set_tile("fused_f32", pinned["tile"], pinned["threads"])
-            tile, thr = pinned["tile"], pinned["threads"]
-            ms = 0.0
-        else:
-            tile, thr = ct.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
-            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            outs = [torch.empty((B, 4), dtype=torch.float64, device=dev), torch.empty(B, dtype=torch.int32, device=dev),
-                    torch.empty(B, dtype=torch.uint8, device=dev)]
-            ev0.record()
-            for _ in range(3):
-                ct.hip.decode_device(S_dev[0].data_ptr(), B, "fused_f32", outs[0].data_ptr(), outs[1].data_ptr(),
-                                     outs[2].data_ptr(), stream)
-            ev1.record()
-            torch.cuda.synchronize()
-            ms = ev0.elapsed_time(ev1) / 3
         st = ct.hip.stats()
+        if pinned:
+            cfg = {"seed": seed, "sc_target": sct, "mode": pinned["mode"], "tile": pinned["tile"],
+                   "threads": pinned["threads"]}
+            if pinned["mode"] == "spec_f32":
+                ct.hip.set_spec(pinned["tile"], pinned["threads"], pinned.get("tables_in_lds", False),
+                                pinned.get("waves", 0))
+                cfg.update(ct.hip.spec_config())
+            else:
+                ct.hip.set_tile("fused_f32", pinned["tile"], pinned["threads"])
+            tried.append((0.0, ct, cfg))
+            continue
+        tile, thr = ct.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
+        ms = time_mode(ct, "fused_f32")
         print(f"[bench] candidate seed={seed} sc_target={sct}: steps={st['n_dep_steps']} macs={st['dep_tc']:.0f} "
-              f"lds/syn={st['fused_lds_entries']} tile={tile} threads={thr} {ms:.3f} ms/batch",
+              f"lds/syn={st['fused_lds_entries']} fused_f32 tile={tile} threads={thr} {ms:.3f} ms/batch",
               file=sys.stderr, flush=True)
-        if best is None or ms < best[0]:
-            best = (ms, ct, {"seed": seed, "sc_target": sct, "tile": tile, "threads": thr})
+        tried.append((ms, ct, {"seed": seed, "sc_target": sct, "mode": "fused_f32", "tile": tile, "threads": thr}))
+    tried.sort(key=lambda x: x[0])
+    best = tried[0]
+    if not pinned and not args.no_spec:
+        for ms_f, ct, cfg in tried[:2]:
+            t1 = time.time()
+            ct.hip.autotune("spec_f32", S_dev[0].data_ptr(), B, stream)
+            sc = ct.hip.spec_config()
+            ms = time_mode(ct, "spec_f32")
+            print(f"[bench] candidate seed={cfg['seed']} sc_target={cfg['sc_target']}: spec_f32 {sc} {ms:.3f} ms/batch "
+                  f"(tuned in {time.time() - t1:.1f}s)", file=sys.stderr, flush=True)
+            if ms < best[0]:
+                best = (ms, ct, dict(cfg, mode="spec_f32", **sc))
     chosen, plan_cfg = best[1], best[2]
     print(f"[bench] plans compiled and tuned in {time.time() - t0:.1f}s: {plan_cfg}", file=sys.stderr, flush=True)
     truth = [true_classes(e, chosen.lx, chosen.lz) for e in eps]
@@ -132,6 +171,15 @@ def main():
     cls = torch.empty(B, dtype=torch.int32, device=dev)
     tie = torch.empty(B, dtype=torch.uint8, device=dev)
     print(f"[bench] {nb} batches of {B} syndromes resident", file=sys.stderr, flush=True)
+    if args.copy_calibration > 0:
+        # one plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the HBM byte counters of the decode kernels
+        n_el = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n_el, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def run(ct, mode, label):
         outs = []
@@ -157,7 +205,8 @@ def main():
         print(f"[bench] {label}: {args.steps} steps in {dt * 1e3:.2f} ms", file=sys.stderr, flush=True)
         return dt, kmain, krec, calls, cls_all.cpu().numpy(), tie_all.cpu().numpy(), mar_all.cpu().numpy()
 
-    dt_c, km_c, kr_c, calls_c, cls_c, tie_c, mar_c = run(chosen, "fused_f32", "chosen fused_f32")
+    mode_c = plan_cfg["mode"]
+    dt_c, km_c, kr_c, calls_c, cls_c, tie_c, mar_c = run(chosen, mode_c, f"chosen {mode_c}")
     thr_c = args.steps * B / dt_c
     res = {}
     truth_t = np.stack(truth[args.warmup:])
@@ -225,15 +274,18 @@ def main():
         "config": {"workload": f"rotated surface code d={d}, code-capacity depolarizing p={p} (p/3 each), "
                                f"{B} syndromes per step, TNMMAP marginals over 4 logical classes",
                    "batch": B, "d": d, "p": p, "seed": args.seed,
-                   "chosen": "fused_f32 (chain-factorized network, evidence as slices, subtrees on <= 4 "
-                             "syndrome bits tabulated, FP64 re-check)",
+                   "chosen": f"{mode_c} (chain-factorized network, evidence as slices, subtrees on <= 4 "
+                             "syndrome bits tabulated, FP32 with per-tensor power-of-two scaling, FP64 re-check"
+                             + (", plan-specialised kernel compiled with hiprtc" if mode_c == "spec_f32" else "")
+                             + ")",
                    "plan": plan_cfg,
                    "plan_stats": {k: st[k] for k in ("n_dep_steps", "dep_tc", "dep_rw", "fused_lds_entries",
                                                      "table_entries", "const_entries")},
                    "plain": "plain_f64 (reference network, one-hot evidence, one kernel per step)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "fused_kernel<float, true>", "kernel_ms_avg": avg_ms,
+                     "kernel": "tqec_spec" if mode_c == "spec_f32" else "fused_kernel<float, true>",
+                     "kernel_ms_avg": avg_ms,
                      "flops_per_syndrome": flops_per_syn,
                      "compute": "FP32 vector ALU with LDS-resident operands (no MFMA on this path)"},
         "cpu_baseline": cpu,

@@ -64,6 +64,11 @@ extern "C" {
                                  scaling; syndromes whose top-2 class
                                  margin is below the re-check threshold are
                                  re-contracted in FP64                    */
+#define TQEC_MODE_SPEC_F32 4  /* FUSED_F32 with a kernel generated for the
+                                 plan and compiled at first use (hiprtc):
+                                 constant index arithmetic, unrolled
+                                 contractions, tables in LDS; results are
+                                 bit-identical to FUSED_F32               */
 
 /*
  * Flattened contraction plan.  Mirrors the NestedEinsum that
@@ -251,6 +256,19 @@ int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int
  * batches), and the number of FP64 re-checks.  Waits for the call. */
 int tqec_mixed_last_call(tqec_mixed *m, double *batch_us, int32_t n_batches, int64_t *recheck_count);
 void tqec_mixed_destroy(tqec_mixed *m);
+
+/* Source of the TQEC_MODE_SPEC_F32 kernel for a tile configuration
+ * (inspection and offline compilation checks).  Writes at most `cap` bytes
+ * (NUL-terminated when it fits) and the full length to *len.  No reference
+ * counterpart. */
+int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, int32_t tables_in_lds, char *buf,
+                          size_t cap, size_t *len);
+
+/* Pin / read the TQEC_MODE_SPEC_F32 configuration (tile, threads, tables in
+ * LDS or global memory, minimum waves per SIMD requested from the compiler,
+ * 0 = none).  No reference counterpart. */
+int tqec_plan_set_spec(tqec_plan *plan, int32_t tile, int32_t threads, int32_t tables_in_lds, int32_t waves);
+int tqec_plan_spec_config(tqec_plan *plan, int32_t *tile, int32_t *threads, int32_t *tables_in_lds, int32_t *waves);
 
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);

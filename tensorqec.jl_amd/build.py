@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "tensorqec", "libtqec_hip.so")
 
-SOURCES = ["tqec_hip.hip", "plan_compile.cpp", "tnmap_hip.hip", "mlp_hip.hip", "sampler_hip.hip"]
+SOURCES = ["tqec_hip.hip", "plan_compile.cpp", "spec_codegen.cpp", "tnmap_hip.hip", "mlp_hip.hip", "sampler_hip.hip"]
 
 # Packed FP32 instructions are disabled for all device code: the compiler is
 # free to form them from float2 arithmetic otherwise.
@@ -44,7 +44,7 @@ def build(verbose: bool = False, extra=None) -> str:
         subprocess.run(cmd, check=True)
         objs.append(o)
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lhiprtc"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1,0 +1,275 @@
+// spec_codegen.cpp — see spec_codegen.hpp.
+//
+// The generated kernel follows the generic FP32 fused kernel of
+// tqec_hip.hip step for step (syndrome staging, per-tensor power-of-two
+// scaling with tagged LDS maxima, low-exponent flag, finalisation and FP64
+// re-check list), with every descriptor field substituted by its value.
+#include "spec_codegen.hpp"
+
+#include <algorithm>
+#include <sstream>
+
+namespace tqec {
+
+namespace {
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Layout {
+    size_t work = 0, tab = 0, mx = 0, ex = 0, syn = 0, low = 0, total = 0;
+    size_t tab_entries = 0;
+};
+
+Layout layout(const CompiledPlan &cp, const SpecCfg &cfg) {
+    const size_t G = size_t(1) << cfg.glog;
+    Layout L;
+    L.tab_entries = cp.const_pool.size() + cp.sliced_pool.size();
+    size_t off = 0;
+    L.work = off;
+    off += round_up((size_t)std::max<int64_t>(1, cp.ws_entries) * G * 4, 16);
+    L.tab = off;
+    if (cfg.tab_lds) off += round_up(std::max<size_t>(1, L.tab_entries) * 4, 16);
+    L.mx = off;
+    off += (size_t)std::max(1, cp.n_slots) * G * 8;
+    L.ex = off;
+    off += (size_t)std::max(1, cp.n_slots) * G * 4;
+    L.syn = off;
+    off += round_up((size_t)std::max(1, cp.n_syn) * G, 4);
+    L.low = off;
+    off += G * 4;
+    L.total = round_up(off, 16);
+    return L;
+}
+
+// gather_bits(c, op) as an expression of `c`.
+std::string gather_expr(const OperandDesc &op) {
+    std::ostringstream s;
+    bool any = false;
+    for (int i = 0; i < op.nseg; ++i) {
+        const Seg &g = op.seg[i];
+        if (g.width <= 0) continue;
+        const unsigned mask = (g.width >= 32) ? 0xffffffffu : ((1u << g.width) - 1u);
+        if (any) s << " | ";
+        s << "(((c >> " << g.src << ") & " << mask << "u) << " << g.dst << ")";
+        any = true;
+    }
+    if (!any) return "0u";
+    return s.str();
+}
+
+// Base pointer of an operand (fused_base).
+void emit_base(std::ostringstream &o, const char *name, const OperandDesc &op, size_t const_n) {
+    if (op.kind == kInter) {
+        o << "    const float *" << name << " = work + " << ((long long)op.off << 0) << " * G + g;\n";
+    } else if (op.kind == kConst) {
+        o << "    const float *" << name << " = tab + " << (long long)op.off << ";\n";
+    } else {
+        o << "    unsigned v_" << name << " = 0u;\n";
+        for (int j = 0; j < op.nev; ++j)
+            o << "    v_" << name << " |= (unsigned)synl[" << op.ev_bit[j] << " * G + g] << " << j << ";\n";
+        o << "    const float *" << name << " = tab + " << (long long)(const_n + (size_t)op.off) << " + (v_" << name
+          << " << " << op.rank << ");\n";
+    }
+}
+
+// Bits an operand index can take from the output index.
+uint32_t gathered_mask(const OperandDesc &op) {
+    uint32_t m = 0;
+    for (int i = 0; i < op.nseg; ++i) {
+        const Seg &g = op.seg[i];
+        if (g.width <= 0) continue;
+        const uint32_t w = (g.width >= 32) ? 0xffffffffu : ((1u << g.width) - 1u);
+        m |= w << g.dst;
+    }
+    return m;
+}
+
+// op_load_sum for element index (idx_var | kconst).  The index is written
+// as a sum when the constant's bits cannot meet the gathered bits, which
+// lets the compiler fold it into the LDS instruction's offset.
+std::string load_expr(const OperandDesc &op, const char *base, const char *idx, uint32_t kconst,
+                      std::ostringstream &pre, const std::string &tmp) {
+    std::ostringstream s;
+    const uint32_t gm = gathered_mask(op);
+    uint32_t allk = kconst;
+    if (op.kind == kInter)
+        for (uint32_t q = 0; q < (1u << op.nsingle); ++q) allk |= kconst | (q << op.sshift);
+    const char *join = (gm & allk) ? " | " : " + ";
+    if (op.kind != kInter) {
+        s << base << "[" << idx << join << kconst << "u]";
+        return s.str();
+    }
+    if (op.nsingle == 0) {
+        s << base << "[(" << idx << join << kconst << "u) << GLOG]";
+        return s.str();
+    }
+    // T s = 0; for q: s += base[(idx | q << sshift) * G]
+    pre << "        float " << tmp << " = 0.f;\n";
+    const uint32_t ns = 1u << op.nsingle;
+    for (uint32_t q = 0; q < ns; ++q)
+        pre << "        " << tmp << " += " << base << "[(" << idx << join << (kconst | (q << op.sshift))
+            << "u) << GLOG];\n";
+    return tmp;
+}
+
+void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep) {
+    const OperandDesc &A = st.a, &B = st.b;
+    o << "  { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << "\n";
+    o << "    int ea, eb, na = 0, nb = 0;\n";
+    for (int side = 0; side < 2; ++side) {
+        const OperandDesc &op = side ? B : A;
+        const char *n = side ? "nb" : "na";
+        const char *e = side ? "eb" : "ea";
+        if (op.kind == kInter) {
+            o << "    { const float m = __uint_as_float((unsigned)(mx[" << op.slot
+              << " * G + g] & 0xffffffffull)); " << n << " = (m > 0.f) ? ilogbf(m) : 0; " << e << " = ex["
+              << op.slot << " * G + g] + " << n << "; }\n";
+        } else {
+            o << "    " << e << " = " << op.cexp << ";\n";
+        }
+    }
+    o << "    const int shift = -(na + nb);\n";
+    {
+        std::string cond;
+        if (A.kind == kInter) cond += "na < -100";
+        if (B.kind == kInter) cond += std::string(cond.empty() ? "" : " || ") + "nb < -100";
+        if (!cond.empty()) o << "    if (" << cond << ") lowf[g] = 1;\n";
+    }
+    emit_base(o, "pa", A, const_n);
+    emit_base(o, "pb", B, const_n);
+    o << "    float *pc = work + " << (long long)st.c_off << " * G + g;\n";
+    o << "    float lmax = 0.f;\n";
+    const long long nc = 1ll << st.rc;
+    const std::string ga = gather_expr(A), gb = gather_expr(B);
+    auto body = [&](const char *indent) {
+        std::ostringstream b;
+        b << indent << "  const unsigned ar = " << ga << ";\n";
+        b << indent << "  const unsigned br = " << gb << ";\n";
+        b << indent << "  float acc = 0.f;\n";
+        const uint32_t nkk = 1u << st.nk;
+        for (uint32_t k = 0; k < nkk; ++k) {
+            std::ostringstream pre;
+            const std::string va = load_expr(A, "pa", "ar", k << A.kshift, pre, "sa" + std::to_string(k));
+            const std::string vb = load_expr(B, "pb", "br", k << B.kshift, pre, "sb" + std::to_string(k));
+            std::string p = pre.str();
+            // re-indent the pre-statements
+            b << p;
+            b << indent << "  acc = fmaf(" << va << ", " << vb << ", acc);\n";
+        }
+        b << indent << "  acc = ldexpf(acc, shift);\n";
+        b << indent << "  lmax = acc > lmax ? acc : lmax;\n";
+        b << indent << "  pc[c << GLOG] = acc;\n";
+        return b.str();
+    };
+    if (nc <= (long long)cstep) {
+        o << "    if (c0 < " << nc << ") {\n      const unsigned c = (unsigned)c0;\n" << body("    ") << "    }\n";
+    } else {
+        o << "    #pragma unroll 1\n";
+        o << "    for (unsigned c = (unsigned)c0; c < " << nc << "u; c += CSTEP) {\n" << body("    ") << "    }\n";
+    }
+    o << "    for (int w = G; w < 64; w <<= 1) { const float other = __shfl_xor(lmax, w); lmax = other > lmax ? other : lmax; }\n";
+    o << "    if (lane < G) atomicMax(&mx[" << st.c_slot << " * G + g], ((unsigned long long)(tag_base + " << s
+      << "u) << 32) | (unsigned long long)__float_as_uint(lmax));\n";
+    o << "    if (tid < G) ex[" << st.c_slot << " * G + g] = ea + eb;\n";
+    o << "  }\n";
+    o << "  __syncthreads();\n";
+}
+
+}  // namespace
+
+size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg) { return layout(cp, cfg).total; }
+
+std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
+    const Layout L = layout(cp, cfg);
+    const size_t const_n = cp.const_pool.size();
+    const int nsteps = (int)cp.steps.size();
+    std::ostringstream o;
+    o << "// plan-specialised FP32 TNMMAP kernel (generated)\n";
+    o << "#define G " << (1 << cfg.glog) << "\n#define GLOG " << cfg.glog << "\n#define NT " << cfg.threads
+      << "\n#define CSTEP " << (cfg.threads >> cfg.glog) << "\n#define NBITS " << std::max(1, cp.n_syn)
+      << "\n#define NOUT " << (1 << cp.n_out) << "\n#define NSLOTS " << std::max(1, cp.n_slots) << "\n#define NSTEPS "
+      << nsteps << "\n#define TABN " << L.tab_entries << "\n";
+    if (cfg.waves > 0)
+        o << "extern \"C\" __global__ void __launch_bounds__(NT, " << cfg.waves << ") tqec_spec(\n";
+    else
+        o << "extern \"C\" __global__ void __launch_bounds__(NT) tqec_spec(\n";
+    o << 
+         "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
+         "    const int *__restrict__ list_count, const float *__restrict__ gtab, double *__restrict__ mar,\n"
+         "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
+         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol) {\n";
+    o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
+    o << "  float *work = reinterpret_cast<float *>(smem + " << L.work << ");\n";
+    if (cfg.tab_lds)
+        o << "  float *tab = reinterpret_cast<float *>(smem + " << L.tab << ");\n";
+    else
+        o << "  const float *tab = gtab;\n";
+    o << "  unsigned long long *mx = reinterpret_cast<unsigned long long *>(smem + " << L.mx << ");\n";
+    o << "  int *ex = reinterpret_cast<int *>(smem + " << L.ex << ");\n";
+    o << "  unsigned char *synl = smem + " << L.syn << ";\n";
+    o << "  int *lowf = reinterpret_cast<int *>(smem + " << L.low << ");\n";
+    o << "  const int tid = threadIdx.x;\n  const int g = tid & (G - 1);\n  const int c0 = tid >> GLOG;\n"
+         "  const int lane = tid & 63;\n";
+    if (cfg.tab_lds) o << "  for (int i = tid; i < TABN; i += NT) tab[i] = gtab[i];\n";
+    o << "  const long long count = list ? (long long)(*list_count) : n;\n";
+    o << "  const long long ntiles = (count + G - 1) >> GLOG;\n";
+    o << "  unsigned iter = 0;\n";
+    o << "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter) {\n";
+    o << "  const long long b0 = t << GLOG;\n";
+    o << "  const unsigned tag_base = iter * (unsigned)NSTEPS + 1u;\n";
+    o << "  if (!list) {\n"
+         "    const long long rows = min((long long)G, count - b0);\n"
+         "    const unsigned char *src = syn + b0 * NBITS;\n"
+         "    for (int i = tid; i < G * NBITS; i += NT) {\n"
+         "      const int r = i / NBITS, bit = i - r * NBITS;\n"
+         "      synl[(bit << GLOG) + r] = (r < rows) ? (src[i] & 1u) : 0u;\n"
+         "    }\n"
+         "  } else {\n"
+         "    for (int i = tid; i < G * NBITS; i += NT) {\n"
+         "      const int r = i / NBITS, bit = i - r * NBITS;\n"
+         "      const long long q = b0 + r;\n"
+         "      synl[(bit << GLOG) + r] = (q < count) ? (syn[(long long)list[q] * NBITS + bit] & 1u) : 0u;\n"
+         "    }\n"
+         "  }\n"
+         "  for (int i = tid; i < NSLOTS * G; i += NT) mx[i] = 0ull;\n"
+         "  if (tid < G) lowf[tid] = 0;\n"
+         "  __syncthreads();\n";
+    for (int s = 0; s < nsteps; ++s) emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog);
+    o << "  if (tid < G) {\n"
+         "    const long long q = b0 + tid;\n"
+         "    if (q < count) {\n"
+         "      const long long b = list ? (long long)list[q] : q;\n";
+    o << "      const int e = ex[" << cp.root_slot << " * G + g];\n";
+    o << "      float best = 0, second = -1;\n"
+         "      int bi = 0;\n"
+         "      for (int j = 0; j < NOUT; ++j) {\n";
+    o << "        const float x = work[(" << (long long)cp.root_off << " + j) * G + g];\n";
+    o << "        if (mar) mar[b * NOUT + j] = ldexp((double)x, e);\n"
+         "        if (j == 0 || x > best) {\n"
+         "          if (j > 0) second = best;\n"
+         "          best = x;\n"
+         "          bi = j;\n"
+         "        } else if (x > second) {\n"
+         "          second = x;\n"
+         "        }\n"
+         "      }\n"
+         "      if (cls) cls[b] = bi;\n"
+         "      const bool low = NOUT > 1 && second >= 0 && (double)(best - second) <= tie_tol * (double)best;\n"
+         "      if (ties) ties[b] = low ? 1 : 0;\n"
+         "      if (rl) {\n"
+         "        const bool rc = !(best > 0) || lowf[tid] != 0 ||\n"
+         "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
+         "        if (rc) {\n"
+         "          const int slot = atomicAdd(rcount, 1);\n"
+         "          if (slot < rcap) rl[slot] = (int)b;\n"
+         "        }\n"
+         "      }\n"
+         "    }\n"
+         "  }\n"
+         "  __syncthreads();\n"
+         "  }\n"
+         "}\n";
+    return o.str();
+}
+
+}  // namespace tqec

@@ -1,0 +1,39 @@
+// spec_codegen.hpp — plan-specialised FP32 kernel source.
+//
+// The generic fused kernel interprets the step descriptors at run time: per
+// output entry it walks the operands' bit-field segments, loops over
+// contracted and single-side labels with run-time trip counts and reads
+// table operands from global memory.  For a fixed plan all of that is
+// known, so this generator writes the same per-syndrome program as straight
+// HIP source (constant shifts and masks, unrolled contraction loops, LDS
+// offsets as immediates, tables staged in LDS), compiled at run time with
+// hiprtc for gfx950.  The arithmetic — order of every sum, fma, scale and
+// comparison — is the generic FP32 kernel's, so both give bit-identical
+// results.
+#pragma once
+#include <stddef.h>
+
+#include <string>
+
+#include "plan_compile.hpp"
+
+namespace tqec {
+
+struct SpecCfg {
+    int glog = 0;        // log2 of syndromes per workgroup tile
+    int threads = 128;   // threads per workgroup (>= tile)
+    bool tab_lds = true; // constant and sliced tables staged in LDS
+    int waves = 0;       // >0: ask the compiler for at least this many waves per SIMD (caps VGPRs)
+};
+
+// Static LDS bytes of the specialised kernel for this configuration.
+size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg);
+
+// HIP source of the kernel `tqec_spec` (extern "C").  Parameters:
+//   (const uint8_t *syn, int64_t n, const int32_t *list, const int32_t *list_count,
+//    const float *tab, double *mar, int32_t *cls, uint8_t *tie,
+//    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol)
+// `tab` is the FP32 constant pool followed by the FP32 sliced pool.
+std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg);
+
+}  // namespace tqec

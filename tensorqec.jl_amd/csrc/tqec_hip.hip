@@ -19,20 +19,25 @@
 // Reference behaviour replaced: src/decoding/tndecoder.jl:148-165 and
 // :240-253 (evidence update, OMEinsum contraction, findmax).
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "plan.hpp"
 #include "plan_compile.hpp"
+#include "spec_codegen.hpp"
 #include "tqec_hip.h"
 
 using namespace tqec;
@@ -788,6 +793,7 @@ int set_error_shared(int code, const std::string &msg) { return set_error(code, 
 namespace {
 
 struct DeviceData {
+    float *tab32 = nullptr;     // FP32 constant pool followed by the sliced pool (specialised kernel)
     MapStep *msteps = nullptr;
     int32_t *mlabels = nullptr;
     StepDesc *steps = nullptr;
@@ -830,6 +836,15 @@ struct FusedCfg {
     int blocks = 0;
 };
 
+// A compiled plan-specialised kernel (TQEC_MODE_SPEC_F32).
+struct SpecKernel {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    SpecCfg cfg;
+    size_t lds = 0;
+    int blocks = 0;
+};
+
 }  // namespace
 
 struct tqec_plan {
@@ -845,6 +860,9 @@ struct tqec_plan {
     bool timing = false;
     size_t lds_limit = 0;
     int n_cu = 0;
+    SpecCfg spec_cfg;
+    bool spec_cfg_set = false;
+    std::map<std::tuple<int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves)
 };
 
 namespace {
@@ -873,6 +891,11 @@ int ensure_device(tqec_plan *p, int device, DeviceData **out) {
     if ((rc = upload<double>(cp.sliced_pool, &dd.spool64))) return rc;
     if ((rc = upload<float>(cp.const_pool, &dd.cpool32))) return rc;
     if ((rc = upload<float>(cp.sliced_pool, &dd.spool32))) return rc;
+    {
+        std::vector<double> both(cp.const_pool);
+        both.insert(both.end(), cp.sliced_pool.begin(), cp.sliced_pool.end());
+        if ((rc = upload<float>(both, &dd.tab32))) return rc;
+    }
     if (cp.semiring == TQEC_SEMIRING_MAX) {
         TQ_HIP(hipMalloc((void **)&dd.msteps, std::max<size_t>(1, cp.map_steps.size()) * sizeof(MapStep)));
         TQ_HIP(hipMemcpy(dd.msteps, cp.map_steps.data(), cp.map_steps.size() * sizeof(MapStep), hipMemcpyHostToDevice));
@@ -1115,6 +1138,10 @@ FusedArgs base_args(tqec_plan *p, DeviceData *dd, const uint8_t *d_syn, int64_t 
     return a;
 }
 
+int prepare_recheck(Workspace &w, int64_t n, hipStream_t stream);
+int enqueue_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+                    int32_t *d_cls, uint8_t *d_tie, hipStream_t stream);
+
 // `in_list` / `in_count` (device): decode only the syndromes listed (at most
 // n of them), writing their outputs at their own row.
 int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, int mode, double *d_mar,
@@ -1143,18 +1170,7 @@ int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, 
     if (c32.glog < 0)
         return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP32)");
     int rc = TQEC_OK;
-    if (w.list_cap < n) {
-        if (w.list) TQ_HIP(hipFree(w.list));
-        w.list = nullptr;
-        w.list_cap = 0;
-        TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
-        w.list_cap = n;
-    }
-    if (!w.count) {
-        TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
-        TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
-    }
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
+    if ((rc = prepare_recheck(w, n, stream))) return rc;
     FusedArgs a = base_args(p, dd, d_syn, n, c32.glog, true);
     a.list = in_list;
     a.list_count = in_count;
@@ -1169,7 +1185,32 @@ int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, 
     if ((rc = record(p, w, 0, stream))) return rc;
     hipLaunchKernelGGL((fused_kernel<float, true>), dim3(grid), dim3(c32.threads), c32.lds, stream, a);
     if ((rc = record(p, w, 1, stream))) return rc;
-    // FP64 re-check of the low-margin syndromes (device-side list, no host sync)
+    return enqueue_recheck(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+}
+
+// Re-check list and counter of a (plan, stream) workspace, cleared on the stream.
+int prepare_recheck(Workspace &w, int64_t n, hipStream_t stream) {
+    if (w.list_cap < n) {
+        if (w.list) TQ_HIP(hipFree(w.list));
+        w.list = nullptr;
+        w.list_cap = 0;
+        TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+        w.list_cap = n;
+    }
+    if (!w.count) {
+        TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
+        TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
+    }
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
+    return TQEC_OK;
+}
+
+// FP64 re-check of the syndromes listed by the FP32 pass (device-side list,
+// no host synchronisation), then the list length is copied to the host.
+int enqueue_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+                    int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+    const FusedCfg &c64 = p->f64;
+    int rc;
     FusedArgs r = base_args(p, dd, d_syn, n, c64.glog, false);
     r.list = w.list;
     r.list_count = w.count;
@@ -1183,6 +1224,236 @@ int run_fused(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, 
     if ((rc = record(p, w, 3, stream))) return rc;
     TQ_HIP(hipMemcpyAsync(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     TQ_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+// ---- plan-specialised FP32 kernels ---------------------------------------
+
+// Code objects by source text, shared by every plan of the process (equal
+// plans and tile choices give equal sources).
+std::mutex g_code_mu;
+std::unordered_map<std::string, std::vector<char>> g_code_cache;
+
+int spec_code(const std::string &src, std::vector<char> &code) {
+    {
+        std::lock_guard<std::mutex> lk(g_code_mu);
+        auto it = g_code_cache.find(src);
+        if (it != g_code_cache.end()) {
+            code = it->second;
+            return TQEC_OK;
+        }
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "tqec_spec.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return set_error(TQEC_EDEVICE, "hiprtcCreateProgram failed");
+    const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        std::string log;
+        if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+            log.resize(ls);
+            hiprtcGetProgramLog(prog, &log[0]);
+        }
+        hiprtcDestroyProgram(&prog);
+        return set_error(TQEC_EDEVICE, std::string("hiprtc: ") + hiprtcGetErrorString(r) + ": " + log.substr(0, 1500));
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.assign(cs, 0);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    std::lock_guard<std::mutex> lk(g_code_mu);
+    g_code_cache[src] = code;
+    return TQEC_OK;
+}
+
+bool spec_fits(const tqec_plan *p, const SpecCfg &c) { return spec_lds_bytes(p->cp, c) <= p->lds_limit; }
+
+// Default before tuning: as choose_fused, preferring tables in LDS.
+SpecCfg choose_spec(const tqec_plan *p) {
+    for (int div : {4, 2, 1}) {
+        for (int gl = 6; gl >= 0; --gl) {
+            for (bool tab : {true, false}) {
+                SpecCfg c;
+                c.glog = gl;
+                c.threads = std::max(64, std::min(128, 1 << gl));
+                c.tab_lds = tab;
+                if (spec_lds_bytes(p->cp, c) <= p->lds_limit / div) return c;
+            }
+        }
+    }
+    SpecCfg c;
+    c.glog = -1;
+    return c;
+}
+
+int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
+    const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves);
+    auto it = p->spec.find(key);
+    if (it != p->spec.end()) {
+        *out = &it->second;
+        return TQEC_OK;
+    }
+    if (cfg.glog < 0 || !spec_fits(p, cfg)) return set_error(TQEC_EUNSUPPORTED, "specialised tile does not fit in LDS");
+    std::vector<char> code;
+    int rc;
+    if ((rc = spec_code(spec_source(p->cp, cfg), code))) return rc;
+    SpecKernel k;
+    k.cfg = cfg;
+    k.lds = spec_lds_bytes(p->cp, cfg);
+    TQ_HIP(hipModuleLoadData(&k.mod, code.data()));
+    const hipError_t e = hipModuleGetFunction(&k.fn, k.mod, "tqec_spec");
+    if (e != hipSuccess) {
+        (void)hipModuleUnload(k.mod);
+        return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
+    }
+    k.blocks = p->n_cu * resident_per_cu(k.lds, cfg.threads, p->lds_limit);
+    p->spec[key] = k;
+    *out = &p->spec[key];
+    return TQEC_OK;
+}
+
+int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+             int32_t *d_cls, uint8_t *d_tie, hipStream_t stream, const int32_t *in_list = nullptr,
+             const int32_t *in_count = nullptr) {
+    if (p->f64.glog < 0)
+        return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP64)");
+    if (!p->spec_cfg_set) {
+        p->spec_cfg = choose_spec(p);
+        p->spec_cfg_set = true;
+    }
+    if (p->spec_cfg.glog < 0)
+        return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP32)");
+    SpecKernel *k;
+    int rc;
+    if ((rc = spec_get(p, w.device, p->spec_cfg, &k))) return rc;
+    if ((rc = prepare_recheck(w, n, stream))) return rc;
+    const int64_t ntiles = (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
+    const unsigned char *a_syn = d_syn;
+    long long a_n = (long long)n;
+    const int *a_list = in_list, *a_count = in_count;
+    const float *a_tab = dd->tab32;
+    double *a_mar = d_mar;
+    int *a_cls = d_cls;
+    unsigned char *a_tie = d_tie;
+    int *a_rl = w.list, *a_rc = w.count;
+    long long a_cap = (long long)w.list_cap;
+    double a_tt = p->tie_tol, a_rt = p->recheck_tol;
+    void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt};
+    if ((rc = record(p, w, 0, stream))) return rc;
+    TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
+    if ((rc = record(p, w, 1, stream))) return rc;
+    return enqueue_recheck(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+}
+
+// Compile a set of specialised configurations on host threads (hiprtc
+// programs are independent); the code objects land in the process cache.
+void spec_precompile(const tqec_plan *p, const std::vector<SpecCfg> &cands) {
+    std::vector<std::future<int>> jobs;
+    const size_t par = 8;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        if (jobs.size() >= par) {
+            jobs.front().get();
+            jobs.erase(jobs.begin());
+        }
+        const std::string src = spec_source(p->cp, cands[i]);
+        jobs.push_back(std::async(std::launch::async, [src]() {
+            std::vector<char> code;
+            return spec_code(src, code);
+        }));
+    }
+    for (auto &j : jobs) j.get();
+}
+
+// Autotuner of the specialised kernel, in two stages: (tile 1..64, 64/128/256
+// threads) with tables in global memory, then the best pair with tables in
+// LDS and/or a 4-waves-per-SIMD register cap.  Each configuration is timed
+// twice on the trial batch and the fastest is kept.
+int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st, int32_t *tile, int32_t *threads) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc;
+    if ((rc = configure(p))) return rc;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    DeviceData *dd;
+    if ((rc = ensure_device(p, dev, &dd))) return rc;
+    Workspace &w = p->ws[std::make_pair(dev, st)];
+    w.device = dev;
+    if (p->f64.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the fused executor");
+    const int nout = 1 << p->cp.n_out;
+    if ((rc = grow(&w.d_mar, &w.d_mar_bytes, (size_t)n * nout * sizeof(double)))) return rc;
+    if ((rc = grow(&w.d_cls, &w.d_cls_bytes, (size_t)n * sizeof(int32_t)))) return rc;
+    if ((rc = grow(&w.d_tie, &w.d_tie_bytes, (size_t)n))) return rc;
+    hipEvent_t e0, e1;
+    TQ_HIP(hipEventCreate(&e0));
+    TQ_HIP(hipEventCreate(&e1));
+    const bool saved_timing = p->timing;
+    p->timing = false;
+    const SpecCfg saved = p->spec_cfg;
+    const bool saved_set = p->spec_cfg_set;
+    SpecCfg best;
+    best.glog = -1;
+    float best_ms = 1e30f;
+    auto time_all = [&](const std::vector<SpecCfg> &cands) {
+        spec_precompile(p, cands);
+        for (const SpecCfg &c : cands) {
+            p->spec_cfg = c;
+            p->spec_cfg_set = true;
+            float ms = 0;
+            for (int rep = 0; rep < 2 && rc == TQEC_OK; ++rep) {
+                if ((rc = record_or(hipEventRecord(e0, st)))) break;
+                rc = run_spec(p, dd, w, d_syn, n, w.d_mar, w.d_cls, w.d_tie, st);
+                if (rc) break;
+                if ((rc = record_or(hipEventRecord(e1, st)))) break;
+                if ((rc = record_or(hipEventSynchronize(e1)))) break;
+                if ((rc = record_or(hipEventElapsedTime(&ms, e0, e1)))) break;
+            }
+            if (rc) return;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    };
+    std::vector<SpecCfg> stage1;
+    for (int gl = 0; gl <= 6; ++gl)
+        for (int thr : {64, 128, 256}) {
+            if (thr < (1 << gl)) continue;
+            SpecCfg c;
+            c.glog = gl;
+            c.threads = thr;
+            c.tab_lds = false;
+            if (spec_fits(p, c)) stage1.push_back(c);
+        }
+    if (stage1.empty()) rc = set_error(TQEC_EUNSUPPORTED, "no specialised tile fits in LDS");
+    if (!rc) time_all(stage1);
+    if (!rc && best.glog >= 0) {
+        std::vector<SpecCfg> stage2;
+        for (bool tab : {false, true})
+            for (int wv : {0, 4}) {
+                if (!tab && wv == 0) continue;
+                SpecCfg c = best;
+                c.tab_lds = tab;
+                c.waves = wv;
+                if (spec_fits(p, c)) stage2.push_back(c);
+            }
+        time_all(stage2);
+    }
+    p->timing = saved_timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        p->spec_cfg = saved;
+        p->spec_cfg_set = saved_set;
+        return rc;
+    }
+    p->spec_cfg = best;
+    p->spec_cfg_set = true;
+    if (tile) *tile = 1 << best.glog;
+    if (threads) *threads = best.threads;
     return TQEC_OK;
 }
 
@@ -1202,6 +1473,7 @@ int decode_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int mode, doubl
     if (mode == TQEC_MODE_WIDE_F32) return run_wide(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
     if (mode == TQEC_MODE_FUSED_F64 || mode == TQEC_MODE_FUSED_F32)
         return run_fused(p, dd, w, d_syn, n, mode, d_mar, d_cls, d_tie, stream);
+    if (mode == TQEC_MODE_SPEC_F32) return run_spec(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
     return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
 }
 
@@ -1352,9 +1624,14 @@ void tqec_plan_destroy(tqec_plan *p) {
         for (auto &a : w.ev)
             for (hipEvent_t e : a) (void)hipEventDestroy(e);
     }
+    for (auto &kv : p->spec) {
+        if (hipSetDevice(std::get<0>(kv.first)) != hipSuccess) continue;
+        if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
+    }
     for (auto &kv : p->dev) {
         if (hipSetDevice(kv.first) != hipSuccess) continue;
         DeviceData &d = kv.second;
+        if (d.tab32) (void)hipFree(d.tab32);
         (void)hipFree(d.steps);
         if (d.msteps) (void)hipFree(d.msteps);
         if (d.mlabels) (void)hipFree(d.mlabels);
@@ -1416,7 +1693,7 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
     if (n > 0 && !syn) return set_error(TQEC_ENULL, "null syndrome pointer");
     if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
-    if (mode < 0 || mode > 3) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+    if (mode < 0 || mode > TQEC_MODE_SPEC_F32) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
     if (p->cp.semiring != TQEC_SEMIRING_SUM)
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1463,6 +1740,7 @@ int tqec_plan_autotune(tqec_plan *p, int32_t mode, const uint8_t *d_syn, int64_t
                        int32_t *tile, int32_t *threads) {
     if (!p || !d_syn) return set_error(TQEC_ENULL, "null argument");
     if (n <= 0 || n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "trial batch size out of range");
+    if (mode == TQEC_MODE_SPEC_F32) return autotune_spec(p, d_syn, n, (hipStream_t)stream, tile, threads);
     if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
         return set_error(TQEC_EINVAL, "only the fused modes have tunable tiles");
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1528,7 +1806,7 @@ int tqec_decode_list_device(tqec_plan *p, const uint8_t *d_syn, const int32_t *d
     if (!p) return set_error(TQEC_ENULL, "null plan");
     if (p->cp.semiring != TQEC_SEMIRING_SUM)
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
-    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
+    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32 && mode != TQEC_MODE_SPEC_F32)
         return set_error(TQEC_EINVAL, "list decoding runs on the fused executors");
     if (n_max < 0 || n_max > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "n_max out of range");
     if (n_max > 0 && (!d_syn || !d_list || !d_count)) return set_error(TQEC_ENULL, "null device pointer");
@@ -1542,6 +1820,8 @@ int tqec_decode_list_device(tqec_plan *p, const uint8_t *d_syn, const int32_t *d
     Workspace &w = p->ws[std::make_pair(dev, (hipStream_t)stream)];
     w.device = dev;
     if (n_max == 0) return TQEC_OK;
+    if (mode == TQEC_MODE_SPEC_F32)
+        return run_spec(p, dd, w, d_syn, n_max, d_mar, d_cls, d_tie, (hipStream_t)stream, d_list, d_count);
     return run_fused(p, dd, w, d_syn, n_max, mode, d_mar, d_cls, d_tie, (hipStream_t)stream, d_list, d_count);
 }
 
@@ -1643,7 +1923,7 @@ int tqec_map_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t n
 
 int tqec_plan_set_tile(tqec_plan *p, int32_t mode, int32_t tile, int32_t threads) {
     if (!p) return set_error(TQEC_ENULL, "null plan");
-    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
+    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32 && mode != TQEC_MODE_SPEC_F32)
         return set_error(TQEC_EINVAL, "only the fused modes have tunable tiles");
     int gl = 0;
     while (gl <= 6 && (1 << gl) != tile) ++gl;
@@ -1653,6 +1933,17 @@ int tqec_plan_set_tile(tqec_plan *p, int32_t mode, int32_t tile, int32_t threads
     std::lock_guard<std::mutex> lk(p->mu);
     int rc;
     if ((rc = configure(p))) return rc;
+    if (mode == TQEC_MODE_SPEC_F32) {
+        SpecCfg c;
+        c.glog = gl;
+        c.threads = threads;
+        c.tab_lds = true;
+        if (!spec_fits(p, c)) c.tab_lds = false;
+        if (!spec_fits(p, c)) return set_error(TQEC_EUNSUPPORTED, "tile does not fit in LDS");
+        p->spec_cfg = c;
+        p->spec_cfg_set = true;
+        return TQEC_OK;
+    }
     const bool f32 = mode == TQEC_MODE_FUSED_F32;
     const size_t elem = f32 ? sizeof(float) : sizeof(double);
     if (fused_lds_bytes(p->cp, gl, elem, f32) > p->lds_limit)
@@ -1788,12 +2079,13 @@ int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batc
         m->slab32_bytes = std::max(m->slab32_bytes, c.slab32);
         m->slab64_bytes = std::max(m->slab64_bytes, c.slab64);
     }
-    if (m->lds32 > lds_limit || m->lds64 > lds_limit)
+    if (m->lds32 > lds_limit - 256 || m->lds64 > lds_limit - 256)
         return set_error(TQEC_EUNSUPPORTED, "a plan's scales and syndrome bits exceed the LDS");
+    // (the kernels' own static LDS is one 8-byte work-queue slot)
     TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel_f32, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_limit));
+                               (int)(lds_limit - 256)));
     TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel_f64, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_limit));
+                               (int)(lds_limit - 256)));
     m->blocks32 = n_cu * resident_per_cu(m->lds32 + 64, kMixedThreads, lds_limit);
     m->blocks64 = n_cu * resident_per_cu(m->lds64 + 64, kMixedThreads, lds_limit);
     tqec_mixed *mp = m.get();
@@ -1864,6 +2156,9 @@ int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int
     hipStream_t st = (hipStream_t)stream;
     MixedSlot &sl = m->slot[m->cur];
     if (sl.used) TQ_HIP(hipEventSynchronize(sl.staged));
+    // Calls share the slabs: a call on another stream starts after the last one.
+    const MixedSlot &prev = m->slot[m->cur ^ 1];
+    if (prev.used) TQ_HIP(hipStreamWaitEvent(st, prev.staged, 0));
     char *h = reinterpret_cast<char *>(sl.h_stage);
     MixedHeader *hdr = reinterpret_cast<MixedHeader *>(h);
     MixedBatch *tb = reinterpret_cast<MixedBatch *>(h + sizeof(MixedHeader));
@@ -1946,4 +2241,72 @@ void tqec_mixed_destroy(tqec_mixed *m) {
     delete m;
 }
 
+
+int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int32_t tables_in_lds, char *buf,
+                          size_t cap, size_t *len) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (p->cp.semiring != TQEC_SEMIRING_SUM) return set_error(TQEC_EINVAL, "max-product plans have no FP32 kernel");
+    int gl = 0;
+    while (gl <= 6 && (1 << gl) != tile) ++gl;
+    if (gl > 6) return set_error(TQEC_EINVAL, "tile must be a power of two in [1, 64]");
+    if ((threads != 64 && threads != 128 && threads != 256) || threads < tile)
+        return set_error(TQEC_EINVAL, "threads must be 64, 128 or 256 and at least the tile");
+    SpecCfg c;
+    c.glog = gl;
+    c.threads = threads;
+    c.tab_lds = tables_in_lds != 0;
+    std::string src;
+    try {
+        src = spec_source(p->cp, c);
+    } catch (const std::exception &e) {
+        return set_error(TQEC_ENOMEM, e.what());
+    }
+    if (len) *len = src.size();
+    if (buf && cap > 0) {
+        const size_t m = std::min(cap - 1, src.size());
+        std::memcpy(buf, src.data(), m);
+        buf[m] = 0;
+    }
+    return TQEC_OK;
+}
+
+
+int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t tables_in_lds, int32_t waves) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    int gl = 0;
+    while (gl <= 6 && (1 << gl) != tile) ++gl;
+    if (gl > 6) return set_error(TQEC_EINVAL, "tile must be a power of two in [1, 64]");
+    if ((threads != 64 && threads != 128 && threads != 256) || threads < tile)
+        return set_error(TQEC_EINVAL, "threads must be 64, 128 or 256 and at least the tile");
+    if (waves < 0 || waves > 8) return set_error(TQEC_EINVAL, "waves must be in [0, 8]");
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc;
+    if ((rc = configure(p))) return rc;
+    SpecCfg c;
+    c.glog = gl;
+    c.threads = threads;
+    c.tab_lds = tables_in_lds != 0;
+    c.waves = waves;
+    if (!spec_fits(p, c)) return set_error(TQEC_EUNSUPPORTED, "tile does not fit in LDS");
+    p->spec_cfg = c;
+    p->spec_cfg_set = true;
+    return TQEC_OK;
+}
+
+int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t *tables_in_lds, int32_t *waves) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc;
+    if ((rc = configure(p))) return rc;
+    if (!p->spec_cfg_set) {
+        p->spec_cfg = choose_spec(p);
+        p->spec_cfg_set = true;
+    }
+    if (p->spec_cfg.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the specialised kernel");
+    if (tile) *tile = 1 << p->spec_cfg.glog;
+    if (threads) *threads = p->spec_cfg.threads;
+    if (tables_in_lds) *tables_in_lds = p->spec_cfg.tab_lds ? 1 : 0;
+    if (waves) *waves = p->spec_cfg.waves;
+    return TQEC_OK;
+}
 }  // extern "C"

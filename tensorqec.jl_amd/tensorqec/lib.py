@@ -25,8 +25,9 @@ MODE_PLAIN_F64 = 0
 MODE_FUSED_F64 = 1
 MODE_FUSED_F32 = 2
 MODE_WIDE_F32 = 3
+MODE_SPEC_F32 = 4
 MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32,
-         "wide_f32": MODE_WIDE_F32}
+         "wide_f32": MODE_WIDE_F32, "spec_f32": MODE_SPEC_F32}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -167,6 +168,11 @@ EXPORTS = [
     ("tqec_mixed_last_call", ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int32,
                                              ctypes.POINTER(ctypes.c_int64)]),
     ("tqec_mixed_destroy", None, [ctypes.c_void_p]),
+    ("tqec_plan_spec_source", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    ("tqec_plan_set_spec", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32]),
+    ("tqec_plan_spec_config", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p, _i32p]),
     ("tqec_last_error", ctypes.c_char_p, []),
 ]
 
@@ -313,6 +319,24 @@ class HipPlan:
     def set_tile(self, mode, tile: int, threads: int) -> None:
         m = MODES[mode] if isinstance(mode, str) else int(mode)
         check(load().tqec_plan_set_tile(self._h, m, int(tile), int(threads)))
+
+    def set_spec(self, tile: int, threads: int, tables_in_lds: bool, waves: int = 0) -> None:
+        check(load().tqec_plan_set_spec(self._h, int(tile), int(threads), int(bool(tables_in_lds)), int(waves)))
+
+    def spec_config(self) -> dict:
+        v = [ctypes.c_int32() for _ in range(4)]
+        check(load().tqec_plan_spec_config(self._h, *[ctypes.byref(x) for x in v]))
+        return {"tile": v[0].value, "threads": v[1].value, "tables_in_lds": bool(v[2].value), "waves": v[3].value}
+
+    def spec_source(self, tile: int, threads: int, tables_in_lds: bool = True) -> str:
+        """HIP source of the plan-specialised FP32 kernel for a tile choice."""
+        n = ctypes.c_size_t()
+        check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), int(bool(tables_in_lds)), None, 0,
+                                           ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), int(bool(tables_in_lds)), buf,
+                                           n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
 
     def set_timing(self, enable: bool) -> None:
         check(load().tqec_plan_set_timing(self._h, 1 if enable else 0))

@@ -1,0 +1,173 @@
+"""Mixed-code executor: batches of several codes decoded by one persistent
+launch (tqec_mixed_*), against the single-plan executors on the same
+syndromes.
+
+  * Plans whose tiles fit the LDS: marginals, classes and tie flags
+    bit-identical to tqec_decode_batch_device(..., fused_f32) — every
+    syndrome goes through the same per-syndrome FP32 arithmetic and the same
+    FP64 re-check rule, only the tiling differs.
+  * Slab plans (intermediates in global memory): marginals within 2e-5 of
+    the row maximum of the plain FP64 executor, classes equal on every row
+    whose FP64 top-2 gap is above 1e-12 (the fused_f32 contract).
+Graph replay and direct enqueue give identical results, and a second replay
+with different (smaller) batches reuses the captured graph.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import tn_oracle as O
+from tensorqec import decoders as D
+from tensorqec import lib as L
+from tensorqec.codes import CSSTannerGraph, ShorCode, SteaneCode, SurfaceCode, ToricCode
+from tensorqec.errors import css_syndrome_rows, iid_error, sample_depolarizing
+
+pytestmark = pytest.mark.gpu
+
+CODES = [("surface3", SurfaceCode(3, 3), 0.05), ("steane", SteaneCode(), 0.08), ("shor", ShorCode(), 0.03),
+         ("surface5", SurfaceCode(5, 5), 0.06), ("surface7", SurfaceCode(7, 7), 0.05),
+         ("surface9", SurfaceCode(9, 9), 0.02), ("toric3", ToricCode(3, 3), 0.07), ("toric4", ToricCode(4, 4), 0.04)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def setup():
+    torch = pytest.importorskip("torch")
+    plans = []
+    for name, code, p in CODES:
+        t = CSSTannerGraph.from_code(code)
+        em = iid_error(p / 3, p / 3, p / 3, t.nq)
+        opt = D.TreeSA(seed=1, ntrials=8, sweeps=1000, restarts=8) if name == "toric4" else D.TreeSA(seed=1)
+        ct = D.compile(D.TNMMAP(optimizer=opt, mode="fused_f32"), t, em)
+        plans.append((name, t, em, ct))
+    return torch, plans
+
+
+def make_batches(torch, plans, spec, seed):
+    dev = torch.device("cuda:0")
+    out = []
+    for k, (pi, n) in enumerate(spec):
+        name, t, em, ct = plans[pi]
+        S = css_syndrome_rows(sample_depolarizing(em, seed=seed + 101 * k, count=max(n, 1)), t)[:n]
+        nout = 1 << ct.hip.n_out
+        dS = torch.from_numpy(np.ascontiguousarray(S)).to(dev) if n else torch.empty((0, t.ns), dtype=torch.uint8, device=dev)
+        out.append(dict(plan=pi, n=n, S=S, dS=dS,
+                        mar=torch.full((max(n, 1), nout), -1.0, dtype=torch.float64, device=dev),
+                        cls=torch.full((max(n, 1),), -1, dtype=torch.int32, device=dev),
+                        tie=torch.full((max(n, 1),), 7, dtype=torch.uint8, device=dev)))
+    return out
+
+
+def table(batches):
+    return [(b["plan"], b["n"], b["dS"].data_ptr() if b["n"] else 0, b["mar"].data_ptr(), b["cls"].data_ptr(),
+             b["tie"].data_ptr()) for b in batches]
+
+
+def check_against_single(torch, plans, mx, batches):
+    dev = torch.device("cuda:0")
+    for b in batches:
+        name, t, em, ct = plans[b["plan"]]
+        n = b["n"]
+        if n == 0:
+            continue
+        tile, slab = mx.plan_config(b["plan"])
+        mar = b["mar"][:n].cpu().numpy()
+        cls = b["cls"][:n].cpu().numpy()
+        tie = b["tie"][:n].cpu().numpy()
+        nout = mar.shape[1]
+        if slab & 1 == 0:
+            rm = torch.empty((n, nout), dtype=torch.float64, device=dev)
+            rc = torch.empty(n, dtype=torch.int32, device=dev)
+            rt = torch.empty(n, dtype=torch.uint8, device=dev)
+            ct.hip.decode_device(b["dS"].data_ptr(), n, "fused_f32", rm.data_ptr(), rc.data_ptr(), rt.data_ptr())
+            torch.cuda.synchronize()
+            assert np.array_equal(mar, rm.cpu().numpy()), name
+            assert np.array_equal(cls, rc.cpu().numpy()), name
+            assert np.array_equal(tie, rt.cpu().numpy()), name
+        else:
+            rm = torch.empty((n, nout), dtype=torch.float64, device=dev)
+            rc = torch.empty(n, dtype=torch.int32, device=dev)
+            rt = torch.empty(n, dtype=torch.uint8, device=dev)
+            ct.hip.decode_device(b["dS"].data_ptr(), n, "plain_f64", rm.data_ptr(), rc.data_ptr(), rt.data_ptr())
+            torch.cuda.synchronize()
+            ref = rm.cpu().numpy()
+            valid = ref.max(axis=1) > 0
+            assert np.all(mar[~valid] == 0.0)
+            err = np.max(np.abs(mar[valid] - ref[valid]) / ref[valid].max(axis=1, keepdims=True)) if valid.any() else 0
+            assert err < 2e-5, (name, err)
+            ok = valid & (O.tie_flags(ref, 1e-12) == 0)
+            assert np.array_equal(cls[ok], O.findmax(ref)[ok]), name
+
+
+SPEC1 = [(0, 1), (4, 300), (1, 7), (6, 33), (3, 64), (2, 0), (5, 17), (4, 5), (7, 9), (0, 1000), (2, 129),
+         (6, 1), (3, 4096)]
+SPEC2 = [(4, 40), (0, 3), (7, 2), (5, 1), (6, 10)]
+
+
+def test_mixed_stream_matches_single_plan_executors(setup):
+    torch, plans = setup
+    say("case mixed: create")
+    mx = L.HipMixed([ct.hip for _, _, _, ct in plans], max_batches=16, max_rows=8192)
+    for i, (name, *_rest) in enumerate(plans):
+        say(f"  plan {name}: tile/slab {mx.plan_config(i)}")
+    assert mx.plan_config(6)[1] & 1 == 1          # toric3 needs the slab
+    assert mx.plan_config(4)[1] == 0
+    stream = torch.cuda.Stream()
+    b1 = make_batches(torch, plans, SPEC1, seed=11)
+    say("case mixed: graph call 1")
+    mx.decode_device(table(b1), stream=stream.cuda_stream)
+    lat, nre = mx.last_call()
+    assert len(lat) == len(SPEC1)
+    for (pi, n), us in zip(SPEC1, lat):
+        assert (us > 0) == (n > 0)
+    say(f"  latencies us {np.round(lat, 1).tolist()} rechecks {nre}")
+    check_against_single(torch, plans, mx, b1)
+    say("case mixed: direct enqueue gives the same results")
+    b1d = make_batches(torch, plans, SPEC1, seed=11)
+    mx.set_graph(False)
+    mx.decode_device(table(b1d), stream=stream.cuda_stream)
+    mx.last_call()
+    for x, y in zip(b1, b1d):
+        for k in ("mar", "cls", "tie"):
+            assert torch.equal(x[k], y[k])
+    mx.set_graph(True)
+    say("case mixed: graph calls 2 and 3 (smaller batches, both slots)")
+    b2 = make_batches(torch, plans, SPEC2, seed=23)
+    mx.decode_device(table(b2), stream=stream.cuda_stream)
+    b3 = make_batches(torch, plans, SPEC2[::-1], seed=37)
+    mx.decode_device(table(b3), stream=stream.cuda_stream)
+    lat3, _ = mx.last_call()
+    assert len(lat3) == len(SPEC2)
+    check_against_single(torch, plans, mx, b2)
+    check_against_single(torch, plans, mx, b3)
+    say("case mixed: replay of slot 1 with the first table again")
+    b4 = make_batches(torch, plans, SPEC1, seed=11)
+    mx.decode_device(table(b4), stream=stream.cuda_stream)
+    mx.last_call()
+    for x, y in zip(b1, b4):
+        for k in ("mar", "cls", "tie"):
+            assert torch.equal(x[k], y[k])
+
+
+def test_mixed_rejects_bad_tables(setup):
+    torch, plans = setup
+    mx = L.HipMixed([plans[0][3].hip, plans[1][3].hip], max_batches=2, max_rows=10)
+    dev = torch.device("cuda:0")
+    S = torch.zeros((8, 8), dtype=torch.uint8, device=dev)
+    with pytest.raises(L.TqecError) as e:
+        mx.decode_device([(2, 1, S.data_ptr(), 0, 0, 0)])
+    assert e.value.code == L.TQEC_EINVAL
+    with pytest.raises(L.TqecError) as e:
+        mx.decode_device([(0, 8, S.data_ptr(), 0, 0, 0), (0, 8, S.data_ptr(), 0, 0, 0)])
+    assert e.value.code == L.TQEC_EINVAL
+    with pytest.raises(L.TqecError) as e:
+        mx.decode_device([(0, 1, S.data_ptr(), 0, 0, 0)] * 3)
+    assert e.value.code == L.TQEC_EINVAL
+    with pytest.raises(L.TqecError) as e:
+        mx.decode_device([(0, 1, 0, 0, 0, 0)])
+    assert e.value.code == L.TQEC_ENULL

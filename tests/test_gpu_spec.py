@@ -1,0 +1,100 @@
+"""Plan-specialised FP32 kernel (TQEC_MODE_SPEC_F32) against the generic
+FP32 fused kernel on the same syndromes: marginals, classes and tie flags
+must be bit-identical — the generated kernel performs the same operations in
+the same order, with the descriptor values substituted.  Several tile
+configurations and the list (gated fall-back) entry point are covered; the
+oracle comparison of the mode itself runs in the parity tests."""
+import sys
+
+import numpy as np
+import pytest
+
+from tensorqec import decoders as D
+from tensorqec import lib as L
+from tensorqec.codes import CSSTannerGraph, ShorCode, SteaneCode, SurfaceCode, ToricCode
+from tensorqec.errors import css_syndrome_rows, iid_error, sample_depolarizing
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(torch, ct, dS, n, mode):
+    dev = torch.device("cuda:0")
+    nout = 1 << ct.hip.n_out
+    mar = torch.full((n, nout), -1.0, dtype=torch.float64, device=dev)
+    cls = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    tie = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    ct.hip.decode_device(dS.data_ptr(), n, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr())
+    torch.cuda.synchronize()
+    return mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy()
+
+
+@pytest.mark.parametrize("name,code,p,n", [("surface3", SurfaceCode(3, 3), 0.05, 777),
+                                           ("steane", SteaneCode(), 0.08, 300), ("shor", ShorCode(), 0.03, 129),
+                                           ("surface5", SurfaceCode(5, 5), 0.06, 4097),
+                                           ("surface7", SurfaceCode(7, 7), 0.05, 20000),
+                                           ("toric3", ToricCode(3, 3), 0.07, 65)])
+def test_spec_bit_identical_to_generic(name, code, p, n):
+    torch = pytest.importorskip("torch")
+    say(f"case spec {name} n={n}")
+    t = CSSTannerGraph.from_code(code)
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="fused_f32"), t, em)
+    S = css_syndrome_rows(sample_depolarizing(em, seed=4242 + t.nq, count=n), t)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+    try:
+        ref = run(torch, ct, dS, n, "fused_f32")
+    except L.TqecError as e:
+        assert e.code == L.TQEC_EUNSUPPORTED
+        pytest.skip(f"{name}: plan does not fit the fused executors")
+    configs = [None, (1, 64), (4, 128), (16, 256)]
+    for cfg in configs:
+        if cfg is not None:
+            try:
+                ct.hip.set_tile("spec_f32", *cfg)
+            except L.TqecError as e:
+                assert e.code == L.TQEC_EUNSUPPORTED
+                continue
+        say(f"  tile {cfg}")
+        got = run(torch, ct, dS, n, "spec_f32")
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (name, cfg)
+
+
+def test_spec_list_decode_and_autotune():
+    torch = pytest.importorskip("torch")
+    t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
+    p = 0.05
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="fused_f32"), t, em)
+    n = 8192
+    S = css_syndrome_rows(sample_depolarizing(em, seed=99, count=n), t)
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to(dev)
+    say("case spec autotune")
+    tile, thr = ct.hip.autotune("spec_f32", dS.data_ptr(), n)
+    say(f"  chosen tile {tile} threads {thr}")
+    ref = run(torch, ct, dS, n, "fused_f32")
+    got = run(torch, ct, dS, n, "spec_f32")
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    say("case spec list decode")
+    rows = np.arange(3, n, 7, dtype=np.int32)
+    dl = torch.from_numpy(rows).to(dev)
+    dc = torch.tensor([len(rows)], dtype=torch.int32, device=dev)
+    mar = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    cls = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    tie = torch.zeros(n, dtype=torch.uint8, device=dev)
+    ct.hip.decode_list_device(dS.data_ptr(), dl.data_ptr(), dc.data_ptr(), len(rows), "spec_f32", mar.data_ptr(),
+                              cls.data_ptr(), tie.data_ptr())
+    torch.cuda.synchronize()
+    c = cls.cpu().numpy()
+    assert np.array_equal(c[rows], ref[1][rows])
+    untouched = np.ones(n, bool)
+    untouched[rows] = False
+    assert np.all(c[untouched] == -1)
+    assert np.array_equal(mar.cpu().numpy()[rows], ref[0][rows])

@@ -149,3 +149,31 @@ def test_stats_mirror_plan():
     assert s["tc"] == ps["tc"] and 0 < s["dep_tc"] <= ps["tc_dep"] and s["sc"] == ps["sc"]
     assert s["n_out"] == 2 and s["n_steps"] == ct.plan.n_steps
     assert s["plain_ws_entries"] > 0
+
+
+@pytest.mark.parametrize("tile,threads,tab", [(1, 64, True), (4, 128, False), (16, 256, True)])
+def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, tab):
+    """The plan-specialised FP32 kernel (TQEC_MODE_SPEC_F32) is generated on
+    the host and compiled for gfx950 at first use; check here, without a
+    GPU, that the generated source compiles and uses no scratch memory."""
+    import shutil
+    import subprocess
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not available")
+    t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
+    p = 0.05 / 3
+    ct = D.compile(D.TNMMAP(mode="fused_f32"), t, iid_error(p, p, p, t.nq))
+    src = ct.hip.spec_source(tile, threads, tab)
+    assert "tqec_spec" in src and f"#define G {tile}\n" in src
+    f = tmp_path / "spec.hip"
+    f.write_text("#include <hip/hip_runtime.h>\n" + src)
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
+                        str(f), "-o", str(tmp_path / "spec.o"), "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
+    assert m and int(m.group(1)) == 0
+    with pytest.raises(L.TqecError):
+        ct.hip.spec_source(3, 64, True)          # not a power of two
