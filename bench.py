@@ -53,6 +53,294 @@ def true_classes(ep, lx, lz):
     return cls
 
 
+STREAM_CODES = (("surface3", ("SurfaceCode", 3)), ("surface5", ("SurfaceCode", 5)), ("surface7", ("SurfaceCode", 7)),
+                ("surface9", ("SurfaceCode", 9)), ("toric3", ("ToricCode", 3)), ("toric4", ("ToricCode", 4)),
+                ("steane", ("SteaneCode", 0)), ("shor", ("ShorCode", 0)))
+
+
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
+    """Uneven-size batched stream: rotated surface codes d=3/5/7/9, toric codes
+    L=3/4, Steane and Shor codes, each under code-capacity depolarizing noise
+    with p drawn uniformly from [0.01, 0.1]; every call decodes a window of
+    batches (code drawn uniformly, size log-uniform in [1, 4096]) in one
+    persistent launch of the mixed-code executor, replayed from a HIP graph.
+    Toric L=5/6 are left out: their exact contraction needs per-syndrome
+    tensors of rank >= 24 (64 MB per syndrome in FP32)."""
+    import numpy as np
+    from tensorqec import codes as C
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.codes import CSSTannerGraph
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+
+    rng = np.random.default_rng(seed)
+    cuda_stream = torch.cuda.Stream()
+    sp = cuda_stream.cuda_stream
+    t0 = time.time()
+    plans, info = [], []
+    for name, (cls_name, size) in STREAM_CODES:
+        code = getattr(C, cls_name)(size, size) if size else getattr(C, cls_name)()
+        t = CSSTannerGraph.from_code(code)
+        p = float(rng.uniform(0.01, 0.1))
+        em = iid_error(p / 3, p / 3, p / 3, t.nq)
+        opt = D.TreeSA(seed=1, ntrials=8, sweeps=1000, restarts=8) if name == "toric4" else D.TreeSA(seed=1)
+        ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=opt), t, em)
+        lx, lz = logical_operator(t)
+        smp = L.HipSampler.css(t, em, lx, lz)
+        S = torch.empty((pool, t.ns), dtype=torch.uint8, device=dev)
+        truth = torch.empty(pool, dtype=torch.int32, device=dev)
+        smp.sample(seed + len(plans), 0, pool, S.data_ptr(), truth.data_ptr())
+        st = ct.hip.stats()
+        try:
+            ct.hip.autotune("fused_f32", S.data_ptr(), pool)
+        except L.TqecError:
+            pass                                  # slab plan: no LDS tile to tune
+        plans.append(ct)
+        info.append(dict(name=name, p=p, ns=t.ns, nout=1 << ct.hip.n_out, S=S, truth=truth,
+                         macs=st["dep_tc"], sc=st["sc"]))
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    nb_total = (warmup + calls) * window
+    code_of = rng.integers(0, len(plans), nb_total)
+    size_of = np.minimum(4096, np.floor(np.exp(rng.uniform(0, np.log(4097), nb_total)))).astype(np.int64)
+    size_of = np.maximum(size_of, 1)
+    off_of = np.array([rng.integers(0, pool - n + 1) for n in size_of])
+    max_rows = int(max(size_of[i * window:(i + 1) * window].sum() for i in range(warmup + calls)))
+    mx = L.HipMixed([ct.hip for ct in plans], max_batches=window, max_rows=max_rows)
+    out_mar = [torch.empty(max_rows * 16, dtype=torch.float64, device=dev) for _ in range(2)]
+    out_cls = [torch.empty(max_rows, dtype=torch.int32, device=dev) for _ in range(2)]
+    out_tie = [torch.empty(max_rows, dtype=torch.uint8, device=dev) for _ in range(2)]
+
+    def table(ci):
+        tab, row, mrow = [], 0, 0
+        k = ci % 2
+        for b in range(ci * window, (ci + 1) * window):
+            j, n, o = int(code_of[b]), int(size_of[b]), int(off_of[b])
+            it = info[j]
+            tab.append((j, n, it["S"][o:].data_ptr(), out_mar[k][mrow:].data_ptr(), out_cls[k][row:].data_ptr(),
+                        out_tie[k][row:].data_ptr()))
+            row += n
+            mrow += n * it["nout"]
+        return tab, row
+
+    tabs = [table(ci) for ci in range(warmup + calls)]
+    rows_timed = sum(r for _, r in tabs[warmup:])
+    for ci in range(warmup):
+        mx.decode_device(tabs[ci][0], stream=sp)
+    cuda_stream.synchronize()
+    # throughput: calls back to back (two in flight)
+    t1 = time.perf_counter()
+    for ci in range(warmup, warmup + calls):
+        mx.decode_device(tabs[ci][0], stream=sp)
+    cuda_stream.synchronize()
+    dt = time.perf_counter() - t1
+    # latency: one call at a time, per-batch device-clock latencies
+    lat, call_ms, rechecks, errors, decided = [], [], 0, 0, 0
+    for ci in range(warmup, warmup + calls):
+        h0 = time.perf_counter()
+        mx.decode_device(tabs[ci][0], stream=sp)
+        us, nre = mx.last_call()
+        call_ms.append((time.perf_counter() - h0) * 1e3)
+        lat.extend(us.tolist())
+        rechecks += nre
+        k = ci % 2
+        cls_np = out_cls[k].cpu().numpy()
+        row = 0
+        for b in range(ci * window, (ci + 1) * window):
+            j, n, o = int(code_of[b]), int(size_of[b]), int(off_of[b])
+            tr = info[j]["truth"][o:o + n].cpu().numpy()
+            errors += int((cls_np[row:row + n] != tr).sum())
+            decided += n
+            row += n
+    # the same batches as one launch sequence per batch (single-plan executors)
+    dev_outs = (torch.empty(4096 * 16, dtype=torch.float64, device=dev), torch.empty(4096, dtype=torch.int32, device=dev),
+                torch.empty(4096, dtype=torch.uint8, device=dev))
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for ci in range(warmup, warmup + calls):
+        for b in range(ci * window, (ci + 1) * window):
+            j, n, o = int(code_of[b]), int(size_of[b]), int(off_of[b])
+            mode = "fused_f32" if mx.plan_config(j)[1] == 0 else "wide_f32"
+            plans[j].hip.decode_device(info[j]["S"][o:].data_ptr(), n, mode, dev_outs[0].data_ptr(),
+                                       dev_outs[1].data_ptr(), dev_outs[2].data_ptr())
+    torch.cuda.synchronize()
+    dt_sep = time.perf_counter() - t2
+    return {
+        "workload": "uneven stream: surface d=3/5/7/9, toric L=3/4, Steane, Shor; p ~ U[0.01, 0.1] per code; "
+                    f"{window} batches per launch, sizes log-uniform in [1, 4096]",
+        "codes": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in it.items()
+                   if k in ("name", "p", "ns", "macs", "sc")} | {"tile_slab": mx.plan_config(i)}
+                  for i, it in enumerate(info)],
+        "syndromes_per_s": rows_timed / dt,
+        "calls": calls, "batches_per_call": window, "syndromes_timed": rows_timed,
+        "ms_per_call": dt / calls * 1e3,
+        "batch_latency_us_p50": pct(lat, 50), "batch_latency_us_p99": pct(lat, 99),
+        "call_latency_ms_p50": pct(call_ms, 50), "call_latency_ms_p99": pct(call_ms, 99),
+        "fp64_rechecks": rechecks, "logical_error_rate_all_codes": errors / max(decided, 1),
+        "separate_launches_syndromes_per_s": rows_timed / dt_sep,
+        "setup_s": setup_s,
+    }
+
+
+def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_train=10_000_000, train_seconds=90.0,
+                batch=65536, eval_batches=8):
+    """Learned front-end at d=7 code capacity p=0.05: a 48-512-512-512-4 MLP
+    (BF16 weights, FP32 accumulation) trained by the library's own kernels on
+    TN-labelled syndromes, then gated: confident syndromes keep the MLP's
+    class, the rest go to the TN decoder on the device.  The gate threshold
+    is the smallest of a fixed ladder whose decisions differ from the TN's on
+    at most 1e-4 of a calibration batch; the evaluation batches are held
+    out.  Reported against plain FP64 TN decisions on the same syndromes."""
+    import numpy as np
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.codes import CSSTannerGraph, SurfaceCode
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+    from tensorqec.learned import GatedDecoder, HipMLP, train_on_tn
+
+    t = CSSTannerGraph.from_code(SurfaceCode(d, d))
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    if ct is None:
+        ct = D.compile(D.TNMMAP(mode=mode), t, em)
+    plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
+    lx, lz = logical_operator(t)
+    smp = L.HipSampler.css(t, em, lx, lz)
+    mlp = HipMLP(t.ns, 4, seed=seed % 1000)
+    rep = train_on_tn(mlp, ct, smp, n_samples=n_train, batch=8192, lr=2e-3, lr_final=1e-4, seed=seed + 5,
+                      torch=torch, mode=mode, max_seconds=train_seconds)
+    stream = torch.cuda.current_stream().cuda_stream
+    S = torch.empty((batch, t.ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(batch, dtype=torch.int32, device=dev)
+    mar = torch.empty((batch, 4), dtype=torch.float64, device=dev)
+    cls_tn = torch.empty(batch, dtype=torch.int32, device=dev)
+    cls_g = torch.empty(batch, dtype=torch.int32, device=dev)
+    conf = torch.empty(batch, dtype=torch.float32, device=dev)
+    tie = torch.empty(batch, dtype=torch.uint8, device=dev)
+    # calibration batch: pick the threshold
+    smp.sample(seed + 77, 0, batch, S.data_ptr(), truth.data_ptr())
+    ct.hip.decode_device(S.data_ptr(), batch, mode, mar.data_ptr(), cls_tn.data_ptr(), tie.data_ptr(), stream)
+    ladder = [0.9, 0.95, 0.98, 0.99, 0.995, 0.998, 0.999, 0.9995, 0.9999]
+    chosen = ladder[-1]
+    for th in ladder:
+        gd = GatedDecoder(mlp, ct, th, batch, torch=torch, mode=mode)
+        gd.decode_device(S.data_ptr(), batch, cls_g.data_ptr(), conf.data_ptr(), mar.data_ptr(), tie.data_ptr(), stream)
+        torch.cuda.synchronize()
+        if float((cls_g != cls_tn).float().mean().item()) <= 1e-4:
+            chosen = th
+            break
+    gd = GatedDecoder(mlp, ct, chosen, batch, torch=torch, mode=mode)
+    # held-out evaluation
+    res = dict(fallback=0, disagree_tn=0, disagree_fp64=0, err_g=0, err_fp64=0, err_tn=0, n=0)
+    tg, ttn = 0.0, 0.0
+    for k in range(eval_batches):
+        smp.sample(seed + 1000, k * batch, batch, S.data_ptr(), truth.data_ptr())
+        torch.cuda.synchronize()
+        h0 = time.perf_counter()
+        ct.hip.decode_device(S.data_ptr(), batch, mode, mar.data_ptr(), cls_tn.data_ptr(), tie.data_ptr(), stream)
+        torch.cuda.synchronize()
+        h1 = time.perf_counter()
+        gd.decode_device(S.data_ptr(), batch, cls_g.data_ptr(), conf.data_ptr(), mar.data_ptr(), tie.data_ptr(), stream)
+        torch.cuda.synchronize()
+        h2 = time.perf_counter()
+        if k > 0:                       # first batch warms up
+            ttn += h1 - h0
+            tg += h2 - h1
+        m64, c64, t64 = plain.hip.decode(S.cpu().numpy(), "plain_f64")
+        tr = truth.cpu().numpy()
+        a, b = cls_tn.cpu().numpy(), cls_g.cpu().numpy()
+        res["fallback"] += gd.fallbacks()
+        res["disagree_tn"] += int((a != b).sum())
+        res["disagree_fp64"] += int((b != c64).sum())
+        res["err_g"] += int((b != tr).sum())
+        res["err_tn"] += int((a != tr).sum())
+        res["err_fp64"] += int((c64 != tr).sum())
+        res["n"] += batch
+    n = res["n"]
+    ler64 = res["err_fp64"] / n
+    ci = wilson(res["err_fp64"], n)
+    ler_g = res["err_g"] / n
+    return {
+        "workload": f"learned front-end, rotated surface code d={d}, code-capacity p={p}: MLP {t.ns}-512-512-512-4 "
+                    "(BF16 weights, FP32 accumulation), trained on TN-labelled syndromes, confidence-gated, "
+                    f"TN ({mode}) for the rest",
+        "train_samples": rep.samples, "train_steps": rep.steps, "train_s": rep.seconds,
+        "train_samples_per_s": rep.samples / max(rep.seconds, 1e-9),
+        "loss_first_last": [rep.losses[0][1], rep.losses[-1][1]] if rep.losses else None,
+        "gate_threshold": chosen, "eval_syndromes": n,
+        "fallback_fraction": res["fallback"] / n,
+        "disagreement_vs_tn": res["disagree_tn"] / n, "disagreement_vs_fp64": res["disagree_fp64"] / n,
+        "ler_gated": ler_g, "ler_tn": res["err_tn"] / n, "ler_fp64": ler64, "ler_fp64_ci95": ci,
+        "ler_gated_within_ci": ci[0] <= ler_g <= ci[1],
+        "gated_syndromes_per_s": batch * (eval_batches - 1) / max(tg, 1e-9),
+        "tn_only_syndromes_per_s": batch * (eval_batches - 1) / max(ttn, 1e-9),
+    }
+
+
+def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
+    """Circuit-level memory-Z experiment (rotated surface code, depolarizing
+    p after every 1- and 2-qubit gate, reset and measurement flips p),
+    compiled to a detector error model by the library's own generator and
+    decoded by TNMMAP over the DEM network; FP32 executors against plain FP64
+    on the same seeded syndromes."""
+    import numpy as np
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+
+    t0 = time.time()
+    dem = detector_error_model_from_stim(rotated_memory_z(d, rounds, p))
+    ct = D.compile(D.TNMMAP(mode="wide_f32"), dem)
+    st = ct.hip.stats()
+    compile_s = time.time() - t0
+    smp = L.HipSampler.flips(dem.error_rates, dem.detector_matrix(), dem.logical_matrix())
+    ns = smp.nsyn
+    S = torch.empty((n, ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(n, dtype=torch.int32, device=dev)
+    smp.sample(seed + 3, 0, n, S.data_ptr(), truth.data_ptr())
+    tr = truth.cpu().numpy()
+    nout = 1 << ct.hip.n_out
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": f"circuit-level memory-Z, rotated surface code d={d}, {rounds} rounds, p={p}: "
+                       f"{ns} detectors, {len(dem.error_rates)} mechanisms, {n} syndromes",
+           "plan": {k: st[k] for k in ("sc", "dep_tc", "dep_rw", "plain_ws_entries", "n_dep_steps")},
+           "compile_s": compile_s, "modes": {}}
+    ref = None
+    for mode in ("plain_f64", "wide_f32", "spec_f32"):
+        mar = torch.empty((n, nout), dtype=torch.float64, device=dev)
+        cls = torch.empty(n, dtype=torch.int32, device=dev)
+        tie = torch.empty(n, dtype=torch.uint8, device=dev)
+        try:
+            ct.hip.decode_device(S.data_ptr(), n, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), stream)
+        except L.TqecError as e:
+            out["modes"][mode] = {"unsupported": str(e)}
+            continue
+        torch.cuda.synchronize()
+        h0 = time.perf_counter()
+        for _ in range(reps):
+            ct.hip.decode_device(S.data_ptr(), n, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - h0) / reps
+        m, c, ti = mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy()
+        r = {"syndromes_per_s": n / dt, "ler": float((c != tr).mean())}
+        if ref is None:
+            ref = (m, c, ti)
+        else:
+            valid = ref[0].max(axis=1) > 0
+            r["max_rel_marginal_err_vs_fp64"] = float(np.max(np.abs(m[valid] - ref[0][valid]) /
+                                                             ref[0][valid].max(axis=1, keepdims=True)))
+            ok = ref[2] == 0
+            r["class_agreement_non_tie"] = float((c[ok] == ref[1][ok]).mean())
+        out["modes"][mode] = r
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +359,10 @@ def main():
                     help="SEED,SC_TARGET,MODE,TILE,THREADS[,TABLES_IN_LDS,WAVES]: pin the plan, executor and tile "
                          "(no autotuning); SC_TARGET may be none; MODE fused_f32 or spec_f32")
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
+    ap.add_argument("--only", default=None, choices=["stream", "learned", "dem"],
+                    help="run one secondary workload alone and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed steps (HBM counter calibration)")
     args = ap.parse_args()
@@ -81,6 +373,11 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.only:
+        fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg}[args.only]
+        print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed)))
+        return
 
     from tensorqec import decoders as D
     from tensorqec.codes import CSSTannerGraph, SurfaceCode
@@ -257,6 +554,18 @@ def main():
                          f"(one-hot evidence, every step per syndrome), OpenMP over syndromes, "
                          f"{dt:.1f}s wall"}
         print(f"[bench] cpu baseline {ncpu / dt:.0f} syn/s on {cores} threads", file=sys.stderr, flush=True)
+
+    if not args.no_extras:
+        t0 = time.time()
+        res["stream"] = stream_leg(torch, dev, args.seed)
+        print(f"[bench] stream leg {time.time() - t0:.1f}s: {res['stream']['syndromes_per_s']:.0f} syn/s", file=sys.stderr,
+              flush=True)
+        t0 = time.time()
+        res["learned"] = learned_leg(torch, dev, args.seed, ct=chosen, mode=mode_c, d=d, p=p)
+        print(f"[bench] learned leg {time.time() - t0:.1f}s: {res['learned']}", file=sys.stderr, flush=True)
+        t0 = time.time()
+        res["dem"] = dem_leg(torch, dev, args.seed)
+        print(f"[bench] dem leg {time.time() - t0:.1f}s: {res['dem']['modes']}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,

@@ -69,6 +69,8 @@ extern "C" {
                                  constant index arithmetic, unrolled
                                  contractions, tables in LDS; results are
                                  bit-identical to FUSED_F32               */
+#define TQEC_MODE_SPEC_F64 5  /* FUSED_F64 with the plan-specialised kernel;
+                                 bit-identical to FUSED_F64               */
 
 /*
  * Flattened contraction plan.  Mirrors the NestedEinsum that
@@ -218,7 +220,7 @@ int tqec_posteriors_from_marginals(const double *d_marginals, int64_t n, int32_t
 
 /* ---- Mixed-code executor ------------------------------------------------
  * Batches of syndromes of different codes (different plans) decoded by one
- * persistent FP32 launch with the FP64 re-check behind it, as
+ * persistent FP32 launch with in-launch FP64 re-checks, as
  * `decode(ct, syndrome)` (src/decoding/interfaces.jl:116-160) for a stream
  * of compiled decoders of uneven size; each batch's results are those of
  * tqec_decode_batch_device(plan, ..., TQEC_MODE_FUSED_F32, ...).  Plans whose
@@ -246,6 +248,13 @@ int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batc
                       tqec_mixed **out);
 /* Graph capture on (default) or off. */
 int tqec_mixed_set_graph(tqec_mixed *m, int32_t enable);
+/* Order in which the tiles of a call's batches are handed out: as given, or
+ * cheapest batch first (default; batch size x the plan's work per syndrome),
+ * which lets small batches finish before large ones.  Results and latencies
+ * are reported in the caller's batch order either way. */
+#define TQEC_MIXED_ORDER_GIVEN 0
+#define TQEC_MIXED_ORDER_CHEAPEST_FIRST 1
+int tqec_mixed_set_order(tqec_mixed *m, int32_t order);
 /* Tile (syndromes per workgroup) chosen for a plan and where its
  * intermediates live: bit 0 FP32 pass in a slab, bit 1 FP64 re-check in a slab. */
 int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int32_t *slab);
@@ -253,15 +262,16 @@ int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int
 int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int32_t n_batches, void *stream);
 /* After the last call: per batch, microseconds from the first tile's start
  * to the batch's last result write on the device clock (0 for empty
- * batches), and the number of FP64 re-checks.  Waits for the call. */
+ * batches), and the number of FP64 re-checks (each done by the workgroup
+ * that decoded the syndrome, right after its tile).  Waits for the call. */
 int tqec_mixed_last_call(tqec_mixed *m, double *batch_us, int32_t n_batches, int64_t *recheck_count);
 void tqec_mixed_destroy(tqec_mixed *m);
 
-/* Source of the TQEC_MODE_SPEC_F32 kernel for a tile configuration
- * (inspection and offline compilation checks).  Writes at most `cap` bytes
- * (NUL-terminated when it fits) and the full length to *len.  No reference
- * counterpart. */
-int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, int32_t tables_in_lds, char *buf,
+/* Source of the plan-specialised kernel for a tile configuration
+ * (inspection and offline compilation checks); `flags` bit 0: tables in LDS,
+ * bit 1: the FP64 variant.  Writes at most `cap` bytes (NUL-terminated when
+ * it fits) and the full length to *len.  No reference counterpart. */
+int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, int32_t flags, char *buf,
                           size_t cap, size_t *len);
 
 /* Pin / read the TQEC_MODE_SPEC_F32 configuration (tile, threads, tables in

@@ -22,21 +22,22 @@ struct Layout {
 
 Layout layout(const CompiledPlan &cp, const SpecCfg &cfg) {
     const size_t G = size_t(1) << cfg.glog;
+    const size_t elem = cfg.fp64 ? 8 : 4;
     Layout L;
     L.tab_entries = cp.const_pool.size() + cp.sliced_pool.size();
     size_t off = 0;
     L.work = off;
-    off += round_up((size_t)std::max<int64_t>(1, cp.ws_entries) * G * 4, 16);
+    off += round_up((size_t)std::max<int64_t>(1, cp.ws_entries) * G * elem, 16);
     L.tab = off;
-    if (cfg.tab_lds) off += round_up(std::max<size_t>(1, L.tab_entries) * 4, 16);
+    if (cfg.tab_lds) off += round_up(std::max<size_t>(1, L.tab_entries) * elem, 16);
     L.mx = off;
-    off += (size_t)std::max(1, cp.n_slots) * G * 8;
+    if (!cfg.fp64) off += (size_t)std::max(1, cp.n_slots) * G * 8;
     L.ex = off;
-    off += (size_t)std::max(1, cp.n_slots) * G * 4;
+    if (!cfg.fp64) off += (size_t)std::max(1, cp.n_slots) * G * 4;
     L.syn = off;
     off += round_up((size_t)std::max(1, cp.n_syn) * G, 4);
     L.low = off;
-    off += G * 4;
+    if (!cfg.fp64) off += G * 4;
     L.total = round_up(off, 16);
     return L;
 }
@@ -60,14 +61,14 @@ std::string gather_expr(const OperandDesc &op) {
 // Base pointer of an operand (fused_base).
 void emit_base(std::ostringstream &o, const char *name, const OperandDesc &op, size_t const_n) {
     if (op.kind == kInter) {
-        o << "    const float *" << name << " = work + " << ((long long)op.off << 0) << " * G + g;\n";
+        o << "    const T *" << name << " = work + " << ((long long)op.off << 0) << " * G + g;\n";
     } else if (op.kind == kConst) {
-        o << "    const float *" << name << " = tab + " << (long long)op.off << ";\n";
+        o << "    const T *" << name << " = tab + " << (long long)op.off << ";\n";
     } else {
         o << "    unsigned v_" << name << " = 0u;\n";
         for (int j = 0; j < op.nev; ++j)
             o << "    v_" << name << " |= (unsigned)synl[" << op.ev_bit[j] << " * G + g] << " << j << ";\n";
-        o << "    const float *" << name << " = tab + " << (long long)(const_n + (size_t)op.off) << " + (v_" << name
+        o << "    const T *" << name << " = tab + " << (long long)(const_n + (size_t)op.off) << " + (v_" << name
           << " << " << op.rank << ");\n";
     }
 }
@@ -104,7 +105,7 @@ std::string load_expr(const OperandDesc &op, const char *base, const char *idx, 
         return s.str();
     }
     // T s = 0; for q: s += base[(idx | q << sshift) * G]
-    pre << "        float " << tmp << " = 0.f;\n";
+    pre << "        T " << tmp << " = 0;\n";
     const uint32_t ns = 1u << op.nsingle;
     for (uint32_t q = 0; q < ns; ++q)
         pre << "        " << tmp << " += " << base << "[(" << idx << join << (kconst | (q << op.sshift))
@@ -112,9 +113,41 @@ std::string load_expr(const OperandDesc &op, const char *base, const char *idx, 
     return tmp;
 }
 
-void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep) {
+void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64) {
     const OperandDesc &A = st.a, &B = st.b;
     o << "  { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << "\n";
+    if (fp64) {
+        // generic FP64 step: no scaling, the folded constant exponents restored per entry
+        emit_base(o, "pa", A, const_n);
+        emit_base(o, "pb", B, const_n);
+        o << "    T *pc = work + " << (long long)st.c_off << " * G + g;\n";
+        const long long nc = 1ll << st.rc;
+        const int cexp = A.cexp + B.cexp;
+        const std::string ga = gather_expr(A), gb = gather_expr(B);
+        std::ostringstream b;
+        b << "      const unsigned ar = " << ga << ";\n";
+        b << "      const unsigned br = " << gb << ";\n";
+        b << "      T acc = 0;\n";
+        const uint32_t nkk = 1u << st.nk;
+        for (uint32_t k = 0; k < nkk; ++k) {
+            std::ostringstream pre;
+            const std::string va = load_expr(A, "pa", "ar", k << A.kshift, pre, "sa" + std::to_string(k));
+            const std::string vb = load_expr(B, "pb", "br", k << B.kshift, pre, "sb" + std::to_string(k));
+            b << pre.str();
+            b << "      acc = fma(" << va << ", " << vb << ", acc);\n";
+        }
+        if (cexp != 0) b << "      acc = ldexp(acc, " << cexp << ");\n";
+        b << "      pc[c << GLOG] = acc;\n";
+        if (nc <= (long long)cstep) {
+            o << "    if (c0 < " << nc << ") {\n      const unsigned c = (unsigned)c0;\n" << b.str() << "    }\n";
+        } else {
+            o << "    #pragma unroll 1\n";
+            o << "    for (unsigned c = (unsigned)c0; c < " << nc << "u; c += CSTEP) {\n" << b.str() << "    }\n";
+        }
+        o << "  }\n";
+        o << "  __syncthreads();\n";
+        return;
+    }
     o << "    int ea, eb, na = 0, nb = 0;\n";
     for (int side = 0; side < 2; ++side) {
         const OperandDesc &op = side ? B : A;
@@ -137,15 +170,15 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
     }
     emit_base(o, "pa", A, const_n);
     emit_base(o, "pb", B, const_n);
-    o << "    float *pc = work + " << (long long)st.c_off << " * G + g;\n";
-    o << "    float lmax = 0.f;\n";
+    o << "    T *pc = work + " << (long long)st.c_off << " * G + g;\n";
+    o << "    T lmax = 0;\n";
     const long long nc = 1ll << st.rc;
     const std::string ga = gather_expr(A), gb = gather_expr(B);
     auto body = [&](const char *indent) {
         std::ostringstream b;
         b << indent << "  const unsigned ar = " << ga << ";\n";
         b << indent << "  const unsigned br = " << gb << ";\n";
-        b << indent << "  float acc = 0.f;\n";
+        b << indent << "  T acc = 0;\n";
         const uint32_t nkk = 1u << st.nk;
         for (uint32_t k = 0; k < nkk; ++k) {
             std::ostringstream pre;
@@ -184,7 +217,8 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     const size_t const_n = cp.const_pool.size();
     const int nsteps = (int)cp.steps.size();
     std::ostringstream o;
-    o << "// plan-specialised FP32 TNMMAP kernel (generated)\n";
+    o << "// plan-specialised " << (cfg.fp64 ? "FP64" : "FP32") << " TNMMAP kernel (generated)\n";
+    o << "typedef " << (cfg.fp64 ? "double" : "float") << " T;\n";
     o << "#define G " << (1 << cfg.glog) << "\n#define GLOG " << cfg.glog << "\n#define NT " << cfg.threads
       << "\n#define CSTEP " << (cfg.threads >> cfg.glog) << "\n#define NBITS " << std::max(1, cp.n_syn)
       << "\n#define NOUT " << (1 << cp.n_out) << "\n#define NSLOTS " << std::max(1, cp.n_slots) << "\n#define NSTEPS "
@@ -195,19 +229,21 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
         o << "extern \"C\" __global__ void __launch_bounds__(NT) tqec_spec(\n";
     o << 
          "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
-         "    const int *__restrict__ list_count, const float *__restrict__ gtab, double *__restrict__ mar,\n"
+         "    const int *__restrict__ list_count, const T *__restrict__ gtab, double *__restrict__ mar,\n"
          "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
          "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol) {\n";
     o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
-    o << "  float *work = reinterpret_cast<float *>(smem + " << L.work << ");\n";
+    o << "  T *work = reinterpret_cast<T *>(smem + " << L.work << ");\n";
     if (cfg.tab_lds)
-        o << "  float *tab = reinterpret_cast<float *>(smem + " << L.tab << ");\n";
+        o << "  T *tab = reinterpret_cast<T *>(smem + " << L.tab << ");\n";
     else
-        o << "  const float *tab = gtab;\n";
-    o << "  unsigned long long *mx = reinterpret_cast<unsigned long long *>(smem + " << L.mx << ");\n";
-    o << "  int *ex = reinterpret_cast<int *>(smem + " << L.ex << ");\n";
+        o << "  const T *tab = gtab;\n";
+    if (!cfg.fp64) {
+        o << "  unsigned long long *mx = reinterpret_cast<unsigned long long *>(smem + " << L.mx << ");\n";
+        o << "  int *ex = reinterpret_cast<int *>(smem + " << L.ex << ");\n";
+        o << "  int *lowf = reinterpret_cast<int *>(smem + " << L.low << ");\n";
+    }
     o << "  unsigned char *synl = smem + " << L.syn << ";\n";
-    o << "  int *lowf = reinterpret_cast<int *>(smem + " << L.low << ");\n";
     o << "  const int tid = threadIdx.x;\n  const int g = tid & (G - 1);\n  const int c0 = tid >> GLOG;\n"
          "  const int lane = tid & 63;\n";
     if (cfg.tab_lds) o << "  for (int i = tid; i < TABN; i += NT) tab[i] = gtab[i];\n";
@@ -216,7 +252,7 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     o << "  unsigned iter = 0;\n";
     o << "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter) {\n";
     o << "  const long long b0 = t << GLOG;\n";
-    o << "  const unsigned tag_base = iter * (unsigned)NSTEPS + 1u;\n";
+    if (!cfg.fp64) o << "  const unsigned tag_base = iter * (unsigned)NSTEPS + 1u;\n";
     o << "  if (!list) {\n"
          "    const long long rows = min((long long)G, count - b0);\n"
          "    const unsigned char *src = syn + b0 * NBITS;\n"
@@ -230,20 +266,24 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
          "      const long long q = b0 + r;\n"
          "      synl[(bit << GLOG) + r] = (q < count) ? (syn[(long long)list[q] * NBITS + bit] & 1u) : 0u;\n"
          "    }\n"
-         "  }\n"
-         "  for (int i = tid; i < NSLOTS * G; i += NT) mx[i] = 0ull;\n"
-         "  if (tid < G) lowf[tid] = 0;\n"
-         "  __syncthreads();\n";
-    for (int s = 0; s < nsteps; ++s) emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog);
+         "  }\n";
+    if (!cfg.fp64)
+        o << "  for (int i = tid; i < NSLOTS * G; i += NT) mx[i] = 0ull;\n"
+             "  if (tid < G) lowf[tid] = 0;\n";
+    o << "  __syncthreads();\n";
+    for (int s = 0; s < nsteps; ++s) emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64);
     o << "  if (tid < G) {\n"
          "    const long long q = b0 + tid;\n"
          "    if (q < count) {\n"
          "      const long long b = list ? (long long)list[q] : q;\n";
-    o << "      const int e = ex[" << cp.root_slot << " * G + g];\n";
-    o << "      float best = 0, second = -1;\n"
+    if (cfg.fp64)
+        o << "      const int e = 0;\n";
+    else
+        o << "      const int e = ex[" << cp.root_slot << " * G + g];\n";
+    o << "      T best = 0, second = -1;\n"
          "      int bi = 0;\n"
          "      for (int j = 0; j < NOUT; ++j) {\n";
-    o << "        const float x = work[(" << (long long)cp.root_off << " + j) * G + g];\n";
+    o << "        const T x = work[(" << (long long)cp.root_off << " + j) * G + g];\n";
     o << "        if (mar) mar[b * NOUT + j] = ldexp((double)x, e);\n"
          "        if (j == 0 || x > best) {\n"
          "          if (j > 0) second = best;\n"
@@ -255,16 +295,17 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
          "      }\n"
          "      if (cls) cls[b] = bi;\n"
          "      const bool low = NOUT > 1 && second >= 0 && (double)(best - second) <= tie_tol * (double)best;\n"
-         "      if (ties) ties[b] = low ? 1 : 0;\n"
-         "      if (rl) {\n"
+         "      if (ties) ties[b] = low ? 1 : 0;\n";
+    if (!cfg.fp64)
+        o << "      if (rl) {\n"
          "        const bool rc = !(best > 0) || lowf[tid] != 0 ||\n"
          "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
          "        if (rc) {\n"
          "          const int slot = atomicAdd(rcount, 1);\n"
          "          if (slot < rcap) rl[slot] = (int)b;\n"
          "        }\n"
-         "      }\n"
-         "    }\n"
+         "      }\n";
+    o << "    }\n"
          "  }\n"
          "  __syncthreads();\n"
          "  }\n"

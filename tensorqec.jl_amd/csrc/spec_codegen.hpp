@@ -1,4 +1,4 @@
-// spec_codegen.hpp — plan-specialised FP32 kernel source.
+// spec_codegen.hpp — plan-specialised FP32 / FP64 kernel source.
 //
 // The generic fused kernel interprets the step descriptors at run time: per
 // output entry it walks the operands' bit-field segments, loops over
@@ -9,7 +9,8 @@
 // offsets as immediates, tables staged in LDS), compiled at run time with
 // hiprtc for gfx950.  The arithmetic — order of every sum, fma, scale and
 // comparison — is the generic FP32 kernel's, so both give bit-identical
-// results.
+// results.  The FP64 variant (no scaling) likewise matches the generic FP64
+// fused kernel and serves the FP64 re-check and TQEC_MODE_SPEC_F64.
 #pragma once
 #include <stddef.h>
 
@@ -24,6 +25,7 @@ struct SpecCfg {
     int threads = 128;   // threads per workgroup (>= tile)
     bool tab_lds = true; // constant and sliced tables staged in LDS
     int waves = 0;       // >0: ask the compiler for at least this many waves per SIMD (caps VGPRs)
+    bool fp64 = false;   // FP64 without scaling (the generic FP64 fused kernel's arithmetic)
 };
 
 // Static LDS bytes of the specialised kernel for this configuration.
@@ -31,9 +33,11 @@ size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg);
 
 // HIP source of the kernel `tqec_spec` (extern "C").  Parameters:
 //   (const uint8_t *syn, int64_t n, const int32_t *list, const int32_t *list_count,
-//    const float *tab, double *mar, int32_t *cls, uint8_t *tie,
+//    const T *tab, double *mar, int32_t *cls, uint8_t *tie,
 //    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol)
-// `tab` is the FP32 constant pool followed by the FP32 sliced pool.
+// `tab` is the constant pool followed by the sliced pool, in the kernel's
+// precision (float, or double when cfg.fp64); the FP64 variant ignores the
+// re-check arguments.
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg);
 
 }  // namespace tqec

@@ -542,8 +542,6 @@ struct MixedArgs {
     const MixedBatch *batches;
     const MixedHeader *hdr;
     MixedCounters *ctr;
-    int32_t *rl_row, *rl_batch;          // FP64 re-check list (row, batch), capacity rl_cap
-    int64_t rl_cap;
     unsigned long long *t_done;          // per batch: device wall clock of its last write
     char *slab32, *slab64;               // per-workgroup global slabs
     int64_t slab32_bytes, slab64_bytes;  // bytes per workgroup
@@ -588,16 +586,22 @@ __device__ __forceinline__ int64_t mixed_grab(unsigned long long *queue, int64_t
     return *slot;
 }
 
-// FP32 pass.  Workgroups pull global tiles from a queue; tile t belongs to
-// the batch whose [tile0, next tile0) range holds it.  All plans share one
-// LDS allocation; slab plans use the workgroup's slab instead.
-__global__ void __launch_bounds__(256) mixed_kernel_f32(MixedArgs m) {
+// Workgroups pull global tiles from a queue; tile t belongs to the batch
+// whose [tile0, next tile0) range holds it.  All plans share one LDS
+// allocation; slab plans use the workgroup's slab instead.  Syndromes of a
+// tile whose FP32 margin is below the re-check threshold are re-contracted
+// in FP64 by the same workgroup right after the tile (one syndrome at a
+// time), so a batch is complete when its last tile is.
+__global__ void __launch_bounds__(256) mixed_kernel(MixedArgs m) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int64_t grab;
+    __shared__ int32_t loc_n;
+    __shared__ int32_t loc_rows[64];
     const int nb = m.hdr->nbatches;
     const int64_t ntiles = m.hdr->ntiles;
     uint32_t tag = 1;
     bool first = true;
+    if (threadIdx.x == 0) loc_n = 0;
     for (;;) {
         const int64_t t = mixed_grab(&m.ctr->next_tile, &grab);
         if (t >= ntiles) break;
@@ -612,11 +616,9 @@ __global__ void __launch_bounds__(256) mixed_kernel_f32(MixedArgs m) {
         const MixedBatch bt = m.batches[lo];
         const MixedPlan &p = m.plans[bt.plan];
         FusedArgs a = mixed_fused_args(m, p, bt, true, bt.glog);
-        a.recheck_list = m.rl_row;
-        a.recheck_count = &m.ctr->rl_count;
-        a.recheck_batch = m.rl_batch;
-        a.recheck_cap = m.rl_cap;
-        a.batch_id = lo;
+        a.recheck_list = loc_rows;
+        a.recheck_count = &loc_n;
+        a.recheck_cap = 64;
         const int64_t b0 = (t - bt.tile0) << bt.glog;
         if (p.gw32)
             fused_tile<float, true, true>(a, b0, bt.n, tag, smem,
@@ -624,28 +626,23 @@ __global__ void __launch_bounds__(256) mixed_kernel_f32(MixedArgs m) {
         else
             fused_tile<float, true, false>(a, b0, bt.n, tag, smem);
         tag += (uint32_t)p.nsteps;
+        const int nre = loc_n;
+        if (nre > 0) {
+            FusedArgs r = mixed_fused_args(m, p, bt, false, 0);
+            for (int e = 0; e < nre; ++e) {
+                const int64_t row = loc_rows[e];
+                if (p.gw64)
+                    fused_tile<double, false, true>(
+                        r, row, bt.n, 1u, smem, reinterpret_cast<double *>(m.slab64 + (int64_t)blockIdx.x * m.slab64_bytes));
+                else
+                    fused_tile<double, false, false>(r, row, bt.n, 1u, smem);
+            }
+            if (threadIdx.x == 0) {
+                atomicAdd(&m.ctr->rl_count, nre);
+                loc_n = 0;
+            }
+        }
         if (threadIdx.x == 0) atomicMax(&m.t_done[lo], (unsigned long long)wall_clock64());
-    }
-}
-
-// FP64 re-check of the listed (batch, row) pairs, one syndrome per tile.
-__global__ void __launch_bounds__(256) mixed_kernel_f64(MixedArgs m) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int64_t grab;
-    const int64_t count = min((int64_t)m.ctr->rl_count, m.rl_cap);
-    for (;;) {
-        const int64_t e = mixed_grab(&m.ctr->next_recheck, &grab);
-        if (e >= count) break;
-        const int bi = m.rl_batch[e];
-        const MixedBatch bt = m.batches[bi];
-        const MixedPlan &p = m.plans[bt.plan];
-        FusedArgs a = mixed_fused_args(m, p, bt, false, 0);
-        if (p.gw64)
-            fused_tile<double, false, true>(a, m.rl_row[e], bt.n, 1u, smem,
-                                            reinterpret_cast<double *>(m.slab64 + (int64_t)blockIdx.x * m.slab64_bytes));
-        else
-            fused_tile<double, false, false>(a, m.rl_row[e], bt.n, 1u, smem);
-        if (threadIdx.x == 0) atomicMax(&m.t_done[bi], (unsigned long long)wall_clock64());
     }
 }
 
@@ -793,7 +790,8 @@ int set_error_shared(int code, const std::string &msg) { return set_error(code, 
 namespace {
 
 struct DeviceData {
-    float *tab32 = nullptr;     // FP32 constant pool followed by the sliced pool (specialised kernel)
+    float *tab32 = nullptr;     // constant pool followed by the sliced pool (specialised kernels)
+    double *tab64 = nullptr;
     MapStep *msteps = nullptr;
     int32_t *mlabels = nullptr;
     StepDesc *steps = nullptr;
@@ -862,7 +860,9 @@ struct tqec_plan {
     int n_cu = 0;
     SpecCfg spec_cfg;
     bool spec_cfg_set = false;
-    std::map<std::tuple<int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves)
+    std::map<std::tuple<int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64)
+    SpecCfg spec64_cfg;                 // TQEC_MODE_SPEC_F64
+    bool spec64_cfg_set = false;
 };
 
 namespace {
@@ -895,6 +895,7 @@ int ensure_device(tqec_plan *p, int device, DeviceData **out) {
         std::vector<double> both(cp.const_pool);
         both.insert(both.end(), cp.sliced_pool.begin(), cp.sliced_pool.end());
         if ((rc = upload<float>(both, &dd.tab32))) return rc;
+        if ((rc = upload<double>(both, &dd.tab64))) return rc;
     }
     if (cp.semiring == TQEC_SEMIRING_MAX) {
         TQ_HIP(hipMalloc((void **)&dd.msteps, std::max<size_t>(1, cp.map_steps.size()) * sizeof(MapStep)));
@@ -1272,7 +1273,7 @@ int spec_code(const std::string &src, std::vector<char> &code) {
 bool spec_fits(const tqec_plan *p, const SpecCfg &c) { return spec_lds_bytes(p->cp, c) <= p->lds_limit; }
 
 // Default before tuning: as choose_fused, preferring tables in LDS.
-SpecCfg choose_spec(const tqec_plan *p) {
+SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
     for (int div : {4, 2, 1}) {
         for (int gl = 6; gl >= 0; --gl) {
             for (bool tab : {true, false}) {
@@ -1280,6 +1281,7 @@ SpecCfg choose_spec(const tqec_plan *p) {
                 c.glog = gl;
                 c.threads = std::max(64, std::min(128, 1 << gl));
                 c.tab_lds = tab;
+                c.fp64 = fp64;
                 if (spec_lds_bytes(p->cp, c) <= p->lds_limit / div) return c;
             }
         }
@@ -1290,7 +1292,7 @@ SpecCfg choose_spec(const tqec_plan *p) {
 }
 
 int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
-    const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves);
+    const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0);
     auto it = p->spec.find(key);
     if (it != p->spec.end()) {
         *out = &it->second;
@@ -1315,6 +1317,67 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     return TQEC_OK;
 }
 
+// Launch a specialised kernel over n syndromes (or the listed ones).
+int spec_launch(SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n, const int32_t *in_list,
+                const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie, int32_t *rl, int32_t *rcount,
+                int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream) {
+    const int64_t ntiles = (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
+    const unsigned char *a_syn = d_syn;
+    long long a_n = (long long)n;
+    const int *a_list = in_list, *a_count = in_count;
+    const void *a_tab = k->cfg.fp64 ? (const void *)dd->tab64 : (const void *)dd->tab32;
+    double *a_mar = d_mar;
+    int *a_cls = d_cls;
+    unsigned char *a_tie = d_tie;
+    int *a_rl = rl, *a_rc = rcount;
+    long long a_cap = (long long)rcap;
+    double a_tt = tie_tol, a_rt = recheck_tol;
+    void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt};
+    TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
+    return TQEC_OK;
+}
+
+// FP64 re-check with the specialised FP64 kernel, one syndrome per 64-thread
+// workgroup: the list is short, so latency, not throughput, matters.
+int spec_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+                 int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+    SpecCfg c;
+    c.glog = 0;
+    c.threads = 64;
+    c.tab_lds = false;
+    c.fp64 = true;
+    SpecKernel *k;
+    int rc;
+    if ((rc = spec_get(p, w.device, c, &k))) return rc;
+    if ((rc = record(p, w, 2, stream))) return rc;
+    if ((rc = spec_launch(k, dd, d_syn, n, w.list, w.count, d_mar, d_cls, d_tie, nullptr, nullptr, 0, p->tie_tol,
+                          p->recheck_tol, stream)))
+        return rc;
+    if ((rc = record(p, w, 3, stream))) return rc;
+    TQ_HIP(hipMemcpyAsync(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    return TQEC_OK;
+}
+
+int run_spec64(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
+               int32_t *d_cls, uint8_t *d_tie, hipStream_t stream, const int32_t *in_list = nullptr,
+               const int32_t *in_count = nullptr) {
+    if (!p->spec64_cfg_set) {
+        p->spec64_cfg = choose_spec(p, true);
+        p->spec64_cfg_set = true;
+    }
+    if (p->spec64_cfg.glog < 0)
+        return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP64)");
+    SpecKernel *k;
+    int rc;
+    if ((rc = spec_get(p, w.device, p->spec64_cfg, &k))) return rc;
+    if ((rc = record(p, w, 0, stream))) return rc;
+    if ((rc = spec_launch(k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, nullptr, nullptr, 0, p->tie_tol,
+                          p->recheck_tol, stream)))
+        return rc;
+    return record(p, w, 1, stream);
+}
+
 int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream, const int32_t *in_list = nullptr,
              const int32_t *in_count = nullptr) {
@@ -1330,23 +1393,12 @@ int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
     int rc;
     if ((rc = spec_get(p, w.device, p->spec_cfg, &k))) return rc;
     if ((rc = prepare_recheck(w, n, stream))) return rc;
-    const int64_t ntiles = (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
-    const unsigned char *a_syn = d_syn;
-    long long a_n = (long long)n;
-    const int *a_list = in_list, *a_count = in_count;
-    const float *a_tab = dd->tab32;
-    double *a_mar = d_mar;
-    int *a_cls = d_cls;
-    unsigned char *a_tie = d_tie;
-    int *a_rl = w.list, *a_rc = w.count;
-    long long a_cap = (long long)w.list_cap;
-    double a_tt = p->tie_tol, a_rt = p->recheck_tol;
-    void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt};
     if ((rc = record(p, w, 0, stream))) return rc;
-    TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
+    if ((rc = spec_launch(k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, w.list, w.count, w.list_cap,
+                          p->tie_tol, p->recheck_tol, stream)))
+        return rc;
     if ((rc = record(p, w, 1, stream))) return rc;
-    return enqueue_recheck(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+    return spec_recheck(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
 }
 
 // Compile a set of specialised configurations on host threads (hiprtc
@@ -1474,20 +1526,21 @@ int decode_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int mode, doubl
     if (mode == TQEC_MODE_FUSED_F64 || mode == TQEC_MODE_FUSED_F32)
         return run_fused(p, dd, w, d_syn, n, mode, d_mar, d_cls, d_tie, stream);
     if (mode == TQEC_MODE_SPEC_F32) return run_spec(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+    if (mode == TQEC_MODE_SPEC_F64) return run_spec64(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
     return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
 }
 
 // Mixed-code executor state.  Each of two call slots owns the per-call
-// tables (header, batch table, counters, batch clocks, re-check list) and a
-// pinned host staging copy of header + batch table, so a call can be staged
-// while the previous one still runs; the slabs are shared because calls on
-// one stream execute in order.
+// tables (header, batch table, counters, batch clocks) and a pinned host
+// staging copy of header + batch table, so a call can be staged while the
+// previous one still runs; the slabs are shared because calls execute in
+// order.
 struct MixedSlot {
     void *h_stage = nullptr;            // pinned: MixedHeader | MixedBatch[max_batches]
     void *d_table = nullptr;            // device copy of the same
     MixedCounters *ctr = nullptr;
     unsigned long long *t_done = nullptr;
-    int32_t *rl_row = nullptr, *rl_batch = nullptr;
+    std::vector<int32_t> perm;          // staged position -> caller's batch index
     hipEvent_t staged = nullptr;        // the device has consumed h_stage
     hipStream_t graph_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1514,8 +1567,10 @@ struct tqec_mixed {
     int device = -1;
     int32_t max_batches = 0;
     int64_t max_rows = 0;
-    size_t lds32 = 0, lds64 = 0;
-    int blocks32 = 0, blocks64 = 0;
+    size_t lds = 0;                     // dynamic LDS of the kernel (FP32 tiles and FP64 re-checks)
+    int blocks = 0;
+    int order = TQEC_MIXED_ORDER_CHEAPEST_FIRST;
+    std::vector<double> cost;           // per plan: FP32 work per syndrome (multiply-adds + entries)
     MixedPlan *d_plans = nullptr;
     char *slab32 = nullptr, *slab64 = nullptr;
     size_t slab32_bytes = 0, slab64_bytes = 0;
@@ -1539,9 +1594,6 @@ MixedArgs mixed_args(tqec_mixed *m, MixedSlot &sl) {
     a.hdr = reinterpret_cast<const MixedHeader *>(sl.d_table);
     a.batches = reinterpret_cast<const MixedBatch *>(reinterpret_cast<const char *>(sl.d_table) + sizeof(MixedHeader));
     a.ctr = sl.ctr;
-    a.rl_row = sl.rl_row;
-    a.rl_batch = sl.rl_batch;
-    a.rl_cap = m->max_rows;
     a.t_done = sl.t_done;
     a.slab32 = m->slab32;
     a.slab64 = m->slab64;
@@ -1550,8 +1602,8 @@ MixedArgs mixed_args(tqec_mixed *m, MixedSlot &sl) {
     return a;
 }
 
-// The whole call: stage the tables, clear the counters, FP32 pass, FP64
-// re-check.  Enqueued directly or recorded once into a graph per slot.
+// The whole call: stage the tables, clear the counters, decode.  Enqueued
+// directly or recorded once into a graph per slot.
 int mixed_enqueue(tqec_mixed *m, MixedSlot &sl, hipStream_t stream) {
     const size_t tb = mixed_table_bytes(m->max_batches);
     TQ_HIP(hipMemcpyAsync(sl.d_table, sl.h_stage, tb, hipMemcpyHostToDevice, stream));
@@ -1559,8 +1611,7 @@ int mixed_enqueue(tqec_mixed *m, MixedSlot &sl, hipStream_t stream) {
     hipLaunchKernelGGL(mixed_reset_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, stream, sl.ctr, sl.t_done,
                        cap);
     MixedArgs a = mixed_args(m, sl);
-    hipLaunchKernelGGL(mixed_kernel_f32, dim3((unsigned)m->blocks32), dim3(kMixedThreads), m->lds32, stream, a);
-    hipLaunchKernelGGL(mixed_kernel_f64, dim3((unsigned)m->blocks64), dim3(kMixedThreads), m->lds64, stream, a);
+    hipLaunchKernelGGL(mixed_kernel, dim3((unsigned)m->blocks), dim3(kMixedThreads), m->lds, stream, a);
     TQ_HIP(hipGetLastError());
     return TQEC_OK;
 }
@@ -1573,7 +1624,7 @@ void mixed_free(tqec_mixed *m) {
         if (sl.graph) (void)hipGraphDestroy(sl.graph);
         if (sl.staged) (void)hipEventDestroy(sl.staged);
         if (sl.h_stage) (void)hipHostFree(sl.h_stage);
-        for (void *q : {sl.d_table, (void *)sl.ctr, (void *)sl.t_done, (void *)sl.rl_row, (void *)sl.rl_batch})
+        for (void *q : {sl.d_table, (void *)sl.ctr, (void *)sl.t_done})
             if (q) (void)hipFree(q);
     }
     if (m->d_plans) (void)hipFree(m->d_plans);
@@ -1632,6 +1683,7 @@ void tqec_plan_destroy(tqec_plan *p) {
         if (hipSetDevice(kv.first) != hipSuccess) continue;
         DeviceData &d = kv.second;
         if (d.tab32) (void)hipFree(d.tab32);
+        if (d.tab64) (void)hipFree(d.tab64);
         (void)hipFree(d.steps);
         if (d.msteps) (void)hipFree(d.msteps);
         if (d.mlabels) (void)hipFree(d.mlabels);
@@ -1693,7 +1745,7 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
     if (n > 0 && !syn) return set_error(TQEC_ENULL, "null syndrome pointer");
     if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
-    if (mode < 0 || mode > TQEC_MODE_SPEC_F32) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+    if (mode < 0 || mode > TQEC_MODE_SPEC_F64) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
     if (p->cp.semiring != TQEC_SEMIRING_SUM)
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1806,7 +1858,8 @@ int tqec_decode_list_device(tqec_plan *p, const uint8_t *d_syn, const int32_t *d
     if (!p) return set_error(TQEC_ENULL, "null plan");
     if (p->cp.semiring != TQEC_SEMIRING_SUM)
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
-    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32 && mode != TQEC_MODE_SPEC_F32)
+    if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32 && mode != TQEC_MODE_SPEC_F32 &&
+        mode != TQEC_MODE_SPEC_F64)
         return set_error(TQEC_EINVAL, "list decoding runs on the fused executors");
     if (n_max < 0 || n_max > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "n_max out of range");
     if (n_max > 0 && (!d_syn || !d_list || !d_count)) return set_error(TQEC_ENULL, "null device pointer");
@@ -1822,6 +1875,8 @@ int tqec_decode_list_device(tqec_plan *p, const uint8_t *d_syn, const int32_t *d
     if (n_max == 0) return TQEC_OK;
     if (mode == TQEC_MODE_SPEC_F32)
         return run_spec(p, dd, w, d_syn, n_max, d_mar, d_cls, d_tie, (hipStream_t)stream, d_list, d_count);
+    if (mode == TQEC_MODE_SPEC_F64)
+        return run_spec64(p, dd, w, d_syn, n_max, d_mar, d_cls, d_tie, (hipStream_t)stream, d_list, d_count);
     return run_fused(p, dd, w, d_syn, n_max, mode, d_mar, d_cls, d_tie, (hipStream_t)stream, d_list, d_count);
 }
 
@@ -2074,20 +2129,16 @@ int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batc
         q.gw64 = c.gw64;
         q.tie_tol = p->tie_tol;
         q.recheck_tol = p->recheck_tol;
-        m->lds32 = std::max(m->lds32, c.lds32);
-        m->lds64 = std::max(m->lds64, c.lds64);
+        m->lds = std::max(m->lds, std::max(c.lds32, c.lds64));
+        m->cost.push_back((double)cp.dep_tc + (double)cp.dep_rw);
         m->slab32_bytes = std::max(m->slab32_bytes, c.slab32);
         m->slab64_bytes = std::max(m->slab64_bytes, c.slab64);
     }
-    if (m->lds32 > lds_limit - 256 || m->lds64 > lds_limit - 256)
-        return set_error(TQEC_EUNSUPPORTED, "a plan's scales and syndrome bits exceed the LDS");
-    // (the kernels' own static LDS is one 8-byte work-queue slot)
-    TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel_f32, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(lds_limit - 256)));
-    TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel_f64, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(lds_limit - 256)));
-    m->blocks32 = n_cu * resident_per_cu(m->lds32 + 64, kMixedThreads, lds_limit);
-    m->blocks64 = n_cu * resident_per_cu(m->lds64 + 64, kMixedThreads, lds_limit);
+    // (the kernel's own static LDS: the work-queue slot and the re-check list)
+    if (m->lds > lds_limit - 512) return set_error(TQEC_EUNSUPPORTED, "a plan's scales and syndrome bits exceed the LDS");
+    TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(lds_limit - 512)));
+    m->blocks = n_cu * resident_per_cu(m->lds + 512, kMixedThreads, lds_limit);
     tqec_mixed *mp = m.get();
     auto fail = [&](int code) {
         mixed_free(mp);
@@ -2102,8 +2153,8 @@ int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batc
     } while (0)
     TQ_MX(hipMalloc((void **)&mp->d_plans, hp.size() * sizeof(MixedPlan)));
     TQ_MX(hipMemcpy(mp->d_plans, hp.data(), hp.size() * sizeof(MixedPlan), hipMemcpyHostToDevice));
-    if (mp->slab32_bytes) TQ_MX(hipMalloc((void **)&mp->slab32, mp->slab32_bytes * (size_t)mp->blocks32));
-    if (mp->slab64_bytes) TQ_MX(hipMalloc((void **)&mp->slab64, mp->slab64_bytes * (size_t)mp->blocks64));
+    if (mp->slab32_bytes) TQ_MX(hipMalloc((void **)&mp->slab32, mp->slab32_bytes * (size_t)mp->blocks));
+    if (mp->slab64_bytes) TQ_MX(hipMalloc((void **)&mp->slab64, mp->slab64_bytes * (size_t)mp->blocks));
     const size_t tb = mixed_table_bytes(max_batches);
     for (MixedSlot &sl : mp->slot) {
         TQ_MX(hipHostMalloc(&sl.h_stage, tb, hipHostMallocDefault));
@@ -2111,8 +2162,6 @@ int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batc
         TQ_MX(hipMalloc(&sl.d_table, tb));
         TQ_MX(hipMalloc((void **)&sl.ctr, sizeof(MixedCounters)));
         TQ_MX(hipMalloc((void **)&sl.t_done, (size_t)max_batches * sizeof(unsigned long long)));
-        TQ_MX(hipMalloc((void **)&sl.rl_row, (size_t)max_rows * sizeof(int32_t)));
-        TQ_MX(hipMalloc((void **)&sl.rl_batch, (size_t)max_rows * sizeof(int32_t)));
         TQ_MX(hipEventCreateWithFlags(&sl.staged, hipEventDisableTiming));
     }
 #undef TQ_MX
@@ -2162,9 +2211,15 @@ int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int
     char *h = reinterpret_cast<char *>(sl.h_stage);
     MixedHeader *hdr = reinterpret_cast<MixedHeader *>(h);
     MixedBatch *tb = reinterpret_cast<MixedBatch *>(h + sizeof(MixedHeader));
+    sl.perm.resize(n_batches);
+    for (int32_t i = 0; i < n_batches; ++i) sl.perm[i] = i;
+    if (m->order == TQEC_MIXED_ORDER_CHEAPEST_FIRST)
+        std::stable_sort(sl.perm.begin(), sl.perm.end(), [&](int32_t x, int32_t y) {
+            return (double)batches[x].n * m->cost[batches[x].plan] < (double)batches[y].n * m->cost[batches[y].plan];
+        });
     int64_t tiles = 0;
     for (int32_t i = 0; i < n_batches; ++i) {
-        const tqec_mixed_batch &b = batches[i];
+        const tqec_mixed_batch &b = batches[sl.perm[i]];
         int gl = 0;
         while (gl < m->cfg[b.plan].glog32 && (int64_t(1) << gl) < b.n) ++gl;
         MixedBatch &q = tb[i];
@@ -2228,9 +2283,9 @@ int tqec_mixed_last_call(tqec_mixed *m, double *batch_us, int32_t n_batches, int
         std::vector<unsigned long long> t(n_batches);
         TQ_HIP(hipMemcpy(t.data(), sl.t_done, (size_t)n_batches * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         for (int32_t i = 0; i < n_batches; ++i)
-            batch_us[i] = (t[i] == 0 || c.t_start == ~0ull || t[i] < c.t_start)
-                              ? 0.0
-                              : (double)(t[i] - c.t_start) * 1000.0 / m->clock_khz;
+            batch_us[sl.perm[i]] = (t[i] == 0 || c.t_start == ~0ull || t[i] < c.t_start)
+                                       ? 0.0
+                                       : (double)(t[i] - c.t_start) * 1000.0 / m->clock_khz;
     }
     return TQEC_OK;
 }
@@ -2242,8 +2297,8 @@ void tqec_mixed_destroy(tqec_mixed *m) {
 }
 
 
-int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int32_t tables_in_lds, char *buf,
-                          size_t cap, size_t *len) {
+int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int32_t flags, char *buf, size_t cap,
+                          size_t *len) {
     if (!p) return set_error(TQEC_ENULL, "null plan");
     if (p->cp.semiring != TQEC_SEMIRING_SUM) return set_error(TQEC_EINVAL, "max-product plans have no FP32 kernel");
     int gl = 0;
@@ -2254,7 +2309,8 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     SpecCfg c;
     c.glog = gl;
     c.threads = threads;
-    c.tab_lds = tables_in_lds != 0;
+    c.tab_lds = (flags & 1) != 0;
+    c.fp64 = (flags & 2) != 0;
     std::string src;
     try {
         src = spec_source(p->cp, c);
@@ -2307,6 +2363,14 @@ int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t
     if (threads) *threads = p->spec_cfg.threads;
     if (tables_in_lds) *tables_in_lds = p->spec_cfg.tab_lds ? 1 : 0;
     if (waves) *waves = p->spec_cfg.waves;
+    return TQEC_OK;
+}
+
+int tqec_mixed_set_order(tqec_mixed *m, int32_t order) {
+    if (!m) return set_error(TQEC_ENULL, "null executor");
+    if (order != TQEC_MIXED_ORDER_GIVEN && order != TQEC_MIXED_ORDER_CHEAPEST_FIRST)
+        return set_error(TQEC_EINVAL, "unknown order");
+    m->order = order;
     return TQEC_OK;
 }
 }  // extern "C"

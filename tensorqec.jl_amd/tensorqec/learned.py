@@ -129,7 +129,7 @@ class TrainReport:
 
 def train_on_tn(mlp: HipMLP, ct, sampler, n_samples: int, batch: int = 8192, lr: float = 1e-3,
                 seed: int = 1, first: int = 0, lr_final: float = 1e-4, log_every: int = 50,
-                torch=None) -> TrainReport:
+                torch=None, mode: str = "fused_f32", max_seconds: Optional[float] = None) -> TrainReport:
     """Train on TN-labelled syndromes: each batch is sampled on the device
     (seeded), labelled by the fused TN decoder (FP32 marginals normalised to
     class posteriors) and used for one Adam step of soft cross-entropy."""
@@ -146,9 +146,14 @@ def train_on_tn(mlp: HipMLP, ct, sampler, n_samples: int, batch: int = 8192, lr:
     losses = []
     stream = torch.cuda.current_stream().cuda_stream
     t0 = time.perf_counter()
+    done = 0
     for s in range(steps):
+        if max_seconds is not None and s % 64 == 63:
+            torch.cuda.synchronize()
+            if time.perf_counter() - t0 > max_seconds:
+                break
         sampler.sample(seed, first + s * batch, batch, S.data_ptr(), stream=stream)
-        ct.hip.decode_device(S.data_ptr(), batch, "fused_f32", M.data_ptr(), C.data_ptr(), T.data_ptr(), stream)
+        ct.hip.decode_device(S.data_ptr(), batch, mode, M.data_ptr(), C.data_ptr(), T.data_ptr(), stream)
         posteriors(M.data_ptr(), batch, nclass, Q.data_ptr(), stream)
         frac = s / max(1, steps - 1)
         rate = lr * (lr_final / lr) ** frac            # geometric decay
@@ -156,19 +161,20 @@ def train_on_tn(mlp: HipMLP, ct, sampler, n_samples: int, batch: int = 8192, lr:
         loss = mlp.train_step(S.data_ptr(), Q.data_ptr(), batch, rate, want_loss=want, stream=stream)
         if want:
             losses.append((s, loss))
+        done = s + 1
     torch.cuda.synchronize()
-    return TrainReport(steps * batch, steps, time.perf_counter() - t0, losses)
+    return TrainReport(done * batch, done, time.perf_counter() - t0, losses)
 
 
 class GatedDecoder:
     """MLP classes where the network is confident, TN (fused FP32 with FP64
     re-check) for the rest — all on the device, no host round trip."""
 
-    def __init__(self, mlp: HipMLP, ct, threshold: float, capacity: int, torch=None):
+    def __init__(self, mlp: HipMLP, ct, threshold: float, capacity: int, torch=None, mode: str = "fused_f32"):
         if torch is None:
             import torch
         dev = torch.device("cuda:0")
-        self.mlp, self.ct, self.threshold = mlp, ct, float(threshold)
+        self.mlp, self.ct, self.threshold, self.mode = mlp, ct, float(threshold), mode
         self.reject = torch.empty(capacity, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
         self.capacity = capacity
@@ -178,7 +184,7 @@ class GatedDecoder:
         if n > self.capacity:
             raise ValueError("batch larger than the gate's capacity")
         self.mlp.gate(d_syn, n, self.threshold, d_cls, d_conf, self.reject.data_ptr(), self.count.data_ptr(), stream)
-        self.ct.hip.decode_list_device(d_syn, self.reject.data_ptr(), self.count.data_ptr(), n, "fused_f32",
+        self.ct.hip.decode_list_device(d_syn, self.reject.data_ptr(), self.count.data_ptr(), n, self.mode,
                                        d_mar, d_cls, d_tie, stream)
 
     def fallbacks(self) -> int:

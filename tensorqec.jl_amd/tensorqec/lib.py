@@ -26,8 +26,9 @@ MODE_FUSED_F64 = 1
 MODE_FUSED_F32 = 2
 MODE_WIDE_F32 = 3
 MODE_SPEC_F32 = 4
+MODE_SPEC_F64 = 5
 MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32,
-         "wide_f32": MODE_WIDE_F32, "spec_f32": MODE_SPEC_F32}
+         "wide_f32": MODE_WIDE_F32, "spec_f32": MODE_SPEC_F32, "spec_f64": MODE_SPEC_F64}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -162,6 +163,7 @@ EXPORTS = [
     ("tqec_mixed_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     ("tqec_mixed_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("tqec_mixed_set_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("tqec_mixed_plan_config", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p]),
     ("tqec_mixed_decode_device", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MixedBatch), ctypes.c_int32,
                                                  ctypes.c_void_p]),
@@ -328,14 +330,14 @@ class HipPlan:
         check(load().tqec_plan_spec_config(self._h, *[ctypes.byref(x) for x in v]))
         return {"tile": v[0].value, "threads": v[1].value, "tables_in_lds": bool(v[2].value), "waves": v[3].value}
 
-    def spec_source(self, tile: int, threads: int, tables_in_lds: bool = True) -> str:
-        """HIP source of the plan-specialised FP32 kernel for a tile choice."""
+    def spec_source(self, tile: int, threads: int, tables_in_lds: bool = True, fp64: bool = False) -> str:
+        """HIP source of the plan-specialised kernel for a tile choice."""
+        flags = int(bool(tables_in_lds)) | (2 if fp64 else 0)
         n = ctypes.c_size_t()
-        check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), int(bool(tables_in_lds)), None, 0,
-                                           ctypes.byref(n)))
+        check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), flags, None, 0, ctypes.byref(n)))
         buf = ctypes.create_string_buffer(n.value + 1)
-        check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), int(bool(tables_in_lds)), buf,
-                                           n.value + 1, ctypes.byref(n)))
+        check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), flags, buf, n.value + 1,
+                                           ctypes.byref(n)))
         return buf.value.decode()
 
     def set_timing(self, enable: bool) -> None:
@@ -487,6 +489,9 @@ class HipMixed:
 
     def set_graph(self, enable: bool) -> None:
         check(load().tqec_mixed_set_graph(self._h, 1 if enable else 0))
+
+    def set_order(self, cheapest_first: bool) -> None:
+        check(load().tqec_mixed_set_order(self._h, 1 if cheapest_first else 0))
 
     def plan_config(self, i: int):
         """(tile, slab flags) of plan i: bit 0 FP32 slab, bit 1 FP64 slab."""

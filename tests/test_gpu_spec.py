@@ -1,6 +1,6 @@
-"""Plan-specialised FP32 kernel (TQEC_MODE_SPEC_F32) against the generic
-FP32 fused kernel on the same syndromes: marginals, classes and tie flags
-must be bit-identical — the generated kernel performs the same operations in
+"""Plan-specialised kernels (TQEC_MODE_SPEC_F32 / SPEC_F64) against the
+generic fused kernels of the same precision on the same syndromes:
+marginals, classes and tie flags must be bit-identical — the generated kernel performs the same operations in
 the same order, with the descriptor values substituted.  Several tile
 configurations and the list (gated fall-back) entry point are covered; the
 oracle comparison of the mode itself runs in the parity tests."""
@@ -51,6 +51,16 @@ def test_spec_bit_identical_to_generic(name, code, p, n):
     except L.TqecError as e:
         assert e.code == L.TQEC_EUNSUPPORTED
         pytest.skip(f"{name}: plan does not fit the fused executors")
+    say("  spec_f64 vs fused_f64")
+    try:
+        ref64 = run(torch, ct, dS, n, "fused_f64")
+    except L.TqecError as e:
+        assert e.code == L.TQEC_EUNSUPPORTED
+        ref64 = None
+    if ref64 is not None:
+        got64 = run(torch, ct, dS, n, "spec_f64")
+        for a, b in zip(got64, ref64):
+            assert np.array_equal(a, b), (name, "f64")
     configs = [None, (1, 64), (4, 128), (16, 256)]
     for cfg in configs:
         if cfg is not None:
