@@ -356,13 +356,16 @@ def main():
     ap.add_argument("--config", default=None,
                     help="JSON line of an earlier run: reuse its plan and tile choice (no autotuning)")
     ap.add_argument("--plan", default=None,
-                    help="SEED,SC_TARGET,MODE,TILE,THREADS[,TABLES_IN_LDS,WAVES]: pin the plan, executor and tile "
+                    help="SEED,SC_TARGET,MODE,TILE,THREADS[,TABLES_IN_LDS,WAVES[,UNROLL[,TABULATE_BITS]]]: pin the plan, "
+                         "executor and tile "
                          "(no autotuning); SC_TARGET may be none; MODE fused_f32 or spec_f32")
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
     ap.add_argument("--only", default=None, choices=["stream", "learned", "dem"],
                     help="run one secondary workload alone and print its JSON")
+    ap.add_argument("--dem-d", type=int, default=3, help="distance of the circuit-level DEM workload")
+    ap.add_argument("--dem-rounds", type=int, default=3, help="syndrome rounds of the circuit-level DEM workload")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed steps (HBM counter calibration)")
     args = ap.parse_args()
@@ -375,8 +378,9 @@ def main():
     import torch
 
     if args.only:
+        kw = {"dem": {"d": args.dem_d, "rounds": args.dem_rounds}}.get(args.only, {})
         fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg}[args.only]
-        print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed)))
+        print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 
     from tensorqec import decoders as D
@@ -407,10 +411,15 @@ def main():
         if len(f) > 5:
             pinned["tables_in_lds"] = bool(int(f[5]))
             pinned["waves"] = int(f[6])
+        if len(f) > 7:
+            pinned["unroll"] = int(f[7])
+        if len(f) > 8:
+            pinned["tabulate_bits"] = int(f[8])
     if pinned:
-        cands = [(pinned["seed"], pinned["sc_target"])]
+        cands = [(pinned["seed"], pinned["sc_target"], pinned.get("tabulate_bits", 4))]
     else:
-        cands = [(1, 10), (2, 10), (3, 10), (1, 9), (3, None), (2, 9), (1, None)]
+        cands = [(1, 10, 4), (2, 10, 4), (3, 10, 4), (1, 9, 4), (3, None, 4), (2, 9, 4), (1, None, 4), (1, None, 3),
+                 (3, None, 3), (1, None, 2)]
     outs = [torch.empty((B, 4), dtype=torch.float64, device=dev), torch.empty(B, dtype=torch.int32, device=dev),
             torch.empty(B, dtype=torch.uint8, device=dev)]
 
@@ -428,15 +437,16 @@ def main():
     # the generic fused kernel; the two fastest are also autotuned with the
     # plan-specialised kernel; the fastest (plan, executor, tile) is kept.
     tried = []
-    for seed, sct in cands:
-        ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.TreeSA(seed=seed, sc_target=sct)), t, em)
+    for seed, sct, tab in cands:
+        ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.TreeSA(seed=seed, sc_target=sct), tabulate_bits=tab), t,
+                       em)
         st = ct.hip.stats()
         if pinned:
-            cfg = {"seed": seed, "sc_target": sct, "mode": pinned["mode"], "tile": pinned["tile"],
+            cfg = {"seed": seed, "sc_target": sct, "tabulate_bits": tab, "mode": pinned["mode"], "tile": pinned["tile"],
                    "threads": pinned["threads"]}
             if pinned["mode"] == "spec_f32":
                 ct.hip.set_spec(pinned["tile"], pinned["threads"], pinned.get("tables_in_lds", False),
-                                pinned.get("waves", 0))
+                                pinned.get("waves", 0), pinned.get("slab", False), pinned.get("unroll", 1))
                 cfg.update(ct.hip.spec_config())
             else:
                 ct.hip.set_tile("fused_f32", pinned["tile"], pinned["threads"])
@@ -444,19 +454,21 @@ def main():
             continue
         tile, thr = ct.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
         ms = time_mode(ct, "fused_f32")
-        print(f"[bench] candidate seed={seed} sc_target={sct}: steps={st['n_dep_steps']} macs={st['dep_tc']:.0f} "
-              f"lds/syn={st['fused_lds_entries']} fused_f32 tile={tile} threads={thr} {ms:.3f} ms/batch",
-              file=sys.stderr, flush=True)
-        tried.append((ms, ct, {"seed": seed, "sc_target": sct, "mode": "fused_f32", "tile": tile, "threads": thr}))
+        print(f"[bench] candidate seed={seed} sc_target={sct} tabulate_bits={tab}: steps={st['n_dep_steps']} "
+              f"macs={st['dep_tc']:.0f} lds/syn={st['fused_lds_entries']} tables={st['table_entries']} "
+              f"fused_f32 tile={tile} threads={thr} {ms:.3f} ms/batch", file=sys.stderr, flush=True)
+        tried.append((ms, ct, {"seed": seed, "sc_target": sct, "tabulate_bits": tab, "mode": "fused_f32", "tile": tile,
+                               "threads": thr}))
     tried.sort(key=lambda x: x[0])
     best = tried[0]
     if not pinned and not args.no_spec:
-        for ms_f, ct, cfg in tried[:2]:
+        for ms_f, ct, cfg in tried[:3]:
             t1 = time.time()
             ct.hip.autotune("spec_f32", S_dev[0].data_ptr(), B, stream)
             sc = ct.hip.spec_config()
             ms = time_mode(ct, "spec_f32")
-            print(f"[bench] candidate seed={cfg['seed']} sc_target={cfg['sc_target']}: spec_f32 {sc} {ms:.3f} ms/batch "
+            print(f"[bench] candidate seed={cfg['seed']} sc_target={cfg['sc_target']} tabulate_bits={cfg['tabulate_bits']}: "
+                  f"spec_f32 {sc} {ms:.3f} ms/batch "
                   f"(tuned in {time.time() - t1:.1f}s)", file=sys.stderr, flush=True)
             if ms < best[0]:
                 best = (ms, ct, dict(cfg, mode="spec_f32", **sc))
@@ -593,7 +605,7 @@ def main():
                    "plain": "plain_f64 (reference network, one-hot evidence, one kernel per step)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "tqec_spec" if mode_c == "spec_f32" else "fused_kernel<float, true>",
+                     "kernel": "tqec_spec_f32" if mode_c == "spec_f32" else "fused_kernel<float, true>",
                      "kernel_ms_avg": avg_ms,
                      "flops_per_syndrome": flops_per_syn,
                      "compute": "FP32 vector ALU with LDS-resident operands (no MFMA on this path)"},

@@ -267,18 +267,27 @@ int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int
 int tqec_mixed_last_call(tqec_mixed *m, double *batch_us, int32_t n_batches, int64_t *recheck_count);
 void tqec_mixed_destroy(tqec_mixed *m);
 
+/* Flags of a plan-specialised kernel configuration. */
+#define TQEC_SPEC_TABLES_IN_LDS 1  /* constant and sliced tables staged in LDS        */
+#define TQEC_SPEC_FP64 2           /* FP64 variant (tqec_plan_spec_source only)       */
+#define TQEC_SPEC_SLAB 4           /* intermediates in a per-workgroup global slab,
+                                      for plans too wide for LDS                      */
+
 /* Source of the plan-specialised kernel for a tile configuration
- * (inspection and offline compilation checks); `flags` bit 0: tables in LDS,
- * bit 1: the FP64 variant.  Writes at most `cap` bytes (NUL-terminated when
- * it fits) and the full length to *len.  No reference counterpart. */
+ * (inspection and offline compilation checks); `flags` as above, with the
+ * unroll of the per-lane output loop in bits 8-15 (0 = 1).  Writes at most `cap` bytes
+ * (NUL-terminated when it fits) and the full length to *len.  No reference
+ * counterpart. */
 int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, int32_t flags, char *buf,
                           size_t cap, size_t *len);
 
-/* Pin / read the TQEC_MODE_SPEC_F32 configuration (tile, threads, tables in
- * LDS or global memory, minimum waves per SIMD requested from the compiler,
- * 0 = none).  No reference counterpart. */
-int tqec_plan_set_spec(tqec_plan *plan, int32_t tile, int32_t threads, int32_t tables_in_lds, int32_t waves);
-int tqec_plan_spec_config(tqec_plan *plan, int32_t *tile, int32_t *threads, int32_t *tables_in_lds, int32_t *waves);
+/* Pin / read the TQEC_MODE_SPEC_F32 configuration: tile, threads,
+ * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB flags, and `waves` = minimum
+ * waves per SIMD requested from the compiler (bits 0-7, 0 = none) | unroll
+ * of the per-lane output loop << 8 (1, 2, 4 or 8; 0 = 1).  No reference
+ * counterpart. */
+int tqec_plan_set_spec(tqec_plan *plan, int32_t tile, int32_t threads, int32_t flags, int32_t waves);
+int tqec_plan_spec_config(tqec_plan *plan, int32_t *tile, int32_t *threads, int32_t *flags, int32_t *waves);
 
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);

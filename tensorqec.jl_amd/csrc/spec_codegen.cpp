@@ -7,6 +7,7 @@
 #include "spec_codegen.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <sstream>
 
 namespace tqec {
@@ -27,7 +28,7 @@ Layout layout(const CompiledPlan &cp, const SpecCfg &cfg) {
     L.tab_entries = cp.const_pool.size() + cp.sliced_pool.size();
     size_t off = 0;
     L.work = off;
-    off += round_up((size_t)std::max<int64_t>(1, cp.ws_entries) * G * elem, 16);
+    if (!cfg.slab) off += round_up((size_t)std::max<int64_t>(1, cp.ws_entries) * G * elem, 16);
     L.tab = off;
     if (cfg.tab_lds) off += round_up(std::max<size_t>(1, L.tab_entries) * elem, 16);
     L.mx = off;
@@ -113,7 +114,71 @@ std::string load_expr(const OperandDesc &op, const char *base, const char *idx, 
     return tmp;
 }
 
-void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64) {
+// The output loop of a step.  Each lane handles outputs c0, c0 + CSTEP, ...;
+// U of them are computed together (all loads, then the fma chains, then the
+// stores), so their LDS reads overlap.  Every output keeps the generic
+// kernel's order of operations.
+void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int unroll, bool fp64) {
+    const OperandDesc &A = st.a, &B = st.b;
+    const long long nc = 1ll << st.rc;
+    const std::string ga = gather_expr(A), gb = gather_expr(B);
+    const uint32_t nkk = 1u << st.nk;
+    const int cexp = A.cexp + B.cexp;
+    int U = 1;
+    if (nc > (long long)cstep) {
+        const long long trips = nc / cstep;   // both powers of two
+        U = (int)std::min<long long>(std::max(1, unroll), trips);
+    }
+    std::ostringstream b;
+    for (int u = 0; u < U; ++u) {
+        const std::string cu = "(c + " + std::to_string((long long)u * cstep) + "u)";
+        std::string ea = ga, eb = gb;
+        // substitute c -> c_u in the gather expressions
+        auto subst = [&](const std::string &e) {
+            std::string r;
+            for (size_t i = 0; i < e.size(); ++i) {
+                if (e[i] == 'c' && (i == 0 || !isalnum((unsigned char)e[i - 1])) &&
+                    (i + 1 == e.size() || !isalnum((unsigned char)e[i + 1])))
+                    r += cu;
+                else
+                    r += e[i];
+            }
+            return r;
+        };
+        b << "      const unsigned ar" << u << " = " << subst(ea) << ";\n";
+        b << "      const unsigned br" << u << " = " << subst(eb) << ";\n";
+        b << "      T acc" << u << " = 0;\n";
+    }
+    for (uint32_t k = 0; k < nkk; ++k)
+        for (int u = 0; u < U; ++u) {
+            std::ostringstream pre;
+            const std::string sfx = std::to_string(k) + "_" + std::to_string(u);
+            const std::string arn = "ar" + std::to_string(u), brn = "br" + std::to_string(u);
+            const std::string va = load_expr(A, "pa", arn.c_str(), k << A.kshift, pre, "sa" + sfx);
+            const std::string vb = load_expr(B, "pb", brn.c_str(), k << B.kshift, pre, "sb" + sfx);
+            b << pre.str();
+            b << "      acc" << u << " = " << (fp64 ? "fma(" : "fmaf(") << va << ", " << vb << ", acc" << u << ");\n";
+        }
+    for (int u = 0; u < U; ++u) {
+        const std::string acc = "acc" + std::to_string(u);
+        if (fp64) {
+            if (cexp != 0) b << "      " << acc << " = ldexp(" << acc << ", " << cexp << ");\n";
+        } else {
+            b << "      " << acc << " = ldexpf(" << acc << ", shift);\n";
+            b << "      lmax = " << acc << " > lmax ? " << acc << " : lmax;\n";
+        }
+        b << "      pc[(c + " << (long long)u * cstep << "u) << GLOG] = " << acc << ";\n";
+    }
+    if (nc <= (long long)cstep) {
+        o << "    if (c0 < " << nc << ") {\n      const unsigned c = (unsigned)c0;\n" << b.str() << "    }\n";
+    } else {
+        o << "    #pragma unroll 1\n";
+        o << "    for (unsigned c = (unsigned)c0; c < " << nc << "u; c += " << (long long)U * cstep << "u) {\n"
+          << b.str() << "    }\n";
+    }
+}
+
+void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64, int unroll) {
     const OperandDesc &A = st.a, &B = st.b;
     o << "  { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << "\n";
     if (fp64) {
@@ -121,29 +186,7 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
         emit_base(o, "pa", A, const_n);
         emit_base(o, "pb", B, const_n);
         o << "    T *pc = work + " << (long long)st.c_off << " * G + g;\n";
-        const long long nc = 1ll << st.rc;
-        const int cexp = A.cexp + B.cexp;
-        const std::string ga = gather_expr(A), gb = gather_expr(B);
-        std::ostringstream b;
-        b << "      const unsigned ar = " << ga << ";\n";
-        b << "      const unsigned br = " << gb << ";\n";
-        b << "      T acc = 0;\n";
-        const uint32_t nkk = 1u << st.nk;
-        for (uint32_t k = 0; k < nkk; ++k) {
-            std::ostringstream pre;
-            const std::string va = load_expr(A, "pa", "ar", k << A.kshift, pre, "sa" + std::to_string(k));
-            const std::string vb = load_expr(B, "pb", "br", k << B.kshift, pre, "sb" + std::to_string(k));
-            b << pre.str();
-            b << "      acc = fma(" << va << ", " << vb << ", acc);\n";
-        }
-        if (cexp != 0) b << "      acc = ldexp(acc, " << cexp << ");\n";
-        b << "      pc[c << GLOG] = acc;\n";
-        if (nc <= (long long)cstep) {
-            o << "    if (c0 < " << nc << ") {\n      const unsigned c = (unsigned)c0;\n" << b.str() << "    }\n";
-        } else {
-            o << "    #pragma unroll 1\n";
-            o << "    for (unsigned c = (unsigned)c0; c < " << nc << "u; c += CSTEP) {\n" << b.str() << "    }\n";
-        }
+        emit_outputs(o, st, cstep, unroll, true);
         o << "  }\n";
         o << "  __syncthreads();\n";
         return;
@@ -172,34 +215,7 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
     emit_base(o, "pb", B, const_n);
     o << "    T *pc = work + " << (long long)st.c_off << " * G + g;\n";
     o << "    T lmax = 0;\n";
-    const long long nc = 1ll << st.rc;
-    const std::string ga = gather_expr(A), gb = gather_expr(B);
-    auto body = [&](const char *indent) {
-        std::ostringstream b;
-        b << indent << "  const unsigned ar = " << ga << ";\n";
-        b << indent << "  const unsigned br = " << gb << ";\n";
-        b << indent << "  T acc = 0;\n";
-        const uint32_t nkk = 1u << st.nk;
-        for (uint32_t k = 0; k < nkk; ++k) {
-            std::ostringstream pre;
-            const std::string va = load_expr(A, "pa", "ar", k << A.kshift, pre, "sa" + std::to_string(k));
-            const std::string vb = load_expr(B, "pb", "br", k << B.kshift, pre, "sb" + std::to_string(k));
-            std::string p = pre.str();
-            // re-indent the pre-statements
-            b << p;
-            b << indent << "  acc = fmaf(" << va << ", " << vb << ", acc);\n";
-        }
-        b << indent << "  acc = ldexpf(acc, shift);\n";
-        b << indent << "  lmax = acc > lmax ? acc : lmax;\n";
-        b << indent << "  pc[c << GLOG] = acc;\n";
-        return b.str();
-    };
-    if (nc <= (long long)cstep) {
-        o << "    if (c0 < " << nc << ") {\n      const unsigned c = (unsigned)c0;\n" << body("    ") << "    }\n";
-    } else {
-        o << "    #pragma unroll 1\n";
-        o << "    for (unsigned c = (unsigned)c0; c < " << nc << "u; c += CSTEP) {\n" << body("    ") << "    }\n";
-    }
+    emit_outputs(o, st, cstep, unroll, false);
     o << "    for (int w = G; w < 64; w <<= 1) { const float other = __shfl_xor(lmax, w); lmax = other > lmax ? other : lmax; }\n";
     o << "    if (lane < G) atomicMax(&mx[" << st.c_slot << " * G + g], ((unsigned long long)(tag_base + " << s
       << "u) << 32) | (unsigned long long)__float_as_uint(lmax));\n";
@@ -211,6 +227,10 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
 }  // namespace
 
 size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg) { return layout(cp, cfg).total; }
+
+size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg) {
+    return cfg.slab ? (size_t)std::max<int64_t>(1, cp.ws_entries) << cfg.glog : 0;
+}
 
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     const Layout L = layout(cp, cfg);
@@ -224,16 +244,19 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
       << "\n#define NOUT " << (1 << cp.n_out) << "\n#define NSLOTS " << std::max(1, cp.n_slots) << "\n#define NSTEPS "
       << nsteps << "\n#define TABN " << L.tab_entries << "\n";
     if (cfg.waves > 0)
-        o << "extern \"C\" __global__ void __launch_bounds__(NT, " << cfg.waves << ") tqec_spec(\n";
+        o << "extern \"C\" __global__ void __launch_bounds__(NT, " << cfg.waves << ") " << spec_kernel_name(cfg) << "(\n";
     else
-        o << "extern \"C\" __global__ void __launch_bounds__(NT) tqec_spec(\n";
+        o << "extern \"C\" __global__ void __launch_bounds__(NT) " << spec_kernel_name(cfg) << "(\n";
     o << 
          "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
          "    const int *__restrict__ list_count, const T *__restrict__ gtab, double *__restrict__ mar,\n"
          "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
-         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol) {\n";
+         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, T *__restrict__ gslab) {\n";
     o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
-    o << "  T *work = reinterpret_cast<T *>(smem + " << L.work << ");\n";
+    if (cfg.slab)
+        o << "  T *work = gslab + (unsigned long long)blockIdx.x * " << spec_slab_entries(cp, cfg) << "ull;\n";
+    else
+        o << "  T *work = reinterpret_cast<T *>(smem + " << L.work << ");\n";
     if (cfg.tab_lds)
         o << "  T *tab = reinterpret_cast<T *>(smem + " << L.tab << ");\n";
     else
@@ -271,7 +294,8 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
         o << "  for (int i = tid; i < NSLOTS * G; i += NT) mx[i] = 0ull;\n"
              "  if (tid < G) lowf[tid] = 0;\n";
     o << "  __syncthreads();\n";
-    for (int s = 0; s < nsteps; ++s) emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64);
+    for (int s = 0; s < nsteps; ++s)
+        emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64, std::max(1, cfg.unroll));
     o << "  if (tid < G) {\n"
          "    const long long q = b0 + tid;\n"
          "    if (q < count) {\n"

@@ -26,18 +26,26 @@ struct SpecCfg {
     bool tab_lds = true; // constant and sliced tables staged in LDS
     int waves = 0;       // >0: ask the compiler for at least this many waves per SIMD (caps VGPRs)
     bool fp64 = false;   // FP64 without scaling (the generic FP64 fused kernel's arithmetic)
+    bool slab = false;   // intermediates in a per-workgroup slab of global memory instead of LDS
+    int unroll = 1;      // output entries per lane in flight (unroll of the output loop)
 };
 
 // Static LDS bytes of the specialised kernel for this configuration.
 size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg);
 
-// HIP source of the kernel `tqec_spec` (extern "C").  Parameters:
+// Entries of one workgroup's slab (slab configurations).
+size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg);
+
+// HIP source of the kernel `tqec_spec_f32` / `tqec_spec_f64` (extern "C",
+// see spec_kernel_name).  Parameters:
 //   (const uint8_t *syn, int64_t n, const int32_t *list, const int32_t *list_count,
 //    const T *tab, double *mar, int32_t *cls, uint8_t *tie,
-//    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol)
+//    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, T *slab)
+// `slab` holds spec_slab_entries() entries per workgroup (slab configurations only).
 // `tab` is the constant pool followed by the sliced pool, in the kernel's
 // precision (float, or double when cfg.fp64); the FP64 variant ignores the
 // re-check arguments.
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg);
+inline const char *spec_kernel_name(const SpecCfg &cfg) { return cfg.fp64 ? "tqec_spec_f64" : "tqec_spec_f32"; }
 
 }  // namespace tqec

@@ -820,6 +820,7 @@ struct Workspace {
     double *g_mar = nullptr; size_t g_mar_bytes = 0;
     int32_t *g_cls = nullptr; size_t g_cls_bytes = 0;
     uint8_t *g_tie = nullptr; size_t g_tie_bytes = 0;
+    void *spec_slab = nullptr; size_t spec_slab_bytes = 0;   // slab configurations of the specialised kernels
     // Per-call HIP events (main start/stop, re-check start/stop), accumulated
     // while timing is enabled and consumed by tqec_kernel_time_totals.
     std::vector<std::array<hipEvent_t, 4>> ev;
@@ -860,7 +861,7 @@ struct tqec_plan {
     int n_cu = 0;
     SpecCfg spec_cfg;
     bool spec_cfg_set = false;
-    std::map<std::tuple<int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64)
+    std::map<std::tuple<int, int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64, slab)
     SpecCfg spec64_cfg;                 // TQEC_MODE_SPEC_F64
     bool spec64_cfg_set = false;
 };
@@ -1286,13 +1287,25 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
             }
         }
     }
+    // intermediates too wide for LDS: per-workgroup slabs of at most 4 MiB
+    for (int gl = 4; gl >= 0; --gl) {
+        SpecCfg c;
+        c.glog = gl;
+        c.threads = 256;
+        c.tab_lds = false;
+        c.fp64 = fp64;
+        c.slab = true;
+        const size_t bytes = spec_slab_entries(p->cp, c) * (fp64 ? 8 : 4);
+        if ((bytes <= (size_t(4) << 20) || gl == 0) && spec_lds_bytes(p->cp, c) <= p->lds_limit) return c;
+    }
     SpecCfg c;
     c.glog = -1;
     return c;
 }
 
 int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
-    const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0);
+    const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
+                                     (cfg.slab ? 1 : 0) | (cfg.unroll << 1));
     auto it = p->spec.find(key);
     if (it != p->spec.end()) {
         *out = &it->second;
@@ -1306,7 +1319,7 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     k.cfg = cfg;
     k.lds = spec_lds_bytes(p->cp, cfg);
     TQ_HIP(hipModuleLoadData(&k.mod, code.data()));
-    const hipError_t e = hipModuleGetFunction(&k.fn, k.mod, "tqec_spec");
+    const hipError_t e = hipModuleGetFunction(&k.fn, k.mod, spec_kernel_name(cfg));
     if (e != hipSuccess) {
         (void)hipModuleUnload(k.mod);
         return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
@@ -1317,12 +1330,26 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     return TQEC_OK;
 }
 
+// Slab bytes a specialised kernel needs for a batch of n.
+size_t spec_slab_need(const tqec_plan *p, const SpecKernel *k, int64_t n) {
+    if (!k->cfg.slab) return 0;
+    const int64_t ntiles = (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
+    return spec_slab_entries(p->cp, k->cfg) * (k->cfg.fp64 ? 8 : 4) * (size_t)grid;
+}
+
 // Launch a specialised kernel over n syndromes (or the listed ones).
-int spec_launch(SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n, const int32_t *in_list,
-                const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie, int32_t *rl, int32_t *rcount,
-                int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream) {
+int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n,
+                const int32_t *in_list, const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie,
+                int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream) {
     const int64_t ntiles = (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
+    void *a_slab = nullptr;
+    if (k->cfg.slab) {
+        // the caller sized the slab before enqueueing anything that uses it
+        if (spec_slab_need(p, k, n) > w.spec_slab_bytes) return set_error(TQEC_EINVAL, "internal: slab too small");
+        a_slab = w.spec_slab;
+    }
     const unsigned char *a_syn = d_syn;
     long long a_n = (long long)n;
     const int *a_list = in_list, *a_count = in_count;
@@ -1333,25 +1360,33 @@ int spec_launch(SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n, 
     int *a_rl = rl, *a_rc = rcount;
     long long a_cap = (long long)rcap;
     double a_tt = tie_tol, a_rt = recheck_tol;
-    void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt};
+    void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt,
+                    &a_slab};
     TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
     return TQEC_OK;
 }
 
-// FP64 re-check with the specialised FP64 kernel, one syndrome per 64-thread
-// workgroup: the list is short, so latency, not throughput, matters.
-int spec_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
-                 int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
+// FP64 re-check kernel: one syndrome per 64-thread workgroup (the list is
+// short, so latency, not throughput, matters), in a slab when too wide for LDS.
+int spec_recheck_kernel(tqec_plan *p, int dev, SpecKernel **out) {
     SpecCfg c;
     c.glog = 0;
     c.threads = 64;
     c.tab_lds = false;
     c.fp64 = true;
-    SpecKernel *k;
+    if (!spec_fits(p, c)) {
+        // too wide for LDS: the re-check works in a global slab, 256 threads
+        c.slab = true;
+        c.threads = 256;
+    }
+    return spec_get(p, dev, c, out);
+}
+
+int spec_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, SpecKernel *k, const uint8_t *d_syn, int64_t n,
+                 double *d_mar, int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
     int rc;
-    if ((rc = spec_get(p, w.device, c, &k))) return rc;
     if ((rc = record(p, w, 2, stream))) return rc;
-    if ((rc = spec_launch(k, dd, d_syn, n, w.list, w.count, d_mar, d_cls, d_tie, nullptr, nullptr, 0, p->tie_tol,
+    if ((rc = spec_launch(p, w, k, dd, d_syn, n, w.list, w.count, d_mar, d_cls, d_tie, nullptr, nullptr, 0, p->tie_tol,
                           p->recheck_tol, stream)))
         return rc;
     if ((rc = record(p, w, 3, stream))) return rc;
@@ -1371,9 +1406,10 @@ int run_spec64(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn,
     SpecKernel *k;
     int rc;
     if ((rc = spec_get(p, w.device, p->spec64_cfg, &k))) return rc;
+    if ((rc = grow((char **)&w.spec_slab, &w.spec_slab_bytes, spec_slab_need(p, k, n)))) return rc;
     if ((rc = record(p, w, 0, stream))) return rc;
-    if ((rc = spec_launch(k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, nullptr, nullptr, 0, p->tie_tol,
-                          p->recheck_tol, stream)))
+    if ((rc = spec_launch(p, w, k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, nullptr, nullptr, 0,
+                          p->tie_tol, p->recheck_tol, stream)))
         return rc;
     return record(p, w, 1, stream);
 }
@@ -1381,24 +1417,26 @@ int run_spec64(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn,
 int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream, const int32_t *in_list = nullptr,
              const int32_t *in_count = nullptr) {
-    if (p->f64.glog < 0)
-        return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP64)");
     if (!p->spec_cfg_set) {
         p->spec_cfg = choose_spec(p);
         p->spec_cfg_set = true;
     }
     if (p->spec_cfg.glog < 0)
         return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP32)");
-    SpecKernel *k;
+    SpecKernel *k, *k64;
     int rc;
     if ((rc = spec_get(p, w.device, p->spec_cfg, &k))) return rc;
+    if ((rc = spec_recheck_kernel(p, w.device, &k64))) return rc;
+    if ((rc = grow((char **)&w.spec_slab, &w.spec_slab_bytes,
+                   std::max(spec_slab_need(p, k, n), spec_slab_need(p, k64, n)))))
+        return rc;
     if ((rc = prepare_recheck(w, n, stream))) return rc;
     if ((rc = record(p, w, 0, stream))) return rc;
-    if ((rc = spec_launch(k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, w.list, w.count, w.list_cap,
+    if ((rc = spec_launch(p, w, k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, w.list, w.count, w.list_cap,
                           p->tie_tol, p->recheck_tol, stream)))
         return rc;
     if ((rc = record(p, w, 1, stream))) return rc;
-    return spec_recheck(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+    return spec_recheck(p, dd, w, k64, d_syn, n, d_mar, d_cls, d_tie, stream);
 }
 
 // Compile a set of specialised configurations on host threads (hiprtc
@@ -1434,7 +1472,6 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
     if ((rc = ensure_device(p, dev, &dd))) return rc;
     Workspace &w = p->ws[std::make_pair(dev, st)];
     w.device = dev;
-    if (p->f64.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the fused executor");
     const int nout = 1 << p->cp.n_out;
     if ((rc = grow(&w.d_mar, &w.d_mar_bytes, (size_t)n * nout * sizeof(double)))) return rc;
     if ((rc = grow(&w.d_cls, &w.d_cls_bytes, (size_t)n * sizeof(int32_t)))) return rc;
@@ -1480,18 +1517,34 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
             c.tab_lds = false;
             if (spec_fits(p, c)) stage1.push_back(c);
         }
-    if (stage1.empty()) rc = set_error(TQEC_EUNSUPPORTED, "no specialised tile fits in LDS");
+    if (stage1.empty()) {
+        // slab configurations (per-workgroup intermediates of at most 8 MiB)
+        for (int gl = 0; gl <= 4; ++gl)
+            for (int thr : {128, 256}) {
+                SpecCfg c;
+                c.glog = gl;
+                c.threads = thr;
+                c.tab_lds = false;
+                c.slab = true;
+                if (spec_fits(p, c) && (gl == 0 || spec_slab_entries(p->cp, c) * 4 <= (size_t(8) << 20)))
+                    stage1.push_back(c);
+            }
+    }
+    if (stage1.empty()) rc = set_error(TQEC_EUNSUPPORTED, "no specialised tile fits");
     if (!rc) time_all(stage1);
     if (!rc && best.glog >= 0) {
         std::vector<SpecCfg> stage2;
         for (bool tab : {false, true})
-            for (int wv : {0, 4}) {
-                if (!tab && wv == 0) continue;
-                SpecCfg c = best;
-                c.tab_lds = tab;
-                c.waves = wv;
-                if (spec_fits(p, c)) stage2.push_back(c);
-            }
+            for (int wv : {0, 4})
+                for (int un : {1, 2, 4}) {
+                    if (!tab && wv == 0 && un == 1) continue;
+                    if (tab && best.slab) continue;
+                    SpecCfg c = best;
+                    c.tab_lds = tab;
+                    c.waves = wv;
+                    c.unroll = un;
+                    if (spec_fits(p, c)) stage2.push_back(c);
+                }
         time_all(stage2);
     }
     p->timing = saved_timing;
@@ -1670,7 +1723,7 @@ void tqec_plan_destroy(tqec_plan *p) {
         if (w.d_mar) (void)hipFree(w.d_mar);
         if (w.d_cls) (void)hipFree(w.d_cls);
         if (w.d_tie) (void)hipFree(w.d_tie);
-        for (void *q : {w.smx, w.sex, (void *)w.g_syn, (void *)w.g_mar, (void *)w.g_cls, (void *)w.g_tie})
+        for (void *q : {w.smx, w.sex, (void *)w.g_syn, (void *)w.g_mar, (void *)w.g_cls, (void *)w.g_tie, w.spec_slab})
             if (q) (void)hipFree(q);
         for (auto &a : w.ev)
             for (hipEvent_t e : a) (void)hipEventDestroy(e);
@@ -2311,6 +2364,8 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     c.threads = threads;
     c.tab_lds = (flags & 1) != 0;
     c.fp64 = (flags & 2) != 0;
+    c.slab = (flags & 4) != 0;
+    c.unroll = std::max(1, (flags >> 8) & 0xff);
     std::string src;
     try {
         src = spec_source(p->cp, c);
@@ -2327,29 +2382,35 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
 }
 
 
-int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t tables_in_lds, int32_t waves) {
+int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t flags, int32_t waves) {
     if (!p) return set_error(TQEC_ENULL, "null plan");
     int gl = 0;
     while (gl <= 6 && (1 << gl) != tile) ++gl;
     if (gl > 6) return set_error(TQEC_EINVAL, "tile must be a power of two in [1, 64]");
     if ((threads != 64 && threads != 128 && threads != 256) || threads < tile)
         return set_error(TQEC_EINVAL, "threads must be 64, 128 or 256 and at least the tile");
-    if (waves < 0 || waves > 8) return set_error(TQEC_EINVAL, "waves must be in [0, 8]");
+    const int unroll = (waves >> 8) & 0xff;
+    waves &= 0xff;
+    if (waves > 8) return set_error(TQEC_EINVAL, "waves must be in [0, 8]");
+    if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8)
+        return set_error(TQEC_EINVAL, "unroll must be 1, 2, 4 or 8");
     std::lock_guard<std::mutex> lk(p->mu);
     int rc;
     if ((rc = configure(p))) return rc;
     SpecCfg c;
     c.glog = gl;
     c.threads = threads;
-    c.tab_lds = tables_in_lds != 0;
+    c.tab_lds = (flags & 1) != 0;
+    c.slab = (flags & 4) != 0;
     c.waves = waves;
+    c.unroll = unroll ? unroll : 1;
     if (!spec_fits(p, c)) return set_error(TQEC_EUNSUPPORTED, "tile does not fit in LDS");
     p->spec_cfg = c;
     p->spec_cfg_set = true;
     return TQEC_OK;
 }
 
-int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t *tables_in_lds, int32_t *waves) {
+int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t *flags, int32_t *waves) {
     if (!p) return set_error(TQEC_ENULL, "null plan");
     std::lock_guard<std::mutex> lk(p->mu);
     int rc;
@@ -2361,8 +2422,8 @@ int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t
     if (p->spec_cfg.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the specialised kernel");
     if (tile) *tile = 1 << p->spec_cfg.glog;
     if (threads) *threads = p->spec_cfg.threads;
-    if (tables_in_lds) *tables_in_lds = p->spec_cfg.tab_lds ? 1 : 0;
-    if (waves) *waves = p->spec_cfg.waves;
+    if (flags) *flags = (p->spec_cfg.tab_lds ? 1 : 0) | (p->spec_cfg.slab ? 4 : 0);
+    if (waves) *waves = p->spec_cfg.waves | (p->spec_cfg.unroll << 8);
     return TQEC_OK;
 }
 

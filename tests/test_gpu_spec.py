@@ -61,11 +61,11 @@ def test_spec_bit_identical_to_generic(name, code, p, n):
         got64 = run(torch, ct, dS, n, "spec_f64")
         for a, b in zip(got64, ref64):
             assert np.array_equal(a, b), (name, "f64")
-    configs = [None, (1, 64), (4, 128), (16, 256)]
+    configs = [None, (1, 64, 1), (4, 128, 1), (16, 256, 1), (4, 64, 2), (2, 64, 4), (8, 256, 4)]
     for cfg in configs:
         if cfg is not None:
             try:
-                ct.hip.set_tile("spec_f32", *cfg)
+                ct.hip.set_spec(cfg[0], cfg[1], False, 0, False, cfg[2])
             except L.TqecError as e:
                 assert e.code == L.TQEC_EUNSUPPORTED
                 continue
@@ -108,3 +108,61 @@ def test_spec_list_decode_and_autotune():
     untouched[rows] = False
     assert np.all(c[untouched] == -1)
     assert np.array_equal(mar.cpu().numpy()[rows], ref[0][rows])
+
+
+def test_spec_slab_mode_bit_identical():
+    """Intermediates in a per-workgroup global slab instead of LDS: same
+    arithmetic, so still bit-identical to the generic fused kernel."""
+    torch = pytest.importorskip("torch")
+    for code, n in ((SurfaceCode(5, 5), 3000), (SurfaceCode(7, 7), 5000)):
+        t = CSSTannerGraph.from_code(code)
+        p = 0.05
+        em = iid_error(p / 3, p / 3, p / 3, t.nq)
+        ct = D.compile(D.TNMMAP(mode="fused_f32"), t, em)
+        S = css_syndrome_rows(sample_depolarizing(em, seed=31 + t.nq, count=n), t)
+        dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+        ref = run(torch, ct, dS, n, "fused_f32")
+        for tile, thr in ((1, 256), (4, 128), (16, 256)):
+            say(f"case spec slab {t.nq} tile {tile} threads {thr}")
+            ct.hip.set_spec(tile, thr, False, 0, slab=True)
+            assert ct.hip.spec_config()["slab"]
+            got = run(torch, ct, dS, n, "spec_f32")
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["toric4", "dem_d3_r3"])
+def test_spec_wide_plans_vs_plain_f64(case):
+    """Plans too wide for LDS (toric L=4 code capacity, circuit-level d=3
+    memory experiment over 3 rounds) run the specialised kernel from slabs;
+    marginals within 2e-5 of the row maximum of plain FP64 and classes equal
+    on every non-tie syndrome."""
+    torch = pytest.importorskip("torch")
+    from oracle import tn_oracle as O
+    dev = torch.device("cuda:0")
+    if case == "toric4":
+        t = CSSTannerGraph.from_code(ToricCode(4, 4))
+        em = iid_error(0.05 / 3, 0.05 / 3, 0.05 / 3, t.nq)
+        ct = D.compile(D.TNMMAP(mode="spec_f32", optimizer=D.TreeSA(seed=1, ntrials=8, sweeps=1000, restarts=8)), t, em)
+        S = css_syndrome_rows(sample_depolarizing(em, seed=5, count=96), t)
+    else:
+        from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+        dem = detector_error_model_from_stim(rotated_memory_z(3, 3, 0.001))
+        ct = D.compile(D.TNMMAP(mode="spec_f32"), dem)
+        smp = L.HipSampler.flips(dem.error_rates, dem.detector_matrix(), dem.logical_matrix())
+        dS0 = torch.empty((2000, smp.nsyn), dtype=torch.uint8, device=dev)
+        smp.sample(17, 0, 2000, dS0.data_ptr())
+        S = dS0.cpu().numpy()
+    n = len(S)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to(dev)
+    say(f"case spec wide {case}: config {ct.hip.spec_config()}")
+    ref = run(torch, ct, dS, n, "plain_f64")
+    got = run(torch, ct, dS, n, "spec_f32")
+    m, r = got[0], ref[0]
+    valid = r.max(axis=1) > 0
+    assert np.all(m[~valid] == 0.0)
+    err = np.max(np.abs(m[valid] - r[valid]) / r[valid].max(axis=1, keepdims=True))
+    say(f"  max rel marginal err {err:.2e}")
+    assert err < 2e-5
+    ok = valid & (O.tie_flags(r, 1e-12) == 0)
+    assert np.array_equal(got[1][ok], O.findmax(r)[ok])

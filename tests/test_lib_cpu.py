@@ -151,10 +151,12 @@ def test_stats_mirror_plan():
     assert s["plain_ws_entries"] > 0
 
 
-@pytest.mark.parametrize("tile,threads,tab,fp64", [(1, 64, True, False), (4, 128, False, False),
-                                                   (16, 256, True, False), (1, 64, False, True),
-                                                   (8, 128, True, True)])
-def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, tab, fp64):
+@pytest.mark.parametrize("tile,threads,tab,fp64,slab,unroll", [(1, 64, True, False, False, 1),
+                                                               (4, 128, False, False, False, 4),
+                                                               (16, 256, True, False, True, 2),
+                                                               (1, 64, False, True, False, 2),
+                                                               (8, 128, True, True, True, 1)])
+def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, tab, fp64, slab, unroll):
     """The plan-specialised FP32 kernel (TQEC_MODE_SPEC_F32) is generated on
     the host and compiled for gfx950 at first use; check here, without a
     GPU, that the generated source compiles and uses no scratch memory."""
@@ -166,8 +168,8 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
     t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
     p = 0.05 / 3
     ct = D.compile(D.TNMMAP(mode="fused_f32"), t, iid_error(p, p, p, t.nq))
-    src = ct.hip.spec_source(tile, threads, tab, fp64)
-    assert "tqec_spec" in src and f"#define G {tile}\n" in src
+    src = ct.hip.spec_source(tile, threads, tab, fp64, slab, unroll)
+    assert ("tqec_spec_f64" if fp64 else "tqec_spec_f32") in src and f"#define G {tile}\n" in src
     assert ("typedef double T;" in src) == fp64
     f = tmp_path / "spec.hip"
     f.write_text("#include <hip/hip_runtime.h>\n" + src)
