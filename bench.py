@@ -344,7 +344,7 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--d", type=int, default=7)

@@ -27,6 +27,9 @@ const libtqec = joinpath(@__DIR__, "..", "tensorqec", "libtqec_hip.so")
 const MODE_PLAIN_F64 = Int32(0)
 const MODE_FUSED_F64 = Int32(1)
 const MODE_FUSED_F32 = Int32(2)
+const MODE_WIDE_F32 = Int32(3)
+const MODE_SPEC_F32 = Int32(4)     # plan-specialised kernel (hiprtc), bit-identical to MODE_FUSED_F32
+const MODE_SPEC_F64 = Int32(5)
 
 # Mirrors `tqec_plan_desc` in include/tqec_hip.h field by field.
 struct PlanDesc
@@ -116,13 +119,13 @@ function HipPlan(code, tensors, n_syndrome::Int; tabulate_bits::Int = 4)
 end
 
 """
-    TNMMAPHIP(; optimizer=TreeSA(), mode=MODE_FUSED_F32)
+    TNMMAPHIP(; optimizer=TreeSA(), mode=MODE_SPEC_F32)
 
 `TNMMAP` (src/decoding/tndecoder.jl:77-80) executed by the HIP library.
 """
 Base.@kwdef struct TNMMAPHIP <: TensorQEC.AbstractGeneralDecoder
     optimizer::OMEinsum.CodeOptimizer = TreeSA()
-    mode::Int32 = MODE_FUSED_F32
+    mode::Int32 = MODE_SPEC_F32
 end
 
 struct CompiledTNMMAPHIP <: TensorQEC.CompiledDecoder
