@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cctype>
 #include <sstream>
+#include <vector>
 
 namespace tqec {
 
@@ -114,68 +115,147 @@ std::string load_expr(const OperandDesc &op, const char *base, const char *idx, 
     return tmp;
 }
 
-// The output loop of a step.  Each lane handles outputs c0, c0 + CSTEP, ...;
-// U of them are computed together (all loads, then the fma chains, then the
-// stores), so their LDS reads overlap.  Every output keeps the generic
-// kernel's order of operations.
-void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int unroll, bool fp64) {
+// Output bits of a step that feed an operand's index (the segments' sources).
+uint32_t source_mask(const OperandDesc &op) {
+    uint32_t m = 0;
+    for (int i = 0; i < op.nseg; ++i) {
+        const Seg &g = op.seg[i];
+        if (g.width <= 0) continue;
+        const uint32_t w = (g.width >= 32) ? 0xffffffffu : ((1u << g.width) - 1u);
+        m |= w << g.src;
+    }
+    return m;
+}
+
+// `expr` with every whole-word `c` replaced by `by`.
+std::string subst_c(const std::string &expr, const std::string &by) {
+    std::string r;
+    for (size_t i = 0; i < expr.size(); ++i) {
+        if (expr[i] == 'c' && (i == 0 || !isalnum((unsigned char)expr[i - 1])) &&
+            (i + 1 == expr.size() || !isalnum((unsigned char)expr[i + 1])))
+            r += "(" + by + ")";
+        else
+            r += expr[i];
+    }
+    return r;
+}
+
+// The output loop of a step.  Lane c0 (< CSTEP) owns the outputs whose low
+// bits are c0.  Up to `block` of them are computed together: the block
+// varies output bits that feed only A or only B, so an A row (or B row)
+// loaded once serves several outputs (register blocking of the small
+// per-syndrome contraction); all loads come before the fma chains and the
+// stores after them.  Every output keeps the generic kernel's order of
+// operations (k ascending, single-side sums first), so results are
+// bit-identical.
+void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int block, bool fp64) {
     const OperandDesc &A = st.a, &B = st.b;
-    const long long nc = 1ll << st.rc;
+    const int rc = st.rc;
+    const long long nc = 1ll << rc;
     const std::string ga = gather_expr(A), gb = gather_expr(B);
     const uint32_t nkk = 1u << st.nk;
     const int cexp = A.cexp + B.cexp;
-    int U = 1;
+    int lc = 0;
+    while ((1 << lc) < cstep) ++lc;
+    // block bits S among the high bits [lc, rc), the loop runs over the rest R
+    std::vector<int> S, R;
     if (nc > (long long)cstep) {
-        const long long trips = nc / cstep;   // both powers of two
-        U = (int)std::min<long long>(std::max(1, unroll), trips);
-    }
-    std::ostringstream b;
-    for (int u = 0; u < U; ++u) {
-        const std::string cu = "(c + " + std::to_string((long long)u * cstep) + "u)";
-        std::string ea = ga, eb = gb;
-        // substitute c -> c_u in the gather expressions
-        auto subst = [&](const std::string &e) {
-            std::string r;
-            for (size_t i = 0; i < e.size(); ++i) {
-                if (e[i] == 'c' && (i == 0 || !isalnum((unsigned char)e[i - 1])) &&
-                    (i + 1 == e.size() || !isalnum((unsigned char)e[i + 1])))
-                    r += cu;
-                else
-                    r += e[i];
-            }
-            return r;
-        };
-        b << "      const unsigned ar" << u << " = " << subst(ea) << ";\n";
-        b << "      const unsigned br" << u << " = " << subst(eb) << ";\n";
-        b << "      T acc" << u << " = 0;\n";
-    }
-    for (uint32_t k = 0; k < nkk; ++k)
-        for (int u = 0; u < U; ++u) {
-            std::ostringstream pre;
-            const std::string sfx = std::to_string(k) + "_" + std::to_string(u);
-            const std::string arn = "ar" + std::to_string(u), brn = "br" + std::to_string(u);
-            const std::string va = load_expr(A, "pa", arn.c_str(), k << A.kshift, pre, "sa" + sfx);
-            const std::string vb = load_expr(B, "pb", brn.c_str(), k << B.kshift, pre, "sb" + sfx);
-            b << pre.str();
-            b << "      acc" << u << " = " << (fp64 ? "fma(" : "fmaf(") << va << ", " << vb << ", acc" << u << ");\n";
+        const uint32_t am = source_mask(A), bm = source_mask(B);
+        std::vector<int> aonly, bonly, other;
+        for (int bit = lc; bit < rc; ++bit) {
+            const bool ia = (am >> bit) & 1u, ib = (bm >> bit) & 1u;
+            if (ia && !ib) aonly.push_back(bit);
+            else if (ib && !ia) bonly.push_back(bit);
+            else other.push_back(bit);
         }
-    for (int u = 0; u < U; ++u) {
-        const std::string acc = "acc" + std::to_string(u);
+        int want = 0;
+        while ((2 << want) <= std::max(1, block)) ++want;
+        size_t ia = 0, ib = 0, io = 0;
+        while ((int)S.size() < want && (ia < aonly.size() || ib < bonly.size() || io < other.size())) {
+            if (ia < aonly.size() && (ia <= ib || ib >= bonly.size())) S.push_back(aonly[ia++]);
+            else if (ib < bonly.size()) S.push_back(bonly[ib++]);
+            else S.push_back(other[io++]);
+        }
+        for (int bit = lc; bit < rc; ++bit)
+            if (std::find(S.begin(), S.end(), bit) == S.end()) R.push_back(bit);
+    }
+    const uint32_t am = source_mask(A), bm = source_mask(B);
+    const int ns = (int)S.size();
+    // subsets of S that matter to A / to B
+    std::vector<uint32_t> acombo, bcombo;   // bit patterns (in output-index bits)
+    auto subsets = [&](uint32_t mask) {
+        std::vector<uint32_t> v;
+        std::vector<int> bits;
+        for (int x : S)
+            if ((mask >> x) & 1u) bits.push_back(x);
+        for (uint32_t t = 0; t < (1u << bits.size()); ++t) {
+            uint32_t m = 0;
+            for (size_t j = 0; j < bits.size(); ++j)
+                if ((t >> j) & 1u) m |= 1u << bits[j];
+            v.push_back(m);
+        }
+        return v;
+    };
+    acombo = subsets(am);
+    bcombo = subsets(bm);
+    std::ostringstream b;
+    for (size_t x = 0; x < acombo.size(); ++x)
+        b << "      const unsigned ar" << x << " = " << subst_c(ga, "c | " + std::to_string(acombo[x]) + "u") << ";\n";
+    for (size_t y = 0; y < bcombo.size(); ++y)
+        b << "      const unsigned br" << y << " = " << subst_c(gb, "c | " + std::to_string(bcombo[y]) + "u") << ";\n";
+    // operand values, loaded once per (row, k)
+    for (uint32_t k = 0; k < nkk; ++k) {
+        for (size_t x = 0; x < acombo.size(); ++x) {
+            std::ostringstream pre;
+            const std::string nm = "ar" + std::to_string(x);
+            const std::string tmp = "sa" + std::to_string(k) + "_" + std::to_string(x);
+            const std::string v = load_expr(A, "pa", nm.c_str(), k << A.kshift, pre, tmp);
+            b << pre.str();
+            if (v != tmp) b << "      const T " << tmp << " = " << v << ";\n";
+        }
+        for (size_t y = 0; y < bcombo.size(); ++y) {
+            std::ostringstream pre;
+            const std::string nm = "br" + std::to_string(y);
+            const std::string tmp = "sb" + std::to_string(k) + "_" + std::to_string(y);
+            const std::string v = load_expr(B, "pb", nm.c_str(), k << B.kshift, pre, tmp);
+            b << pre.str();
+            if (v != tmp) b << "      const T " << tmp << " = " << v << ";\n";
+        }
+    }
+    // outputs of the block
+    for (uint32_t t = 0; t < (1u << ns); ++t) {
+        uint32_t m = 0;
+        for (int j = 0; j < ns; ++j)
+            if ((t >> j) & 1u) m |= 1u << S[j];
+        const uint32_t ma = m & am, mb = m & bm;
+        const size_t x = std::find(acombo.begin(), acombo.end(), ma) - acombo.begin();
+        const size_t y = std::find(bcombo.begin(), bcombo.end(), mb) - bcombo.begin();
+        const std::string acc = "acc" + std::to_string(t);
+        b << "      T " << acc << " = 0;\n";
+        for (uint32_t k = 0; k < nkk; ++k)
+            b << "      " << acc << " = " << (fp64 ? "fma(" : "fmaf(") << "sa" << k << "_" << x << ", sb" << k << "_" << y
+              << ", " << acc << ");\n";
         if (fp64) {
             if (cexp != 0) b << "      " << acc << " = ldexp(" << acc << ", " << cexp << ");\n";
         } else {
             b << "      " << acc << " = ldexpf(" << acc << ", shift);\n";
             b << "      lmax = " << acc << " > lmax ? " << acc << " : lmax;\n";
         }
-        b << "      pc[(c + " << (long long)u * cstep << "u) << GLOG] = " << acc << ";\n";
+        b << "      pc[(c | " << m << "u) << GLOG] = " << acc << ";\n";
     }
     if (nc <= (long long)cstep) {
         o << "    if (c0 < " << nc << ") {\n      const unsigned c = (unsigned)c0;\n" << b.str() << "    }\n";
-    } else {
-        o << "    #pragma unroll 1\n";
-        o << "    for (unsigned c = (unsigned)c0; c < " << nc << "u; c += " << (long long)U * cstep << "u) {\n"
-          << b.str() << "    }\n";
+        return;
     }
+    // c = c0 | deposit(i, R)
+    std::string dep = "(unsigned)c0";
+    for (size_t j = 0; j < R.size(); ++j)
+        dep += " | (((i >> " + std::to_string(j) + ") & 1u) << " + std::to_string(R[j]) + ")";
+    o << "    #pragma unroll 1\n";
+    o << "    for (unsigned i = 0; i < " << (1u << R.size()) << "u; ++i) {\n";
+    o << "      const unsigned c = " << dep << ";\n";
+    o << b.str();
+    o << "    }\n";
 }
 
 void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64, int unroll) {

@@ -1460,7 +1460,7 @@ void spec_precompile(const tqec_plan *p, const std::vector<SpecCfg> &cands) {
 
 // Autotuner of the specialised kernel, in two stages: (tile 1..64, 64/128/256
 // threads) with tables in global memory, then the best pair with tables in
-// LDS and/or a 4-waves-per-SIMD register cap.  Each configuration is timed
+// LDS, a 4-waves-per-SIMD register cap and 1/2/4/8 outputs per lane block.  Each configuration is timed
 // twice on the trial batch and the fastest is kept.
 int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st, int32_t *tile, int32_t *threads) {
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1536,7 +1536,7 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
         std::vector<SpecCfg> stage2;
         for (bool tab : {false, true})
             for (int wv : {0, 4})
-                for (int un : {1, 2, 4}) {
+                for (int un : {1, 2, 4, 8}) {
                     if (!tab && wv == 0 && un == 1) continue;
                     if (tab && best.slab) continue;
                     SpecCfg c = best;
