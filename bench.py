@@ -68,7 +68,8 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
     L=3/4, Steane and Shor codes, each under code-capacity depolarizing noise
     with p drawn uniformly from [0.01, 0.1]; every call decodes a window of
     batches (code drawn uniformly, size log-uniform in [1, 4096]) in one
-    persistent launch of the mixed-code executor, replayed from a HIP graph.
+    persistent launch of the mixed-code executor (kernel generated from every
+    plan's autotuned specialised code), replayed from a HIP graph.
     Toric L=5/6 are left out: their exact contraction needs per-syndrome
     tensors of rank >= 24 (64 MB per syndrome in FP32)."""
     import numpy as np
@@ -101,6 +102,7 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
             ct.hip.autotune("fused_f32", S.data_ptr(), pool)
         except L.TqecError:
             pass                                  # slab plan: no LDS tile to tune
+        ct.hip.autotune("spec_f32", S.data_ptr(), min(pool, 2048 if st["sc"] > 16 else pool))
         plans.append(ct)
         info.append(dict(name=name, p=p, ns=t.ns, nout=1 << ct.hip.n_out, S=S, truth=truth,
                          macs=st["dep_tc"], sc=st["sc"]))
@@ -113,6 +115,7 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
     off_of = np.array([rng.integers(0, pool - n + 1) for n in size_of])
     max_rows = int(max(size_of[i * window:(i + 1) * window].sum() for i in range(warmup + calls)))
     mx = L.HipMixed([ct.hip for ct in plans], max_batches=window, max_rows=max_rows)
+    mx.set_kernel(True)                           # one kernel generated from every plan's specialised code
     out_mar = [torch.empty(max_rows * 16, dtype=torch.float64, device=dev) for _ in range(2)]
     out_cls = [torch.empty(max_rows, dtype=torch.int32, device=dev) for _ in range(2)]
     out_tie = [torch.empty(max_rows, dtype=torch.uint8, device=dev) for _ in range(2)]
@@ -158,6 +161,16 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
             errors += int((cls_np[row:row + n] != tr).sum())
             decided += n
             row += n
+    # the same batches through the generic (descriptor-interpreting) mixed kernel
+    mx.set_kernel(False)
+    for ci in range(warmup):
+        mx.decode_device(tabs[ci][0], stream=sp)
+    cuda_stream.synchronize()
+    t3 = time.perf_counter()
+    for ci in range(warmup, warmup + calls):
+        mx.decode_device(tabs[ci][0], stream=sp)
+    cuda_stream.synchronize()
+    dt_gen = time.perf_counter() - t3
     # the same batches as one launch sequence per batch (single-plan executors)
     dev_outs = (torch.empty(4096 * 16, dtype=torch.float64, device=dev), torch.empty(4096, dtype=torch.int32, device=dev),
                 torch.empty(4096, dtype=torch.uint8, device=dev))
@@ -166,7 +179,7 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
     for ci in range(warmup, warmup + calls):
         for b in range(ci * window, (ci + 1) * window):
             j, n, o = int(code_of[b]), int(size_of[b]), int(off_of[b])
-            mode = "fused_f32" if mx.plan_config(j)[1] == 0 else "wide_f32"
+            mode = "spec_f32"
             plans[j].hip.decode_device(info[j]["S"][o:].data_ptr(), n, mode, dev_outs[0].data_ptr(),
                                        dev_outs[1].data_ptr(), dev_outs[2].data_ptr())
     torch.cuda.synchronize()
@@ -183,7 +196,9 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
         "batch_latency_us_p50": pct(lat, 50), "batch_latency_us_p99": pct(lat, 99),
         "call_latency_ms_p50": pct(call_ms, 50), "call_latency_ms_p99": pct(call_ms, 99),
         "fp64_rechecks": rechecks, "logical_error_rate_all_codes": errors / max(decided, 1),
+        "generic_mixed_kernel_syndromes_per_s": rows_timed / dt_gen,
         "separate_launches_syndromes_per_s": rows_timed / dt_sep,
+        "separate_launches": "one specialised single-plan launch (plus its FP64 re-check) per batch",
         "setup_s": setup_s,
     }
 

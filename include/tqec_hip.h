@@ -255,6 +255,13 @@ int tqec_mixed_set_graph(tqec_mixed *m, int32_t enable);
 #define TQEC_MIXED_ORDER_GIVEN 0
 #define TQEC_MIXED_ORDER_CHEAPEST_FIRST 1
 int tqec_mixed_set_order(tqec_mixed *m, int32_t order);
+/* Kernel of the executor: the generic interpreter of the step descriptors
+ * (default), or one kernel generated from every plan's specialised code
+ * (each plan's current SPEC_F32 tile, 128 threads, compiled with hiprtc
+ * here; results bit-identical).  Waits for calls in flight. */
+#define TQEC_MIXED_KERNEL_GENERIC 0
+#define TQEC_MIXED_KERNEL_SPEC 1
+int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind);
 /* Tile (syndromes per workgroup) chosen for a plan and where its
  * intermediates live: bit 0 FP32 pass in a slab, bit 1 FP64 re-check in a slab. */
 int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int32_t *slab);

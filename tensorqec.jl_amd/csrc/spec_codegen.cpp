@@ -7,6 +7,7 @@
 #include "spec_codegen.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cctype>
 #include <sstream>
 #include <vector>
@@ -312,35 +313,30 @@ size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg) {
     return cfg.slab ? (size_t)std::max<int64_t>(1, cp.ws_entries) << cfg.glog : 0;
 }
 
-std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
+namespace {
+
+// One tile of G syndromes (rows b0.., or list entries b0..) through the
+// whole step program, as a device function: the body of the generic fused
+// kernel's tile with every descriptor value substituted.  Starts and ends
+// with a workgroup barrier.
+void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &cfg, const std::string &name) {
     const Layout L = layout(cp, cfg);
     const size_t const_n = cp.const_pool.size();
     const int nsteps = (int)cp.steps.size();
-    std::ostringstream o;
-    o << "// plan-specialised " << (cfg.fp64 ? "FP64" : "FP32") << " TNMMAP kernel (generated)\n";
-    o << "typedef " << (cfg.fp64 ? "double" : "float") << " T;\n";
-    o << "#define G " << (1 << cfg.glog) << "\n#define GLOG " << cfg.glog << "\n#define NT " << cfg.threads
-      << "\n#define CSTEP " << (cfg.threads >> cfg.glog) << "\n#define NBITS " << std::max(1, cp.n_syn)
-      << "\n#define NOUT " << (1 << cp.n_out) << "\n#define NSLOTS " << std::max(1, cp.n_slots) << "\n#define NSTEPS "
-      << nsteps << "\n#define TABN " << L.tab_entries << "\n";
-    if (cfg.waves > 0)
-        o << "extern \"C\" __global__ void __launch_bounds__(NT, " << cfg.waves << ") " << spec_kernel_name(cfg) << "(\n";
-    else
-        o << "extern \"C\" __global__ void __launch_bounds__(NT) " << spec_kernel_name(cfg) << "(\n";
-    o << 
-         "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
-         "    const int *__restrict__ list_count, const T *__restrict__ gtab, double *__restrict__ mar,\n"
-         "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
-         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, T *__restrict__ gslab) {\n";
-    o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
+    const char *tn = cfg.fp64 ? "double" : "float";
+    o << "__device__ __forceinline__ void " << name << "(\n"
+      << "    const unsigned char *__restrict__ syn, long long count, const int *__restrict__ list, long long b0,\n"
+      << "    const " << tn << " *__restrict__ tab, double *__restrict__ mar, int *__restrict__ cls,\n"
+      << "    unsigned char *__restrict__ ties, int *rl, int *rcount, long long rcap, double tie_tol,\n"
+      << "    double recheck_tol, unsigned char *smem, " << tn << " *__restrict__ slabw, unsigned tag_base) {\n";
+    o << "  using T = " << tn << ";\n";
+    o << "  constexpr int G = " << (1 << cfg.glog) << ", GLOG = " << cfg.glog << ", NT = " << cfg.threads
+      << ", CSTEP = " << (cfg.threads >> cfg.glog) << ", NBITS = " << std::max(1, cp.n_syn) << ", NOUT = "
+      << (1 << cp.n_out) << ", NSLOTS = " << std::max(1, cp.n_slots) << ";\n";
     if (cfg.slab)
-        o << "  T *work = gslab + (unsigned long long)blockIdx.x * " << spec_slab_entries(cp, cfg) << "ull;\n";
+        o << "  T *work = slabw;\n";
     else
         o << "  T *work = reinterpret_cast<T *>(smem + " << L.work << ");\n";
-    if (cfg.tab_lds)
-        o << "  T *tab = reinterpret_cast<T *>(smem + " << L.tab << ");\n";
-    else
-        o << "  const T *tab = gtab;\n";
     if (!cfg.fp64) {
         o << "  unsigned long long *mx = reinterpret_cast<unsigned long long *>(smem + " << L.mx << ");\n";
         o << "  int *ex = reinterpret_cast<int *>(smem + " << L.ex << ");\n";
@@ -348,14 +344,8 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     }
     o << "  unsigned char *synl = smem + " << L.syn << ";\n";
     o << "  const int tid = threadIdx.x;\n  const int g = tid & (G - 1);\n  const int c0 = tid >> GLOG;\n"
-         "  const int lane = tid & 63;\n";
-    if (cfg.tab_lds) o << "  for (int i = tid; i < TABN; i += NT) tab[i] = gtab[i];\n";
-    o << "  const long long count = list ? (long long)(*list_count) : n;\n";
-    o << "  const long long ntiles = (count + G - 1) >> GLOG;\n";
-    o << "  unsigned iter = 0;\n";
-    o << "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter) {\n";
-    o << "  const long long b0 = t << GLOG;\n";
-    if (!cfg.fp64) o << "  const unsigned tag_base = iter * (unsigned)NSTEPS + 1u;\n";
+         "  const int lane = tid & 63;\n  (void)lane; (void)c0; (void)tag_base; (void)rl; (void)rcount; (void)rcap;\n"
+         "  (void)recheck_tol; (void)NSLOTS;\n";
     o << "  if (!list) {\n"
          "    const long long rows = min((long long)G, count - b0);\n"
          "    const unsigned char *src = syn + b0 * NBITS;\n"
@@ -412,6 +402,129 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     o << "    }\n"
          "  }\n"
          "  __syncthreads();\n"
+         "}\n\n";
+}
+
+std::string dbl(double x) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.17g", x);
+    return buf;
+}
+
+}  // namespace
+
+std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
+    const Layout L = layout(cp, cfg);
+    const int nsteps = (int)cp.steps.size();
+    std::ostringstream o;
+    o << "// plan-specialised " << (cfg.fp64 ? "FP64" : "FP32") << " TNMMAP kernel (generated)\n";
+    emit_tile_fn(o, cp, cfg, "tile");
+    const char *tn = cfg.fp64 ? "double" : "float";
+    const int G = 1 << cfg.glog;
+    o << "extern \"C\" __global__ void __launch_bounds__(" << cfg.threads;
+    if (cfg.waves > 0) o << ", " << cfg.waves;
+    o << ") " << spec_kernel_name(cfg) << "(\n"
+      << "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
+      << "    const int *__restrict__ list_count, const " << tn << " *__restrict__ gtab, double *__restrict__ mar,\n"
+      << "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
+      << "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, " << tn
+      << " *__restrict__ gslab) {\n";
+    o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
+    if (cfg.tab_lds) {
+        o << "  " << tn << " *tab = reinterpret_cast<" << tn << " *>(smem + " << L.tab << ");\n";
+        o << "  for (int i = threadIdx.x; i < " << L.tab_entries << "; i += " << cfg.threads << ") tab[i] = gtab[i];\n";
+    } else {
+        o << "  const " << tn << " *tab = gtab;\n";
+    }
+    if (cfg.slab)
+        o << "  " << tn << " *slabw = gslab + (unsigned long long)blockIdx.x * " << spec_slab_entries(cp, cfg) << "ull;\n";
+    else
+        o << "  " << tn << " *slabw = nullptr;\n";
+    o << "  const long long count = list ? (long long)(*list_count) : n;\n";
+    o << "  const long long ntiles = (count + " << G - 1 << ") >> " << cfg.glog << ";\n";
+    o << "  unsigned iter = 0;\n";
+    o << "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter)\n";
+    o << "    tile(syn, count, list, t << " << cfg.glog << ", tab, mar, cls, ties, rl, rcount, rcap, tie_tol, recheck_tol,"
+         " smem, slabw, iter * " << nsteps << "u + 1u);\n";
+    o << "}\n";
+    return o.str();
+}
+
+size_t spec_mixed_lds_bytes(const std::vector<MixedSpecPlan> &plans) {
+    size_t m = 16;
+    for (const MixedSpecPlan &q : plans) {
+        m = std::max(m, layout(*q.cp, q.f32).total);
+        m = std::max(m, layout(*q.cp, q.f64).total);
+    }
+    return m;
+}
+
+std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threads, size_t slab32_entries,
+                              size_t slab64_entries) {
+    std::ostringstream o;
+    o << "// mixed-code kernel: plan-specialised tiles of " << plans.size() << " plans (generated)\n";
+    // the layouts of the host's batch table, header and counters
+    o << "struct MBatch { const unsigned char *syn; double *mar; int *cls; unsigned char *tie; long long n;\n"
+         "                long long tile0; int plan; int glog; };\n"
+         "struct MHeader { int nbatches; int pad; long long ntiles; };\n"
+         "struct MCounters { unsigned long long next_tile; unsigned long long next_recheck; int rl_count; int pad;\n"
+         "                   unsigned long long t_start; };\n\n";
+    for (size_t i = 0; i < plans.size(); ++i) {
+        emit_tile_fn(o, *plans[i].cp, plans[i].f32, "p" + std::to_string(i) + "_f32");
+        emit_tile_fn(o, *plans[i].cp, plans[i].f64, "p" + std::to_string(i) + "_f64");
+    }
+    o << "extern \"C\" __global__ void __launch_bounds__(" << threads << ") tqec_mixed_spec(\n"
+         "    const MBatch *__restrict__ batches, const MHeader *__restrict__ hdr, MCounters *ctr,\n"
+         "    unsigned long long *t_done, const float *const *tabs32, const double *const *tabs64, float *slab32,\n"
+         "    double *slab64) {\n";
+    o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << spec_mixed_lds_bytes(plans) << "];\n";
+    o << "  __shared__ long long grab;\n  __shared__ int loc_n;\n  __shared__ int loc_rows[64];\n";
+    o << "  float *s32 = slab32 ? slab32 + (unsigned long long)blockIdx.x * " << slab32_entries << "ull : nullptr;\n";
+    o << "  double *s64 = slab64 ? slab64 + (unsigned long long)blockIdx.x * " << slab64_entries << "ull : nullptr;\n";
+    o << "  const int nb = hdr->nbatches;\n  const long long ntiles = hdr->ntiles;\n"
+         "  unsigned tag = 1;\n  bool first = true;\n"
+         "  if (threadIdx.x == 0) loc_n = 0;\n"
+         "  for (;;) {\n"
+         "    __syncthreads();\n"
+         "    if (threadIdx.x == 0) grab = (long long)atomicAdd(&ctr->next_tile, 1ull);\n"
+         "    __syncthreads();\n"
+         "    const long long t = grab;\n"
+         "    if (t >= ntiles) break;\n"
+         "    if (first && threadIdx.x == 0) atomicMin(&ctr->t_start, (unsigned long long)wall_clock64());\n"
+         "    first = false;\n"
+         "    int lo = 0, hi = nb - 1;\n"
+         "    while (lo < hi) {\n"
+         "      const int mid = (lo + hi + 1) >> 1;\n"
+         "      if (batches[mid].tile0 <= t) lo = mid; else hi = mid - 1;\n"
+         "    }\n"
+         "    const MBatch bt = batches[lo];\n"
+         "    switch (bt.plan) {\n";
+    for (size_t i = 0; i < plans.size(); ++i) {
+        const MixedSpecPlan &q = plans[i];
+        o << "    case " << i << ":\n      p" << i << "_f32(bt.syn, bt.n, nullptr, (t - bt.tile0) << " << q.f32.glog
+          << ", tabs32[" << i << "], bt.mar, bt.cls, bt.tie, loc_rows, &loc_n, 64, " << dbl(q.tie_tol) << ", "
+          << dbl(q.recheck_tol) << ", smem, s32, tag);\n      tag += " << q.cp->steps.size() << "u;\n      break;\n";
+    }
+    o << "    default: break;\n    }\n";
+    o << "    const int nre = loc_n;\n"
+         "    if (nre > 0) {\n"
+         "      for (int e = 0; e < nre; ++e) {\n"
+         "        const long long row = loc_rows[e];\n"
+         "        switch (bt.plan) {\n";
+    for (size_t i = 0; i < plans.size(); ++i) {
+        const MixedSpecPlan &q = plans[i];
+        o << "        case " << i << ":\n          p" << i << "_f64(bt.syn, bt.n, nullptr, row, tabs64[" << i
+          << "], bt.mar, bt.cls, bt.tie, nullptr, nullptr, 0, " << dbl(q.tie_tol) << ", " << dbl(q.recheck_tol)
+          << ", smem, s64, 1u);\n          break;\n";
+    }
+    o << "        default: break;\n        }\n"
+         "      }\n"
+         "      if (threadIdx.x == 0) {\n"
+         "        atomicAdd(&ctr->rl_count, nre);\n"
+         "        loc_n = 0;\n"
+         "      }\n"
+         "    }\n"
+         "    if (threadIdx.x == 0) atomicMax(&t_done[lo], (unsigned long long)wall_clock64());\n"
          "  }\n"
          "}\n";
     return o.str();

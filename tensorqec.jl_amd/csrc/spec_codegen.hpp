@@ -15,6 +15,7 @@
 #include <stddef.h>
 
 #include <string>
+#include <vector>
 
 #include "plan_compile.hpp"
 
@@ -46,6 +47,25 @@ size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg);
 // precision (float, or double when cfg.fp64); the FP64 variant ignores the
 // re-check arguments.
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg);
+
+// One plan of the mixed-code kernel: its FP32 tile configuration, the FP64
+// (tile 1) configuration of its re-checks and its tolerances.  Both share
+// the kernel's thread count.
+struct MixedSpecPlan {
+    const CompiledPlan *cp = nullptr;
+    SpecCfg f32, f64;
+    double tie_tol = 1e-12, recheck_tol = 1e-3;
+};
+
+// Static LDS of the mixed kernel (the largest tile of any plan) and its
+// source: kernel `tqec_mixed_spec(const MixedBatch *, const MixedHeader *,
+// MixedCounters *, unsigned long long *t_done, const float *const *tabs32,
+// const double *const *tabs64, float *slab32, double *slab64)` with the
+// batch table, header and counters of the mixed executor (tqec_hip.hip);
+// slabs hold slab32_entries / slab64_entries per workgroup.
+size_t spec_mixed_lds_bytes(const std::vector<MixedSpecPlan> &plans);
+std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threads, size_t slab32_entries,
+                              size_t slab64_entries);
 inline const char *spec_kernel_name(const SpecCfg &cfg) { return cfg.fp64 ? "tqec_spec_f64" : "tqec_spec_f32"; }
 
 }  // namespace tqec

@@ -1625,6 +1625,15 @@ struct tqec_mixed {
     int order = TQEC_MIXED_ORDER_CHEAPEST_FIRST;
     std::vector<double> cost;           // per plan: FP32 work per syndrome (multiply-adds + entries)
     MixedPlan *d_plans = nullptr;
+    // specialised variant (TQEC_MIXED_KERNEL_SPEC): one generated kernel
+    bool spec = false;
+    hipModule_t spec_mod = nullptr;
+    hipFunction_t spec_fn = nullptr;
+    int spec_threads = 128, spec_blocks = 0;
+    std::vector<int> spec_glog, spec_slab;   // per plan: FP32 tile, slab flags (bit 0 FP32, bit 1 FP64)
+    const void **d_tabs32 = nullptr, **d_tabs64 = nullptr;
+    float *spec_slab32 = nullptr;
+    double *spec_slab64 = nullptr;
     char *slab32 = nullptr, *slab64 = nullptr;
     size_t slab32_bytes = 0, slab64_bytes = 0;
     std::array<MixedSlot, 2> slot;
@@ -1663,10 +1672,45 @@ int mixed_enqueue(tqec_mixed *m, MixedSlot &sl, hipStream_t stream) {
     const int cap = std::max(1, m->max_batches);
     hipLaunchKernelGGL(mixed_reset_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, stream, sl.ctr, sl.t_done,
                        cap);
-    MixedArgs a = mixed_args(m, sl);
-    hipLaunchKernelGGL(mixed_kernel, dim3((unsigned)m->blocks), dim3(kMixedThreads), m->lds, stream, a);
+    if (m->spec) {
+        const void *a_batches = reinterpret_cast<const char *>(sl.d_table) + sizeof(MixedHeader);
+        const void *a_hdr = sl.d_table;
+        void *a_ctr = sl.ctr, *a_done = sl.t_done;
+        const void *a_t32 = m->d_tabs32, *a_t64 = m->d_tabs64;
+        void *a_s32 = m->spec_slab32, *a_s64 = m->spec_slab64;
+        void *args[] = {&a_batches, &a_hdr, &a_ctr, &a_done, &a_t32, &a_t64, &a_s32, &a_s64};
+        TQ_HIP(hipModuleLaunchKernel(m->spec_fn, (unsigned)m->spec_blocks, 1, 1, (unsigned)m->spec_threads, 1, 1, 0,
+                                     stream, args, nullptr));
+    } else {
+        MixedArgs a = mixed_args(m, sl);
+        hipLaunchKernelGGL(mixed_kernel, dim3((unsigned)m->blocks), dim3(kMixedThreads), m->lds, stream, a);
+    }
     TQ_HIP(hipGetLastError());
     return TQEC_OK;
+}
+
+void mixed_free_spec(tqec_mixed *m) {
+    if (m->spec_mod) (void)hipModuleUnload(m->spec_mod);
+    for (void *q : {(void *)m->d_tabs32, (void *)m->d_tabs64, (void *)m->spec_slab32, (void *)m->spec_slab64})
+        if (q) (void)hipFree(q);
+    m->spec_mod = nullptr;
+    m->spec_fn = nullptr;
+    m->d_tabs32 = m->d_tabs64 = nullptr;
+    m->spec_slab32 = nullptr;
+    m->spec_slab64 = nullptr;
+    m->spec = false;
+}
+
+// Drop the captured graphs (they hold the old kernel) after the calls finish.
+void mixed_drop_graphs(tqec_mixed *m) {
+    for (MixedSlot &sl : m->slot) {
+        if (sl.used && sl.staged) (void)hipEventSynchronize(sl.staged);
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        sl.exec = nullptr;
+        sl.graph = nullptr;
+        sl.graph_stream = nullptr;
+    }
 }
 
 void mixed_free(tqec_mixed *m) {
@@ -1683,6 +1727,7 @@ void mixed_free(tqec_mixed *m) {
     if (m->d_plans) (void)hipFree(m->d_plans);
     if (m->slab32) (void)hipFree(m->slab32);
     if (m->slab64) (void)hipFree(m->slab64);
+    mixed_free_spec(m);
 }
 
 }  // namespace
@@ -2231,6 +2276,11 @@ int tqec_mixed_set_graph(tqec_mixed *m, int32_t enable) {
 int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int32_t *slab) {
     if (!m) return set_error(TQEC_ENULL, "null executor");
     if (plan < 0 || plan >= (int32_t)m->cfg.size()) return set_error(TQEC_EINVAL, "plan index out of range");
+    if (m->spec) {
+        if (tile) *tile = 1 << m->spec_glog[plan];
+        if (slab) *slab = m->spec_slab[plan];
+        return TQEC_OK;
+    }
     if (tile) *tile = 1 << m->cfg[plan].glog32;
     if (slab) *slab = (m->cfg[plan].gw32 ? 1 : 0) | (m->cfg[plan].gw64 ? 2 : 0);
     return TQEC_OK;
@@ -2274,7 +2324,10 @@ int tqec_mixed_decode_device(tqec_mixed *m, const tqec_mixed_batch *batches, int
     for (int32_t i = 0; i < n_batches; ++i) {
         const tqec_mixed_batch &b = batches[sl.perm[i]];
         int gl = 0;
-        while (gl < m->cfg[b.plan].glog32 && (int64_t(1) << gl) < b.n) ++gl;
+        if (m->spec)
+            gl = m->spec_glog[b.plan];   // the generated tile has a fixed size
+        else
+            while (gl < m->cfg[b.plan].glog32 && (int64_t(1) << gl) < b.n) ++gl;
         MixedBatch &q = tb[i];
         q.syn = b.syndromes;
         q.mar = b.marginals;
@@ -2432,6 +2485,103 @@ int tqec_mixed_set_order(tqec_mixed *m, int32_t order) {
     if (order != TQEC_MIXED_ORDER_GIVEN && order != TQEC_MIXED_ORDER_CHEAPEST_FIRST)
         return set_error(TQEC_EINVAL, "unknown order");
     m->order = order;
+    return TQEC_OK;
+}
+
+int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind) {
+    if (!m) return set_error(TQEC_ENULL, "null executor");
+    if (kind != TQEC_MIXED_KERNEL_GENERIC && kind != TQEC_MIXED_KERNEL_SPEC) return set_error(TQEC_EINVAL, "unknown kind");
+    TQ_HIP(hipSetDevice(m->device));
+    mixed_drop_graphs(m);
+    mixed_free_spec(m);
+    if (kind == TQEC_MIXED_KERNEL_GENERIC) return TQEC_OK;
+    const int nt = m->spec_threads;
+    std::vector<MixedSpecPlan> sp(m->plans.size());
+    size_t lds_limit = 0;
+    int n_cu = 0;
+    size_t s32 = 0, s64 = 0;
+    std::vector<int> glog(m->plans.size()), slabf(m->plans.size());
+    int rc;
+    for (size_t i = 0; i < m->plans.size(); ++i) {
+        tqec_plan *p = m->plans[i];
+        std::lock_guard<std::mutex> lk(p->mu);
+        if ((rc = configure(p))) return rc;
+        lds_limit = p->lds_limit;
+        n_cu = p->n_cu;
+        const size_t budget = p->lds_limit / 2 - 1024;
+        SpecCfg c = p->spec_cfg_set && p->spec_cfg.glog >= 0 ? p->spec_cfg : choose_spec(p);
+        if (c.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan " + std::to_string(i) + " has no specialised tile");
+        c.threads = nt;
+        c.tab_lds = false;
+        c.waves = 0;
+        c.fp64 = false;
+        if (c.glog > 6) c.glog = 6;
+        if (!c.slab) {
+            while (c.glog > 0 && spec_lds_bytes(p->cp, c) > budget) --c.glog;
+            if (spec_lds_bytes(p->cp, c) > budget) {
+                c.slab = true;
+                c.glog = 0;
+                while (c.glog < 4 && spec_slab_entries(p->cp, SpecCfg{c.glog + 1, nt, false, 0, false, true, 1}) * 4 <=
+                                         (size_t(4) << 20))
+                    ++c.glog;
+            }
+        }
+        SpecCfg d;
+        d.glog = 0;
+        d.threads = nt;
+        d.tab_lds = false;
+        d.fp64 = true;
+        d.unroll = c.unroll;
+        if (spec_lds_bytes(p->cp, d) > budget) d.slab = true;
+        sp[i].cp = &p->cp;
+        sp[i].f32 = c;
+        sp[i].f64 = d;
+        sp[i].tie_tol = p->tie_tol;
+        sp[i].recheck_tol = p->recheck_tol;
+        glog[i] = c.glog;
+        slabf[i] = (c.slab ? 1 : 0) | (d.slab ? 2 : 0);
+        if (c.slab) s32 = std::max(s32, spec_slab_entries(p->cp, c));
+        if (d.slab) s64 = std::max(s64, spec_slab_entries(p->cp, d));
+    }
+    const size_t lds = spec_mixed_lds_bytes(sp) + 8 + 4 + 256;
+    if (lds > lds_limit) return set_error(TQEC_EUNSUPPORTED, "the plans' tiles exceed the LDS");
+    std::vector<char> code;
+    if ((rc = spec_code(spec_mixed_source(sp, nt, std::max<size_t>(s32, 1), std::max<size_t>(s64, 1)), code))) return rc;
+    TQ_HIP(hipModuleLoadData(&m->spec_mod, code.data()));
+    hipError_t e = hipModuleGetFunction(&m->spec_fn, m->spec_mod, "tqec_mixed_spec");
+    if (e != hipSuccess) {
+        mixed_free_spec(m);
+        return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
+    }
+    m->spec_blocks = n_cu * resident_per_cu(lds, nt, lds_limit);
+    std::vector<const void *> t32(m->plans.size()), t64(m->plans.size());
+    for (size_t i = 0; i < m->plans.size(); ++i) {
+        DeviceData *dd;
+        if ((rc = ensure_device(m->plans[i], m->device, &dd))) {
+            mixed_free_spec(m);
+            return rc;
+        }
+        t32[i] = dd->tab32;
+        t64[i] = dd->tab64;
+    }
+#define TQ_SP(call)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            mixed_free_spec(m);                                                                  \
+            return set_error(TQEC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+        }                                                                                        \
+    } while (0)
+    TQ_SP(hipMalloc((void **)&m->d_tabs32, t32.size() * sizeof(void *)));
+    TQ_SP(hipMalloc((void **)&m->d_tabs64, t64.size() * sizeof(void *)));
+    TQ_SP(hipMemcpy(m->d_tabs32, t32.data(), t32.size() * sizeof(void *), hipMemcpyHostToDevice));
+    TQ_SP(hipMemcpy(m->d_tabs64, t64.data(), t64.size() * sizeof(void *), hipMemcpyHostToDevice));
+    if (s32) TQ_SP(hipMalloc((void **)&m->spec_slab32, s32 * sizeof(float) * (size_t)m->spec_blocks));
+    if (s64) TQ_SP(hipMalloc((void **)&m->spec_slab64, s64 * sizeof(double) * (size_t)m->spec_blocks));
+#undef TQ_SP
+    m->spec_glog = glog;
+    m->spec_slab = slabf;
+    m->spec = true;
     return TQEC_OK;
 }
 }  // extern "C"

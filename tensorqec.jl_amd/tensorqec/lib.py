@@ -164,6 +164,7 @@ EXPORTS = [
                                           ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     ("tqec_mixed_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("tqec_mixed_set_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("tqec_mixed_set_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("tqec_mixed_plan_config", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p]),
     ("tqec_mixed_decode_device", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MixedBatch), ctypes.c_int32,
                                                  ctypes.c_void_p]),
@@ -493,6 +494,11 @@ class HipMixed:
 
     def set_graph(self, enable: bool) -> None:
         check(load().tqec_mixed_set_graph(self._h, 1 if enable else 0))
+
+    def set_kernel(self, specialised: bool) -> None:
+        """Generic interpreter kernel, or one kernel generated from the plans'
+        specialised code (compiled here, seconds)."""
+        check(load().tqec_mixed_set_kernel(self._h, 1 if specialised else 0))
 
     def set_order(self, cheapest_first: bool) -> None:
         check(load().tqec_mixed_set_order(self._h, 1 if cheapest_first else 0))

@@ -10,9 +10,12 @@ syndromes.
     the row maximum of the plain FP64 executor, classes equal on every row
     whose FP64 top-2 gap is above 1e-12 (the fused_f32 contract).
 Graph replay and direct enqueue give identical results, and a second replay
-with different (smaller) batches reuses the captured graph.
+with different (smaller) batches reuses the captured graph.  Both kernels of
+the executor are covered: the generic interpreter and the kernel generated
+from every plan's specialised code.
 """
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -109,10 +112,15 @@ SPEC1 = [(0, 1), (4, 300), (1, 7), (6, 33), (3, 64), (2, 0), (5, 17), (4, 5), (7
 SPEC2 = [(4, 40), (0, 3), (7, 2), (5, 1), (6, 10)]
 
 
-def test_mixed_stream_matches_single_plan_executors(setup):
+@pytest.mark.parametrize("kernel", ["generic", "specialised"])
+def test_mixed_stream_matches_single_plan_executors(setup, kernel):
     torch, plans = setup
-    say("case mixed: create")
+    say(f"case mixed ({kernel}): create")
     mx = L.HipMixed([ct.hip for _, _, _, ct in plans], max_batches=16, max_rows=8192)
+    if kernel == "specialised":
+        t0 = time.time()
+        mx.set_kernel(True)
+        say(f"  generated kernel compiled in {time.time() - t0:.1f}s")
     for i, (name, *_rest) in enumerate(plans):
         say(f"  plan {name}: tile/slab {mx.plan_config(i)}")
     assert mx.plan_config(6)[1] & 1 == 1          # toric3 needs the slab

@@ -169,8 +169,8 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
     p = 0.05 / 3
     ct = D.compile(D.TNMMAP(mode="fused_f32"), t, iid_error(p, p, p, t.nq))
     src = ct.hip.spec_source(tile, threads, tab, fp64, slab, unroll)
-    assert ("tqec_spec_f64" if fp64 else "tqec_spec_f32") in src and f"#define G {tile}\n" in src
-    assert ("typedef double T;" in src) == fp64
+    assert ("tqec_spec_f64" if fp64 else "tqec_spec_f32") in src and f"constexpr int G = {tile}," in src
+    assert ("using T = double;" in src) == fp64
     f = tmp_path / "spec.hip"
     f.write_text("#include <hip/hip_runtime.h>\n" + src)
     r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
