@@ -58,6 +58,22 @@ STREAM_CODES = (("surface3", ("SurfaceCode", 3)), ("surface5", ("SurfaceCode", 5
                 ("steane", ("SteaneCode", 0)), ("shor", ("ShorCode", 0)))
 
 
+PROFILED = {"seed": 3, "sc_target": None, "tabulate_bits": 3, "mode": "spec_f32", "tile": 8, "threads": 128,
+            "tables_in_lds": True, "slab": False, "waves": 4, "unroll": 4}
+
+
+def measured_traffic(mode, plan_cfg, batch):
+    """Calibrated HBM bytes per launch of the chosen kernel, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when that summary
+    was taken on the same configuration and batch; otherwise None."""
+    path = os.path.join(REPO, "profiles", "spec_f32_d7_p005_counters.md")
+    if batch != 65536 or not os.path.exists(path) or {k: plan_cfg.get(k) for k in PROFILED} != PROFILED:
+        return None
+    import re
+    m = re.search(r"fetch ([0-9.]+) MB \+ write ([0-9.]+) MB", open(path).read())
+    return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
+
+
 def pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
@@ -496,14 +512,15 @@ def main():
     tie = torch.empty(B, dtype=torch.uint8, device=dev)
     print(f"[bench] {nb} batches of {B} syndromes resident", file=sys.stderr, flush=True)
     if args.copy_calibration > 0:
-        # one plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters
-        # calibrate the HBM byte counters of the decode kernels
+        # kernels of known traffic: a fill writes and a sum reads the given
+        # size once; their WRITE_SIZE / FETCH_SIZE counters calibrate the HBM
+        # byte counters of the decode kernels
         n_el = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(n_el, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
+        src = torch.empty(n_el, dtype=torch.float32, device=dev)
+        src.fill_(1.0)
+        float(src.sum().item())
         torch.cuda.synchronize()
-        del src, dst
+        del src
 
     def run(ct, mode, label):
         outs = []
@@ -553,6 +570,7 @@ def main():
             np.max(np.abs(mar_c - mar_p) / mar_p.max(axis=2, keepdims=True)))
         res["plain_kernel_ms_per_step"] = km_p / max(calls_p, 1)
     st = chosen.hip.stats()
+    traffic = measured_traffic(mode_c, plan_cfg, B)
     flops_per_syn = 2.0 * st["dep_tc"]
     avg_ms = km_c / max(calls_c, 1)
     achieved = flops_per_syn * B / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
@@ -619,7 +637,7 @@ def main():
                                                      "table_entries", "const_entries")},
                    "plain": "plain_f64 (reference network, one-hot evidence, one kernel per step)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "tqec_spec_f32" if mode_c == "spec_f32" else "fused_kernel<float, true>",
                      "kernel_ms_avg": avg_ms,
                      "flops_per_syndrome": flops_per_syn,
