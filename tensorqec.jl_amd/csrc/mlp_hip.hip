@@ -153,6 +153,99 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
         }
 }
 
+// Forward GEMM of a hidden layer: H[M][N] = relu(A[M][K] * W + bias) with
+// the weights given transposed, Wt[N][K], so both tiles are rows of 16-byte
+// k-chunks.  Block tile 128 x (64 * NW), K step KS (64, or 32 when K is not
+// a multiple of 64), 4 waves in a 2 x 2 grid each owning 64 x (32 * NW) =
+// 2 x NW MFMA tiles;
+// the next K step's tiles are loaded into registers while the current one
+// is multiplied out of LDS.  Same MFMA sequence (k ascending in steps of 16)
+// as gemm_bf16_kernel, so the results are identical.
+template <int KS, int NW>
+__global__ void __launch_bounds__(256) gemm_fwd_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ Wt,
+                                                       int M, int N, int K, const float *__restrict__ bias,
+                                                       bf16 *__restrict__ out) {
+    constexpr int TN = 64 * NW;                // block tile columns
+    constexpr int LK = KS + 8;                 // padded LDS row (bf16)
+    constexpr int PA = 128 * KS / 8 / 256;     // 16-byte chunks per thread: A tile
+    constexpr int PB = TN * KS / 8 / 256;      //                             B tile
+    __shared__ __align__(16) bf16 As[128 * LK];
+    __shared__ __align__(16) bf16 Bs[TN * LK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * 128, n0 = blockIdx.x * TN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (32 * NW);
+    uint4 ra[PA], rb[PB];
+    auto fetch = [&](int k0) {
+        for (int j = 0; j < PA; ++j) {
+            const int c = tid + j * 256;
+            const int r = c / (KS / 8), kc = (c % (KS / 8)) * 8;
+            const int gm = m0 + r;
+            ra[j] = gm < M ? *reinterpret_cast<const uint4 *>(A + (size_t)gm * K + k0 + kc) : make_uint4(0, 0, 0, 0);
+        }
+        for (int j = 0; j < PB; ++j) {
+            const int c = tid + j * 256;
+            const int r = c / (KS / 8), kc = (c % (KS / 8)) * 8;
+            const int gn = n0 + r;
+            rb[j] = gn < N ? *reinterpret_cast<const uint4 *>(Wt + (size_t)gn * K + k0 + kc) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto stash = [&]() {
+        for (int j = 0; j < PA; ++j) {
+            const int c = tid + j * 256;
+            *reinterpret_cast<uint4 *>(As + (c / (KS / 8)) * LK + (c % (KS / 8)) * 8) = ra[j];
+        }
+        for (int j = 0; j < PB; ++j) {
+            const int c = tid + j * 256;
+            *reinterpret_cast<uint4 *>(Bs + (c / (KS / 8)) * LK + (c % (KS / 8)) * 8) = rb[j];
+        }
+    };
+    f32x16 acc[2][NW];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < NW; ++j)
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += KS) {
+        const bool more = k0 + KS < K;
+        if (more) fetch(k0 + KS);
+        for (int kk = 0; kk < KS; kk += 16) {
+            bf16x8 a[2], b[NW];
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(As + (wm + i * 32 + (lane & 31)) * LK + kk + 8 * (lane >> 5));
+            for (int j = 0; j < NW; ++j)
+                b[j] = *reinterpret_cast<const bf16x8 *>(Bs + (wn + j * 32 + (lane & 31)) * LK + kk + 8 * (lane >> 5));
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < NW; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            stash();
+            __syncthreads();
+        }
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < NW; ++j) {
+            const int col = n0 + wn + j * 32 + (lane & 31);
+            if (col >= N) continue;
+            const float bc = bias[col];
+            for (int v = 0; v < 16; ++v) {
+                const int row = m0 + wm + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (row >= M) continue;
+                out[(size_t)row * N + col] = f2bf(fmaxf(acc[i][j][v] + bc, 0.f));
+            }
+        }
+}
+
+// Wt[n][k] = W[k][n] for the three hidden-layer weight matrices.
+__global__ void transpose_bf16_kernel(const bf16 *__restrict__ W, int K, int N, bf16 *__restrict__ Wt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)K * N) return;
+    const int n = (int)(i / K), k = (int)(i - (int64_t)n * K);
+    Wt[i] = W[(int64_t)k * N + n];
+}
+
 // ---------------------------------------------------------------------------
 // Small kernels
 // ---------------------------------------------------------------------------
@@ -164,6 +257,38 @@ __global__ void embed_kernel(const uint8_t *__restrict__ syn, int64_t B, int n_i
     const int64_t b = i / kin;
     const int f = (int)(i - b * kin);
     X[i] = (f < n_in && (syn[b * n_in + f] & 1)) ? (bf16)0x3f80 : (bf16)0;
+}
+
+// Output layer for hid = 512, nout <= 4: each lane keeps W[8*lane .. 8*lane+7][0..nout)
+// in registers and reads its 8 hidden values of a row with one 16-byte load;
+// waves stride over the rows.  Each lane sums its 8 products, then an
+// xor-shuffle tree adds the lanes (FP32 throughout).
+template <int NOUT>
+__global__ void __launch_bounds__(256) out_fwd512_kernel(const bf16 *__restrict__ H, int64_t B,
+                                                         const bf16 *__restrict__ W, const float *__restrict__ bias,
+                                                         float *__restrict__ z) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    float w[8][NOUT];
+    for (int j = 0; j < 8; ++j)
+        for (int c = 0; c < NOUT; ++c) w[j][c] = bf2f(W[(8 * lane + j) * NOUT + c]);
+    for (int64_t b = wave; b < B; b += nwaves) {
+        const uint4 u = *reinterpret_cast<const uint4 *>(H + b * 512 + 8 * lane);
+        bf16 h[8];
+        memcpy(h, &u, 16);
+        float part[NOUT];
+        for (int c = 0; c < NOUT; ++c) part[c] = 0.f;
+        for (int j = 0; j < 8; ++j) {
+            const float x = bf2f(h[j]);
+            for (int c = 0; c < NOUT; ++c) part[c] += x * w[j][c];
+        }
+        for (int c = 0; c < NOUT; ++c) {
+            float s = part[c];
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            if (lane == 0) z[b * NOUT + c] = s + bias[c];
+        }
+    }
 }
 
 // Output layer: z[b][c] = sum_k H[b][k] W[k][c] + bias[c]; one wave per row.
@@ -321,6 +446,8 @@ struct tqec_mlp {
     int64_t off[8] = {0};          // W1 b1 W2 b2 W3 b3 W4 b4 in the flat parameter vector
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16 *wb = nullptr;
+    bf16 *wt[3] = {nullptr, nullptr, nullptr};   // transposed BF16 weights of the hidden layers (inference GEMM)
+    bool wt_stale = true;
     int64_t step = 0;
     // activations / scratch for up to `cap` rows
     int64_t cap = 0;
@@ -379,23 +506,51 @@ int forward(tqec_mlp *p, const uint8_t *d_syn, int64_t B, hipStream_t st) {
     const bf16 *in[3] = {p->X, p->H1, p->H2};
     bf16 *out[3] = {p->H1, p->H2, p->H3};
     const int kdim[3] = {p->kin, p->hid, p->hid};
+    if (p->wt_stale) {
+        for (int l = 0; l < 3; ++l) {
+            const int64_t n = (int64_t)kdim[l] * p->hid;
+            hipLaunchKernelGGL(transpose_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                               p->wb + p->off[2 * l], kdim[l], p->hid, p->wt[l]);
+        }
+        p->wt_stale = false;
+    }
     for (int l = 0; l < 3; ++l) {
-        GemmArgs a{};
-        a.A = in[l];
-        a.B = p->wb + p->off[2 * l];
-        a.M = (int)B;
-        a.N = p->hid;
-        a.K = kdim[l];
-        a.lda = kdim[l];
-        a.ldb = p->hid;
-        a.ldc = p->hid;
-        a.bias = p->w + p->off[2 * l + 1];
-        a.outb = out[l];
-        gemm<false, false, kEpiBiasReluBf16>(a, st);
+        // 64 x 128 wave tiles (NW = 4) measured slower at d=7 sizes: 252 VGPRs leave one wave per SIMD
+        const bool wide = false;
+        const dim3 grid((unsigned)(p->hid / (wide ? 256 : 128)), (unsigned)((B + 127) / 128));
+        const float *bias = p->w + p->off[2 * l + 1];
+        if (kdim[l] % 64 == 0) {
+            if (wide)
+                hipLaunchKernelGGL((gemm_fwd_kernel<64, 4>), grid, dim3(256), 0, st, in[l], p->wt[l], (int)B, p->hid,
+                                   kdim[l], bias, out[l]);
+            else
+                hipLaunchKernelGGL((gemm_fwd_kernel<64, 2>), grid, dim3(256), 0, st, in[l], p->wt[l], (int)B, p->hid,
+                                   kdim[l], bias, out[l]);
+        } else {
+            if (wide)
+                hipLaunchKernelGGL((gemm_fwd_kernel<32, 4>), grid, dim3(256), 0, st, in[l], p->wt[l], (int)B, p->hid,
+                                   kdim[l], bias, out[l]);
+            else
+                hipLaunchKernelGGL((gemm_fwd_kernel<32, 2>), grid, dim3(256), 0, st, in[l], p->wt[l], (int)B, p->hid,
+                                   kdim[l], bias, out[l]);
+        }
     }
     const unsigned rows_per_block = 4;
-    hipLaunchKernelGGL(out_fwd_kernel, dim3((unsigned)((B + rows_per_block - 1) / rows_per_block)), dim3(256), 0, st,
-                       p->H3, B, p->hid, p->wb + p->off[6], p->w + p->off[7], p->nout, p->z);
+    if (p->hid == 512 && p->nout <= 4) {
+        const unsigned blocks = (unsigned)std::min<int64_t>((B + 3) / 4, 4096);
+        if (p->nout == 4)
+            hipLaunchKernelGGL(out_fwd512_kernel<4>, dim3(blocks), dim3(256), 0, st, p->H3, B, p->wb + p->off[6],
+                               p->w + p->off[7], p->z);
+        else if (p->nout == 2)
+            hipLaunchKernelGGL(out_fwd512_kernel<2>, dim3(blocks), dim3(256), 0, st, p->H3, B, p->wb + p->off[6],
+                               p->w + p->off[7], p->z);
+        else
+            hipLaunchKernelGGL(out_fwd_kernel, dim3((unsigned)((B + rows_per_block - 1) / rows_per_block)), dim3(256),
+                               0, st, p->H3, B, p->hid, p->wb + p->off[6], p->w + p->off[7], p->nout, p->z);
+    } else {
+        hipLaunchKernelGGL(out_fwd_kernel, dim3((unsigned)((B + rows_per_block - 1) / rows_per_block)), dim3(256), 0,
+                           st, p->H3, B, p->hid, p->wb + p->off[6], p->w + p->off[7], p->nout, p->z);
+    }
     TQM_HIP(hipGetLastError());
     return TQEC_OK;
 }
@@ -482,6 +637,8 @@ int tqec_mlp_create(int32_t n_in, int32_t hidden, int32_t n_out, tqec_mlp **out)
     if (e == hipSuccess) e = hipMalloc((void **)&p->m, nb * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&p->v, nb * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&p->wb, nb * 2);
+    for (int l = 0; l < 3 && e == hipSuccess; ++l)
+        e = hipMalloc((void **)&p->wt[l], (size_t)(l == 0 ? p->kin : p->hid) * p->hid * 2);
     if (e == hipSuccess) e = hipMemset(p->w, 0, nb * 4);
     if (e == hipSuccess) e = hipMemset(p->m, 0, nb * 4);
     if (e == hipSuccess) e = hipMemset(p->v, 0, nb * 4);
@@ -509,6 +666,7 @@ int tqec_mlp_set_params(tqec_mlp *p, const float *host) {
                        p->nparams);
     TQM_HIP(hipDeviceSynchronize());
     p->step = 0;
+    p->wt_stale = true;
     return TQEC_OK;
 }
 
@@ -560,6 +718,7 @@ int tqec_mlp_train_step(tqec_mlp *p, const uint8_t *d_syn, const float *d_target
     const float c1 = 1.f - std::pow(b1, (float)p->step), c2 = 1.f - std::pow(b2, (float)p->step);
     hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((p->nparams + 255) / 256)), dim3(256), 0, st, p->w, p->g, p->m,
                        p->v, p->wb, p->nparams, lr, b1, b2, 1e-8f, c1, c2);
+    p->wt_stale = true;
     // padded rows of W1 get zero gradient (zero inputs) and so stay zero
     TQM_HIP(hipGetLastError());
     return TQEC_OK;
@@ -592,7 +751,8 @@ int tqec_posteriors_from_marginals(const double *d_marginals, int64_t n, int32_t
 
 void tqec_mlp_destroy(tqec_mlp *p) {
     if (!p) return;
-    void *bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->X, p->H1, p->H2, p->H3, p->dA, p->dB, p->z, p->dz, p->loss_parts};
+    void *bufs[] = {p->w,  p->g,  p->m,  p->v,  p->wb, p->X,  p->H1,         p->H2,    p->H3,
+                    p->dA, p->dB, p->z,  p->dz, p->loss_parts, p->wt[0], p->wt[1], p->wt[2]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
