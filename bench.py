@@ -342,6 +342,13 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
                        f"{ns} detectors, {len(dem.error_rates)} mechanisms, {n} syndromes",
            "plan": {k: st[k] for k in ("sc", "dep_tc", "dep_rw", "plain_ws_entries", "n_dep_steps")},
            "compile_s": compile_s, "modes": {}}
+    t1 = time.time()
+    try:
+        ct.hip.autotune("spec_f32", S.data_ptr(), min(n, 4096), stream)
+        out["spec_config"] = ct.hip.spec_config()
+    except L.TqecError as e:
+        out["spec_config"] = str(e)
+    out["spec_tune_s"] = time.time() - t1
     ref = None
     for mode in ("plain_f64", "wide_f32", "spec_f32"):
         mar = torch.empty((n, nout), dtype=torch.float64, device=dev)

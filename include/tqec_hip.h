@@ -262,6 +262,11 @@ int tqec_mixed_set_order(tqec_mixed *m, int32_t order);
 #define TQEC_MIXED_KERNEL_GENERIC 0
 #define TQEC_MIXED_KERNEL_SPEC 1
 int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind);
+/* Source of that generated kernel for a set of plans (inspection and
+ * offline compilation checks; no device needed).  Output as
+ * tqec_plan_spec_source. */
+int tqec_mixed_spec_source(tqec_plan *const *plans, int32_t n_plans, int32_t threads, char *buf, size_t cap,
+                           size_t *len);
 /* Tile (syndromes per workgroup) chosen for a plan and where its
  * intermediates live: bit 0 FP32 pass in a slab, bit 1 FP64 re-check in a slab. */
 int tqec_mixed_plan_config(const tqec_mixed *m, int32_t plan, int32_t *tile, int32_t *slab);

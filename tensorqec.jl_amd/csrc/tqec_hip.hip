@@ -1713,6 +1713,65 @@ void mixed_drop_graphs(tqec_mixed *m) {
     }
 }
 
+// Per-plan tiles of the generated mixed kernel: each plan's current SPEC_F32
+// configuration (or the default) with the kernel's thread count and no
+// tables in LDS, its tile shrunk until it fits half the LDS (else a slab),
+// and an FP64 tile of one syndrome for the re-checks.
+int mixed_spec_layout(const std::vector<tqec_plan *> &plans, int nt, size_t lds_limit, std::vector<MixedSpecPlan> &sp,
+                      std::vector<int> &glog, std::vector<int> &slabf, size_t &s32, size_t &s64) {
+    sp.assign(plans.size(), MixedSpecPlan());
+    glog.assign(plans.size(), 0);
+    slabf.assign(plans.size(), 0);
+    s32 = s64 = 0;
+    const size_t budget = lds_limit / 2 - 1024;
+    for (size_t i = 0; i < plans.size(); ++i) {
+        tqec_plan *p = plans[i];
+        if (p->cp.semiring != TQEC_SEMIRING_SUM)
+            return set_error(TQEC_EINVAL, "plan " + std::to_string(i) + " is a max-product plan");
+        SpecCfg c;
+        if (p->spec_cfg_set && p->spec_cfg.glog >= 0) {
+            c = p->spec_cfg;
+        } else {
+            // the default tile of the largest size whose LDS image fits the budget
+            c.glog = 6;
+            c.threads = nt;
+            c.tab_lds = false;
+            while (c.glog > 0 && spec_lds_bytes(p->cp, c) > budget) --c.glog;
+            if (spec_lds_bytes(p->cp, c) > budget) c.slab = true;
+        }
+        c.threads = nt;
+        c.tab_lds = false;
+        c.waves = 0;
+        c.fp64 = false;
+        if (c.glog > 6) c.glog = 6;
+        if (!c.slab) {
+            while (c.glog > 0 && spec_lds_bytes(p->cp, c) > budget) --c.glog;
+            if (spec_lds_bytes(p->cp, c) > budget) c.slab = true;
+        }
+        if (c.slab) {
+            c.glog = std::min(c.glog, 4);
+            while (c.glog > 0 && spec_slab_entries(p->cp, c) * 4 > (size_t(4) << 20)) --c.glog;
+        }
+        SpecCfg d;
+        d.glog = 0;
+        d.threads = nt;
+        d.tab_lds = false;
+        d.fp64 = true;
+        d.unroll = c.unroll;
+        if (spec_lds_bytes(p->cp, d) > budget) d.slab = true;
+        sp[i].cp = &p->cp;
+        sp[i].f32 = c;
+        sp[i].f64 = d;
+        sp[i].tie_tol = p->tie_tol;
+        sp[i].recheck_tol = p->recheck_tol;
+        glog[i] = c.glog;
+        slabf[i] = (c.slab ? 1 : 0) | (d.slab ? 2 : 0);
+        if (c.slab) s32 = std::max(s32, spec_slab_entries(p->cp, c));
+        if (d.slab) s64 = std::max(s64, spec_slab_entries(p->cp, d));
+    }
+    return TQEC_OK;
+}
+
 void mixed_free(tqec_mixed *m) {
     if (m->device >= 0) (void)hipSetDevice(m->device);
     for (MixedSlot &sl : m->slot) {
@@ -2496,53 +2555,19 @@ int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind) {
     mixed_free_spec(m);
     if (kind == TQEC_MIXED_KERNEL_GENERIC) return TQEC_OK;
     const int nt = m->spec_threads;
-    std::vector<MixedSpecPlan> sp(m->plans.size());
     size_t lds_limit = 0;
     int n_cu = 0;
-    size_t s32 = 0, s64 = 0;
-    std::vector<int> glog(m->plans.size()), slabf(m->plans.size());
     int rc;
-    for (size_t i = 0; i < m->plans.size(); ++i) {
-        tqec_plan *p = m->plans[i];
+    for (tqec_plan *p : m->plans) {
         std::lock_guard<std::mutex> lk(p->mu);
         if ((rc = configure(p))) return rc;
         lds_limit = p->lds_limit;
         n_cu = p->n_cu;
-        const size_t budget = p->lds_limit / 2 - 1024;
-        SpecCfg c = p->spec_cfg_set && p->spec_cfg.glog >= 0 ? p->spec_cfg : choose_spec(p);
-        if (c.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan " + std::to_string(i) + " has no specialised tile");
-        c.threads = nt;
-        c.tab_lds = false;
-        c.waves = 0;
-        c.fp64 = false;
-        if (c.glog > 6) c.glog = 6;
-        if (!c.slab) {
-            while (c.glog > 0 && spec_lds_bytes(p->cp, c) > budget) --c.glog;
-            if (spec_lds_bytes(p->cp, c) > budget) {
-                c.slab = true;
-                c.glog = 0;
-                while (c.glog < 4 && spec_slab_entries(p->cp, SpecCfg{c.glog + 1, nt, false, 0, false, true, 1}) * 4 <=
-                                         (size_t(4) << 20))
-                    ++c.glog;
-            }
-        }
-        SpecCfg d;
-        d.glog = 0;
-        d.threads = nt;
-        d.tab_lds = false;
-        d.fp64 = true;
-        d.unroll = c.unroll;
-        if (spec_lds_bytes(p->cp, d) > budget) d.slab = true;
-        sp[i].cp = &p->cp;
-        sp[i].f32 = c;
-        sp[i].f64 = d;
-        sp[i].tie_tol = p->tie_tol;
-        sp[i].recheck_tol = p->recheck_tol;
-        glog[i] = c.glog;
-        slabf[i] = (c.slab ? 1 : 0) | (d.slab ? 2 : 0);
-        if (c.slab) s32 = std::max(s32, spec_slab_entries(p->cp, c));
-        if (d.slab) s64 = std::max(s64, spec_slab_entries(p->cp, d));
     }
+    std::vector<MixedSpecPlan> sp;
+    std::vector<int> glog, slabf;
+    size_t s32 = 0, s64 = 0;
+    if ((rc = mixed_spec_layout(m->plans, nt, lds_limit, sp, glog, slabf, s32, s64))) return rc;
     const size_t lds = spec_mixed_lds_bytes(sp) + 8 + 4 + 256;
     if (lds > lds_limit) return set_error(TQEC_EUNSUPPORTED, "the plans' tiles exceed the LDS");
     std::vector<char> code;
@@ -2582,6 +2607,29 @@ int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind) {
     m->spec_glog = glog;
     m->spec_slab = slabf;
     m->spec = true;
+    return TQEC_OK;
+}
+
+int tqec_mixed_spec_source(tqec_plan *const *plans, int32_t n_plans, int32_t threads, char *buf, size_t cap,
+                           size_t *len) {
+    if (!plans || n_plans < 1) return set_error(TQEC_EINVAL, "need at least one plan");
+    if (threads != 64 && threads != 128 && threads != 256) return set_error(TQEC_EINVAL, "threads must be 64, 128 or 256");
+    std::vector<tqec_plan *> v(plans, plans + n_plans);
+    for (tqec_plan *p : v)
+        if (!p) return set_error(TQEC_ENULL, "null plan");
+    std::vector<MixedSpecPlan> sp;
+    std::vector<int> glog, slabf;
+    size_t s32 = 0, s64 = 0;
+    int rc;
+    // gfx950: 160 KiB of LDS per workgroup
+    if ((rc = mixed_spec_layout(v, threads, size_t(160) << 10, sp, glog, slabf, s32, s64))) return rc;
+    const std::string src = spec_mixed_source(sp, threads, std::max<size_t>(s32, 1), std::max<size_t>(s64, 1));
+    if (len) *len = src.size();
+    if (buf && cap > 0) {
+        const size_t k = std::min(cap - 1, src.size());
+        std::memcpy(buf, src.data(), k);
+        buf[k] = 0;
+    }
     return TQEC_OK;
 }
 }  // extern "C"

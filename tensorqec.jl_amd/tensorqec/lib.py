@@ -165,6 +165,8 @@ EXPORTS = [
     ("tqec_mixed_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("tqec_mixed_set_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("tqec_mixed_set_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("tqec_mixed_spec_source", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("tqec_mixed_plan_config", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p]),
     ("tqec_mixed_decode_device", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MixedBatch), ctypes.c_int32,
                                                  ctypes.c_void_p]),
@@ -525,3 +527,13 @@ class HipMixed:
         rc = ctypes.c_int64()
         check(load().tqec_mixed_last_call(self._h, _ptr(us, _f64p), self.n_last, ctypes.byref(rc)))
         return us[:self.n_last], rc.value
+
+
+def mixed_spec_source(plans, threads: int = 128) -> str:
+    """Source of the mixed-code kernel generated for these HipPlans."""
+    arr = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
+    n = ctypes.c_size_t()
+    check(load().tqec_mixed_spec_source(arr, len(plans), int(threads), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    check(load().tqec_mixed_spec_source(arr, len(plans), int(threads), buf, n.value + 1, ctypes.byref(n)))
+    return buf.value.decode()

@@ -182,3 +182,30 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
     assert m and int(m.group(1)) == 0
     with pytest.raises(L.TqecError):
         ct.hip.spec_source(3, 64, True)          # not a power of two
+
+
+def test_mixed_kernel_source_compiles_for_gfx950(tmp_path):
+    """The mixed-code kernel generated from several plans' specialised code
+    (one LDS plan, one slab plan) compiles for gfx950 without scratch."""
+    import shutil
+    import subprocess
+    from tensorqec.codes import SteaneCode, ToricCode
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not available")
+    plans = []
+    for code in (SurfaceCode(3, 3), SteaneCode(), ToricCode(3, 3)):
+        t = CSSTannerGraph.from_code(code)
+        p = 0.05 / 3
+        plans.append(D.compile(D.TNMMAP(mode="spec_f32"), t, iid_error(p, p, p, t.nq)).hip)
+    src = L.mixed_spec_source(plans, 128)
+    assert "tqec_mixed_spec" in src and "p2_f64" in src and "slabw" in src
+    f = tmp_path / "mixed.hip"
+    f.write_text("#include <hip/hip_runtime.h>\n" + src)
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
+                        str(f), "-o", str(tmp_path / "mixed.o"), "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
+    assert m and int(m.group(1)) == 0
