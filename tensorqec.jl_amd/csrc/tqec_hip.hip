@@ -2582,6 +2582,7 @@ int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind) {
     std::vector<const void *> t32(m->plans.size()), t64(m->plans.size());
     for (size_t i = 0; i < m->plans.size(); ++i) {
         DeviceData *dd;
+        std::lock_guard<std::mutex> lk(m->plans[i]->mu);
         if ((rc = ensure_device(m->plans[i], m->device, &dd))) {
             mixed_free_spec(m);
             return rc;
