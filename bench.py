@@ -348,9 +348,13 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
         out["spec_config"] = ct.hip.spec_config()
     except L.TqecError as e:
         out["spec_config"] = str(e)
+    try:
+        ct.hip.autotune("spec_bf16", S.data_ptr(), min(n, 4096), stream)
+    except L.TqecError:
+        pass
     out["spec_tune_s"] = time.time() - t1
     ref = None
-    for mode in ("plain_f64", "wide_f32", "spec_f32"):
+    for mode in ("plain_f64", "wide_f32", "spec_f32", "spec_bf16"):
         mar = torch.empty((n, nout), dtype=torch.float64, device=dev)
         cls = torch.empty(n, dtype=torch.int32, device=dev)
         tie = torch.empty(n, dtype=torch.uint8, device=dev)
@@ -367,6 +371,8 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
         dt = (time.perf_counter() - h0) / reps
         m, c, ti = mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy()
         r = {"syndromes_per_s": n / dt, "ler": float((c != tr).mean())}
+        if mode.startswith("spec"):
+            r["fp64_rechecks"] = ct.hip.last_recheck_count(stream)
         if ref is None:
             ref = (m, c, ti)
         else:

@@ -71,6 +71,12 @@ extern "C" {
                                  bit-identical to FUSED_F32               */
 #define TQEC_MODE_SPEC_F64 5  /* FUSED_F64 with the plan-specialised kernel;
                                  bit-identical to FUSED_F64               */
+#define TQEC_MODE_SPEC_BF16 6 /* SPEC_F32 with one syndrome per tile and the
+                                 GEMM-shaped steps (M, N >= 32, K >= 16 per
+                                 syndrome) on BF16 MFMA with FP32
+                                 accumulation; syndromes whose top-2 margin
+                                 is below max(recheck_tol, 0.05) are
+                                 re-contracted in FP64                    */
 
 /*
  * Flattened contraction plan.  Mirrors the NestedEinsum that
@@ -284,6 +290,9 @@ void tqec_mixed_destroy(tqec_mixed *m);
 #define TQEC_SPEC_FP64 2           /* FP64 variant (tqec_plan_spec_source only)       */
 #define TQEC_SPEC_SLAB 4           /* intermediates in a per-workgroup global slab,
                                       for plans too wide for LDS                      */
+#define TQEC_SPEC_MFMA 8           /* GEMM-shaped steps on BF16 MFMA (tile 1); with
+                                      tqec_plan_set_spec this pins the SPEC_BF16
+                                      configuration                                   */
 
 /* Source of the plan-specialised kernel for a tile configuration
  * (inspection and offline compilation checks); `flags` as above, with the

@@ -259,7 +259,83 @@ void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int bloc
     o << "    }\n";
 }
 
-void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64, int unroll) {
+// Bits of `mask` (ascending).
+std::vector<int> bits_of(uint32_t mask) {
+    std::vector<int> v;
+    for (int b = 0; b < 32; ++b)
+        if ((mask >> b) & 1u) v.push_back(b);
+    return v;
+}
+
+// `(x)` spread onto the given bit positions: bit t of x goes to pos[t].
+std::string deposit(const std::string &x, const std::vector<int> &pos) {
+    if (pos.empty()) return "0u";
+    std::string r;
+    for (size_t t = 0; t < pos.size(); ++t) {
+        if (t) r += " | ";
+        r += "(((" + x + ") >> " + std::to_string(t) + ") & 1u) << " + std::to_string(pos[t]);
+    }
+    return "(" + r + ")";
+}
+
+// GEMM-shaped step on BF16 MFMA: per syndrome, C[i, j] = sum_k A[i, k] B[k, j]
+// with i over the output bits that feed only A, j over those that feed only
+// B, and the remaining output bits (shared, or beyond the 32 x 32 tile)
+// enumerating tiles.  Operands are rounded to BF16 (nearest even) as they are
+// read, the products accumulate in FP32 inside the MFMA, the output is scaled
+// like every FP32 step.  Layouts (v_mfma_f32_32x32x16_bf16, wave64): lane l
+// holds A[l % 32][8 (l / 32) .. +7], B[8 (l / 32) .. +7][l % 32], and
+// D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] for v = 0..15.
+void emit_outputs_mfma(std::ostringstream &o, const StepDesc &st, int threads) {
+    const OperandDesc &A = st.a, &B = st.b;
+    const uint32_t am = source_mask(A), bm = source_mask(B);
+    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
+    const std::vector<int> aon = bits_of(am & ~bm & outm), bon = bits_of(bm & ~am & outm);
+    const std::vector<int> ai(aon.begin(), aon.begin() + 5), bj(bon.begin(), bon.begin() + 5);
+    std::vector<int> rest;
+    for (int b = 0; b < st.rc; ++b)
+        if (std::find(ai.begin(), ai.end(), b) == ai.end() && std::find(bj.begin(), bj.end(), b) == bj.end())
+            rest.push_back(b);
+    const unsigned ntiles = 1u << rest.size();
+    const int nwaves = threads / 64;
+    const int chunks = 1 << (st.nk - 4);
+    const std::string ga = gather_expr(A), gb = gather_expr(B);
+    auto ld = [&](const OperandDesc &op, const char *base, const char *idx, const std::string &k) {
+        const std::string e = std::string(idx) + " | ((" + k + ") << " + std::to_string(op.kshift) + ")";
+        if (op.kind == kInter) return std::string(base) + "[(" + e + ") << GLOG]";
+        return std::string(base) + "[" + e + "]";
+    };
+    o << "    {\n";
+    o << "      const int l = tid & 63, w = tid >> 6;\n";
+    o << "      for (unsigned t = (unsigned)w; t < " << ntiles << "u; t += " << nwaves << "u) {\n";
+    o << "        const unsigned tb = " << deposit("t", rest) << ";\n";
+    o << "        const unsigned ci = tb | " << deposit("(unsigned)(l & 31)", ai) << ";\n";
+    o << "        const unsigned cj = tb | " << deposit("(unsigned)(l & 31)", bj) << ";\n";
+    o << "        const unsigned ar = " << subst_c(ga, "ci") << ";\n";
+    o << "        const unsigned br = " << subst_c(gb, "cj") << ";\n";
+    o << "        f32x16 acc = {};\n";
+    for (int ch = 0; ch < chunks; ++ch) {
+        o << "        {\n          const unsigned kb = " << ch * 16 << "u + 8u * (unsigned)(l >> 5);\n";
+        o << "          bf16x8 av, bv;\n";
+        for (int e = 0; e < 8; ++e) {
+            const std::string k = "kb + " + std::to_string(e) + "u";
+            o << "          av[" << e << "] = (short)to_bf16(" << ld(A, "pa", "ar", k) << ");\n";
+            o << "          bv[" << e << "] = (short)to_bf16(" << ld(B, "pb", "br", k) << ");\n";
+        }
+        o << "          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);\n        }\n";
+    }
+    o << "        #pragma unroll\n";
+    o << "        for (int v = 0; v < 16; ++v) {\n";
+    o << "          const unsigned i = 8u * (unsigned)(v >> 2) + 4u * (unsigned)(l >> 5) + (unsigned)(v & 3);\n";
+    o << "          const unsigned c = tb | " << deposit("i", ai) << " | " << deposit("(unsigned)(l & 31)", bj) << ";\n";
+    o << "          const float x = ldexpf(acc[v], shift);\n";
+    o << "          lmax = x > lmax ? x : lmax;\n";
+    o << "          pc[c << GLOG] = x;\n";
+    o << "        }\n      }\n    }\n";
+}
+
+void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64, int unroll,
+               const SpecCfg *mfma_cfg) {
     const OperandDesc &A = st.a, &B = st.b;
     o << "  { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << "\n";
     if (fp64) {
@@ -296,7 +372,10 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
     emit_base(o, "pb", B, const_n);
     o << "    T *pc = work + " << (long long)st.c_off << " * G + g;\n";
     o << "    T lmax = 0;\n";
-    emit_outputs(o, st, cstep, unroll, false);
+    if (mfma_cfg && spec_step_uses_mfma(st, *mfma_cfg))
+        emit_outputs_mfma(o, st, mfma_cfg->threads);
+    else
+        emit_outputs(o, st, cstep, unroll, false);
     o << "    for (int w = G; w < 64; w <<= 1) { const float other = __shfl_xor(lmax, w); lmax = other > lmax ? other : lmax; }\n";
     o << "    if (lane < G) atomicMax(&mx[" << st.c_slot << " * G + g], ((unsigned long long)(tag_base + " << s
       << "u) << 32) | (unsigned long long)__float_as_uint(lmax));\n";
@@ -365,7 +444,8 @@ void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &
              "  if (tid < G) lowf[tid] = 0;\n";
     o << "  __syncthreads();\n";
     for (int s = 0; s < nsteps; ++s)
-        emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64, std::max(1, cfg.unroll));
+        emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64, std::max(1, cfg.unroll),
+                  (cfg.mfma && !cfg.fp64) ? &cfg : nullptr);
     o << "  if (tid < G) {\n"
          "    const long long q = b0 + tid;\n"
          "    if (q < count) {\n"
@@ -411,13 +491,32 @@ std::string dbl(double x) {
     return buf;
 }
 
+// Types and helpers of the generated MFMA steps.
+const char *kMfmaPrelude =
+    "typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;\n"
+    "typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;\n"
+    "__device__ __forceinline__ unsigned short to_bf16(float f) {\n"
+    "  const unsigned u = __float_as_uint(f);\n"
+    "  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);\n"
+    "}\n\n";
+
 }  // namespace
+
+bool spec_step_uses_mfma(const StepDesc &st, const SpecCfg &cfg) {
+    if (!cfg.mfma || cfg.fp64 || cfg.glog != 0 || cfg.threads % 64 != 0) return false;
+    if (st.nk < 4) return false;
+    if ((st.a.kind == kInter && st.a.nsingle) || (st.b.kind == kInter && st.b.nsingle)) return false;
+    const uint32_t am = source_mask(st.a), bm = source_mask(st.b);
+    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
+    return __builtin_popcount(am & ~bm & outm) >= 5 && __builtin_popcount(bm & ~am & outm) >= 5;
+}
 
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     const Layout L = layout(cp, cfg);
     const int nsteps = (int)cp.steps.size();
     std::ostringstream o;
     o << "// plan-specialised " << (cfg.fp64 ? "FP64" : "FP32") << " TNMMAP kernel (generated)\n";
+    if (cfg.mfma) o << kMfmaPrelude;
     emit_tile_fn(o, cp, cfg, "tile");
     const char *tn = cfg.fp64 ? "double" : "float";
     const int G = 1 << cfg.glog;
@@ -463,6 +562,9 @@ std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threa
                               size_t slab64_entries) {
     std::ostringstream o;
     o << "// mixed-code kernel: plan-specialised tiles of " << plans.size() << " plans (generated)\n";
+    bool any_mfma = false;
+    for (const MixedSpecPlan &q : plans) any_mfma = any_mfma || q.f32.mfma;
+    if (any_mfma) o << kMfmaPrelude;
     // the layouts of the host's batch table, header and counters
     o << "struct MBatch { const unsigned char *syn; double *mar; int *cls; unsigned char *tie; long long n;\n"
          "                long long tile0; int plan; int glog; };\n"

@@ -29,7 +29,14 @@ struct SpecCfg {
     bool fp64 = false;   // FP64 without scaling (the generic FP64 fused kernel's arithmetic)
     bool slab = false;   // intermediates in a per-workgroup slab of global memory instead of LDS
     int unroll = 1;      // output entries per lane in flight (unroll of the output loop)
+    bool mfma = false;   // FP32 tiles of one syndrome: GEMM-shaped steps on BF16 MFMA (FP32 accumulation)
 };
+
+// A step runs on MFMA (v_mfma_f32_32x32x16_bf16) when the configuration asks
+// for it, the tile holds one syndrome, and per syndrome the step is a GEMM
+// with M, N >= 32 (output bits feeding only A / only B) and K >= 16
+// (contracted bits), with no labels summed on one side only.
+bool spec_step_uses_mfma(const StepDesc &st, const SpecCfg &cfg);
 
 // Static LDS bytes of the specialised kernel for this configuration.
 size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg);

@@ -864,6 +864,8 @@ struct tqec_plan {
     std::map<std::tuple<int, int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64, slab)
     SpecCfg spec64_cfg;                 // TQEC_MODE_SPEC_F64
     bool spec64_cfg_set = false;
+    SpecCfg specb_cfg;                  // TQEC_MODE_SPEC_BF16
+    bool specb_cfg_set = false;
 };
 
 namespace {
@@ -1305,7 +1307,7 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
 
 int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
-                                     (cfg.slab ? 1 : 0) | (cfg.unroll << 1));
+                                     (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.unroll << 2));
     auto it = p->spec.find(key);
     if (it != p->spec.end()) {
         *out = &it->second;
@@ -1414,26 +1416,40 @@ int run_spec64(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn,
     return record(p, w, 1, stream);
 }
 
+// Margin below which the BF16-MFMA mode re-checks a syndrome in FP64: BF16
+// operands carry a relative error of 2^-9 each, so a step's products are off
+// by up to 2^-8 and a chain of steps by a few percent at worst.
+constexpr double kBf16RecheckTol = 0.05;
+
 int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
              int32_t *d_cls, uint8_t *d_tie, hipStream_t stream, const int32_t *in_list = nullptr,
-             const int32_t *in_count = nullptr) {
-    if (!p->spec_cfg_set) {
-        p->spec_cfg = choose_spec(p);
-        p->spec_cfg_set = true;
+             const int32_t *in_count = nullptr, bool bf16 = false) {
+    SpecCfg &cfg = bf16 ? p->specb_cfg : p->spec_cfg;
+    bool &cfg_set = bf16 ? p->specb_cfg_set : p->spec_cfg_set;
+    if (!cfg_set) {
+        cfg = choose_spec(p);
+        if (bf16 && cfg.glog >= 0) {
+            // MFMA steps need tiles of one syndrome
+            cfg.glog = 0;
+            cfg.threads = 256;
+            cfg.mfma = true;
+        }
+        cfg_set = true;
     }
-    if (p->spec_cfg.glog < 0)
+    if (cfg.glog < 0)
         return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP32)");
     SpecKernel *k, *k64;
     int rc;
-    if ((rc = spec_get(p, w.device, p->spec_cfg, &k))) return rc;
+    if ((rc = spec_get(p, w.device, cfg, &k))) return rc;
     if ((rc = spec_recheck_kernel(p, w.device, &k64))) return rc;
     if ((rc = grow((char **)&w.spec_slab, &w.spec_slab_bytes,
                    std::max(spec_slab_need(p, k, n), spec_slab_need(p, k64, n)))))
         return rc;
     if ((rc = prepare_recheck(w, n, stream))) return rc;
     if ((rc = record(p, w, 0, stream))) return rc;
+    const double rtol = bf16 ? std::max(p->recheck_tol, kBf16RecheckTol) : p->recheck_tol;
     if ((rc = spec_launch(p, w, k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, w.list, w.count, w.list_cap,
-                          p->tie_tol, p->recheck_tol, stream)))
+                          p->tie_tol, rtol, stream)))
         return rc;
     if ((rc = record(p, w, 1, stream))) return rc;
     return spec_recheck(p, dd, w, k64, d_syn, n, d_mar, d_cls, d_tie, stream);
@@ -1462,7 +1478,10 @@ void spec_precompile(const tqec_plan *p, const std::vector<SpecCfg> &cands) {
 // threads) with tables in global memory, then the best pair with tables in
 // LDS, a 4-waves-per-SIMD register cap and 1/2/4/8 outputs per lane block.  Each configuration is timed
 // twice on the trial batch and the fastest is kept.
-int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st, int32_t *tile, int32_t *threads) {
+int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st, int32_t *tile, int32_t *threads,
+                  bool bf16) {
+    SpecCfg &tcfg = bf16 ? p->specb_cfg : p->spec_cfg;
+    bool &tset = bf16 ? p->specb_cfg_set : p->spec_cfg_set;
     std::lock_guard<std::mutex> lk(p->mu);
     int rc;
     if ((rc = configure(p))) return rc;
@@ -1481,20 +1500,20 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
     TQ_HIP(hipEventCreate(&e1));
     const bool saved_timing = p->timing;
     p->timing = false;
-    const SpecCfg saved = p->spec_cfg;
-    const bool saved_set = p->spec_cfg_set;
+    const SpecCfg saved = tcfg;
+    const bool saved_set = tset;
     SpecCfg best;
     best.glog = -1;
     float best_ms = 1e30f;
     auto time_all = [&](const std::vector<SpecCfg> &cands) {
         spec_precompile(p, cands);
         for (const SpecCfg &c : cands) {
-            p->spec_cfg = c;
-            p->spec_cfg_set = true;
+            tcfg = c;
+            tset = true;
             float ms = 0;
             for (int rep = 0; rep < 2 && rc == TQEC_OK; ++rep) {
                 if ((rc = record_or(hipEventRecord(e0, st)))) break;
-                rc = run_spec(p, dd, w, d_syn, n, w.d_mar, w.d_cls, w.d_tie, st);
+                rc = run_spec(p, dd, w, d_syn, n, w.d_mar, w.d_cls, w.d_tie, st, nullptr, nullptr, bf16);
                 if (rc) break;
                 if ((rc = record_or(hipEventRecord(e1, st)))) break;
                 if ((rc = record_or(hipEventSynchronize(e1)))) break;
@@ -1508,24 +1527,26 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
         }
     };
     std::vector<SpecCfg> stage1;
-    for (int gl = 0; gl <= 6; ++gl)
+    for (int gl = 0; gl <= (bf16 ? 0 : 6); ++gl)
         for (int thr : {64, 128, 256}) {
             if (thr < (1 << gl)) continue;
             SpecCfg c;
             c.glog = gl;
             c.threads = thr;
             c.tab_lds = false;
+            c.mfma = bf16;
             if (spec_fits(p, c)) stage1.push_back(c);
         }
     if (stage1.empty()) {
         // slab configurations (per-workgroup intermediates of at most 8 MiB)
-        for (int gl = 0; gl <= 4; ++gl)
+        for (int gl = 0; gl <= (bf16 ? 0 : 4); ++gl)
             for (int thr : {128, 256}) {
                 SpecCfg c;
                 c.glog = gl;
                 c.threads = thr;
                 c.tab_lds = false;
                 c.slab = true;
+                c.mfma = bf16;
                 if (spec_fits(p, c) && (gl == 0 || spec_slab_entries(p->cp, c) * 4 <= (size_t(8) << 20)))
                     stage1.push_back(c);
             }
@@ -1551,12 +1572,12 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) {
-        p->spec_cfg = saved;
-        p->spec_cfg_set = saved_set;
+        tcfg = saved;
+        tset = saved_set;
         return rc;
     }
-    p->spec_cfg = best;
-    p->spec_cfg_set = true;
+    tcfg = best;
+    tset = true;
     if (tile) *tile = 1 << best.glog;
     if (threads) *threads = best.threads;
     return TQEC_OK;
@@ -1580,6 +1601,8 @@ int decode_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int mode, doubl
         return run_fused(p, dd, w, d_syn, n, mode, d_mar, d_cls, d_tie, stream);
     if (mode == TQEC_MODE_SPEC_F32) return run_spec(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
     if (mode == TQEC_MODE_SPEC_F64) return run_spec64(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream);
+    if (mode == TQEC_MODE_SPEC_BF16)
+        return run_spec(p, dd, w, d_syn, n, d_mar, d_cls, d_tie, stream, nullptr, nullptr, true);
     return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
 }
 
@@ -1902,7 +1925,7 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
     if (n > 0 && !syn) return set_error(TQEC_ENULL, "null syndrome pointer");
     if (n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "batch too large (max 2^31-1)");
-    if (mode < 0 || mode > TQEC_MODE_SPEC_F64) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
+    if (mode < 0 || mode > TQEC_MODE_SPEC_BF16) return set_error(TQEC_EINVAL, "unknown mode " + std::to_string(mode));
     if (p->cp.semiring != TQEC_SEMIRING_SUM)
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1949,7 +1972,8 @@ int tqec_plan_autotune(tqec_plan *p, int32_t mode, const uint8_t *d_syn, int64_t
                        int32_t *tile, int32_t *threads) {
     if (!p || !d_syn) return set_error(TQEC_ENULL, "null argument");
     if (n <= 0 || n > (int64_t(1) << 31) - 1) return set_error(TQEC_EINVAL, "trial batch size out of range");
-    if (mode == TQEC_MODE_SPEC_F32) return autotune_spec(p, d_syn, n, (hipStream_t)stream, tile, threads);
+    if (mode == TQEC_MODE_SPEC_F32) return autotune_spec(p, d_syn, n, (hipStream_t)stream, tile, threads, false);
+    if (mode == TQEC_MODE_SPEC_BF16) return autotune_spec(p, d_syn, n, (hipStream_t)stream, tile, threads, true);
     if (mode != TQEC_MODE_FUSED_F64 && mode != TQEC_MODE_FUSED_F32)
         return set_error(TQEC_EINVAL, "only the fused modes have tunable tiles");
     std::lock_guard<std::mutex> lk(p->mu);
@@ -2477,6 +2501,7 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     c.tab_lds = (flags & 1) != 0;
     c.fp64 = (flags & 2) != 0;
     c.slab = (flags & 4) != 0;
+    c.mfma = (flags & 8) != 0;
     c.unroll = std::max(1, (flags >> 8) & 0xff);
     std::string src;
     try {
@@ -2514,11 +2539,18 @@ int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t flag
     c.threads = threads;
     c.tab_lds = (flags & 1) != 0;
     c.slab = (flags & 4) != 0;
+    c.mfma = (flags & 8) != 0;
     c.waves = waves;
     c.unroll = unroll ? unroll : 1;
     if (!spec_fits(p, c)) return set_error(TQEC_EUNSUPPORTED, "tile does not fit in LDS");
-    p->spec_cfg = c;
-    p->spec_cfg_set = true;
+    if (c.mfma) {
+        if (c.glog != 0) return set_error(TQEC_EINVAL, "MFMA steps need tiles of one syndrome");
+        p->specb_cfg = c;
+        p->specb_cfg_set = true;
+    } else {
+        p->spec_cfg = c;
+        p->spec_cfg_set = true;
+    }
     return TQEC_OK;
 }
 

@@ -27,8 +27,10 @@ MODE_FUSED_F32 = 2
 MODE_WIDE_F32 = 3
 MODE_SPEC_F32 = 4
 MODE_SPEC_F64 = 5
+MODE_SPEC_BF16 = 6
 MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32,
-         "wide_f32": MODE_WIDE_F32, "spec_f32": MODE_SPEC_F32, "spec_f64": MODE_SPEC_F64}
+         "wide_f32": MODE_WIDE_F32, "spec_f32": MODE_SPEC_F32, "spec_f64": MODE_SPEC_F64,
+         "spec_bf16": MODE_SPEC_BF16}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -337,9 +339,10 @@ class HipPlan:
                 "slab": bool(v[2].value & 4), "waves": v[3].value & 0xff, "unroll": max(1, v[3].value >> 8)}
 
     def spec_source(self, tile: int, threads: int, tables_in_lds: bool = True, fp64: bool = False,
-                    slab: bool = False, unroll: int = 1) -> str:
+                    slab: bool = False, unroll: int = 1, mfma: bool = False) -> str:
         """HIP source of the plan-specialised kernel for a tile choice."""
-        flags = int(bool(tables_in_lds)) | (2 if fp64 else 0) | (4 if slab else 0) | (int(unroll) << 8)
+        flags = (int(bool(tables_in_lds)) | (2 if fp64 else 0) | (4 if slab else 0) | (8 if mfma else 0)
+                 | (int(unroll) << 8))
         n = ctypes.c_size_t()
         check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), flags, None, 0, ctypes.byref(n)))
         buf = ctypes.create_string_buffer(n.value + 1)

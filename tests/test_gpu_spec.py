@@ -166,3 +166,33 @@ def test_spec_wide_plans_vs_plain_f64(case):
     assert err < 2e-5
     ok = valid & (O.tie_flags(r, 1e-12) == 0)
     assert np.array_equal(got[1][ok], O.findmax(r)[ok])
+
+
+def test_spec_bf16_mfma_dem_vs_plain_f64():
+    """Circuit-level d=3, 3-round DEM with its GEMM-shaped steps (M, N >= 32,
+    K >= 16 per syndrome) on BF16 MFMA: marginals within 3e-2 of the row
+    maximum of plain FP64 (BF16 operands: 2^-9 relative each), classes equal
+    on every non-tie syndrome (margins below 5% are re-checked in FP64)."""
+    torch = pytest.importorskip("torch")
+    from oracle import tn_oracle as O
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    dev = torch.device("cuda:0")
+    dem = detector_error_model_from_stim(rotated_memory_z(3, 3, 0.001))
+    ct = D.compile(D.TNMMAP(mode="spec_bf16"), dem)
+    smp = L.HipSampler.flips(dem.error_rates, dem.detector_matrix(), dem.logical_matrix())
+    n = 4096
+    dS = torch.empty((n, smp.nsyn), dtype=torch.uint8, device=dev)
+    smp.sample(29, 0, n, dS.data_ptr())
+    say("case spec bf16 dem d3 r3")
+    ref = run(torch, ct, dS, n, "plain_f64")
+    got = run(torch, ct, dS, n, "spec_bf16")
+    f32 = run(torch, ct, dS, n, "spec_f32")
+    m, r = got[0], ref[0]
+    valid = r.max(axis=1) > 0
+    err = np.max(np.abs(m[valid] - r[valid]) / r[valid].max(axis=1, keepdims=True))
+    err32 = np.max(np.abs(f32[0][valid] - r[valid]) / r[valid].max(axis=1, keepdims=True))
+    say(f"  max rel marginal err bf16 {err:.2e} (fp32 {err32:.2e}), rechecks {ct.hip.last_recheck_count()}")
+    assert err < 3e-2
+    assert err > err32 or err == 0.0          # the MFMA steps did run in BF16
+    ok = valid & (O.tie_flags(r, 1e-12) == 0)
+    assert np.array_equal(got[1][ok], O.findmax(r)[ok])
