@@ -30,6 +30,7 @@ const MODE_FUSED_F32 = Int32(2)
 const MODE_WIDE_F32 = Int32(3)
 const MODE_SPEC_F32 = Int32(4)     # plan-specialised kernel (hiprtc), bit-identical to MODE_FUSED_F32
 const MODE_SPEC_F64 = Int32(5)
+const MODE_SPEC_BF16 = Int32(6)    # GEMM-shaped steps on BF16 MFMA (circuit-level DEMs)
 
 # Mirrors `tqec_plan_desc` in include/tqec_hip.h field by field.
 struct PlanDesc
