@@ -65,9 +65,12 @@ PROFILED = {"seed": 3, "sc_target": None, "tabulate_bits": 3, "mode": "spec_f32"
 def measured_traffic(mode, plan_cfg, batch):
     """Calibrated HBM bytes per launch of the chosen kernel, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when that summary
-    was taken on the same configuration and batch; otherwise None."""
+    was taken on the same plan, executor and batch; otherwise None."""
     path = os.path.join(REPO, "profiles", "spec_f32_d7_p005_counters.md")
-    if batch != 65536 or not os.path.exists(path) or {k: plan_cfg.get(k) for k in PROFILED} != PROFILED:
+    # HBM traffic of an LDS-resident configuration is its syndromes, outputs
+    # and tables: it depends on the plan, not on the tile, threads or unroll
+    same = all(plan_cfg.get(k) == PROFILED[k] for k in ("seed", "sc_target", "tabulate_bits", "mode", "slab"))
+    if batch != 65536 or not os.path.exists(path) or not same:
         return None
     import re
     m = re.search(r"fetch ([0-9.]+) MB \+ write ([0-9.]+) MB", open(path).read())

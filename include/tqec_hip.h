@@ -313,8 +313,8 @@ int tqec_plan_spec_config(tqec_plan *plan, int32_t *tile, int32_t *threads, int3
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);
 
-/* Syndromes re-contracted in FP64 by the last FUSED_F32 call on `stream`
- * (host-visible after the call has completed). */
+/* Syndromes re-contracted in FP64 by the last FP32 call (FUSED_F32,
+ * WIDE_F32, SPEC_F32, SPEC_BF16) on `stream`; waits for that call. */
 int tqec_last_recheck_count(tqec_plan *plan, void *stream, int64_t *count);
 
 /* Project autotuner: time every feasible (syndromes per workgroup, threads

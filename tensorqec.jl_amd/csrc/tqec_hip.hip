@@ -814,6 +814,7 @@ struct Workspace {
     int32_t *d_cls = nullptr; size_t d_cls_bytes = 0;
     uint8_t *d_tie = nullptr; size_t d_tie_bytes = 0;
     int64_t last_recheck = -1;
+    bool count_on_device = false;   // the last call's re-check count is in `count` (not yet read)
     void *smx = nullptr; size_t smx_bytes = 0;       // wide FP32 scales
     void *sex = nullptr; size_t sex_bytes = 0;
     uint8_t *g_syn = nullptr; size_t g_syn_bytes = 0; // wide FP32 re-check staging
@@ -1001,7 +1002,8 @@ int record(tqec_plan *p, Workspace &w, int i, hipStream_t stream) {
     if (i == 0) {
         if (w.n_ev == w.ev.size()) {
             std::array<hipEvent_t, 4> e{};
-            for (auto &x : e) TQ_HIP(hipEventCreate(&x));
+            // timing only: no system-scope cache writeback when they are recorded
+            for (auto &x : e) TQ_HIP(hipEventCreateWithFlags(&x, hipEventDisableSystemFence));
             w.ev.push_back(e);
             w.ev_used.push_back({0, 0, 0, 0});
         }
@@ -1100,6 +1102,7 @@ int run_wide(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
     TQ_HIP(hipStreamSynchronize(stream));
     const int64_t m = *w.host_count;
     w.last_recheck = m;
+    w.count_on_device = false;
     if (m == 0) return TQEC_OK;
     // FP64 re-check of the listed rows
     if ((rc = grow(&w.g_syn, &w.g_syn_bytes, (size_t)m * std::max(1, cp.n_syn)))) return rc;
@@ -1226,7 +1229,7 @@ int enqueue_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d
     if ((rc = record(p, w, 2, stream))) return rc;
     hipLaunchKernelGGL((fused_kernel<double, false>), dim3(rgrid), dim3(c64.threads), c64.lds, stream, r);
     if ((rc = record(p, w, 3, stream))) return rc;
-    TQ_HIP(hipMemcpyAsync(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    w.count_on_device = true;   // read on demand by tqec_last_recheck_count
     TQ_HIP(hipGetLastError());
     return TQEC_OK;
 }
@@ -1392,7 +1395,7 @@ int spec_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, SpecKernel *k, cons
                           p->recheck_tol, stream)))
         return rc;
     if ((rc = record(p, w, 3, stream))) return rc;
-    TQ_HIP(hipMemcpyAsync(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    w.count_on_device = true;   // read on demand by tqec_last_recheck_count
     return TQEC_OK;
 }
 
@@ -1950,7 +1953,11 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (cls) TQ_HIP(hipMemcpyAsync(cls, w.d_cls, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     if (tie) TQ_HIP(hipMemcpyAsync(tie, w.d_tie, (size_t)n, hipMemcpyDeviceToHost, st));
     TQ_HIP(hipStreamSynchronize(st));
-    if (mode == TQEC_MODE_FUSED_F32 && wp && wp->host_count) wp->last_recheck = *wp->host_count;
+    if (wp && wp->count_on_device && wp->host_count) {
+        TQ_HIP(hipMemcpy(wp->host_count, wp->count, sizeof(int32_t), hipMemcpyDeviceToHost));
+        wp->last_recheck = *wp->host_count;
+        wp->count_on_device = false;
+    }
     return TQEC_OK;
 }
 
@@ -1964,7 +1971,14 @@ int tqec_last_recheck_count(tqec_plan *p, void *stream, int64_t *count) {
         *count = -1;
         return TQEC_OK;
     }
-    *count = *it->second.host_count;
+    Workspace &w = it->second;
+    if (w.count_on_device) {
+        TQ_HIP(hipStreamSynchronize((hipStream_t)stream));
+        TQ_HIP(hipMemcpy(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost));
+        w.last_recheck = *w.host_count;
+        w.count_on_device = false;
+    }
+    *count = w.last_recheck;
     return TQEC_OK;
 }
 
