@@ -465,8 +465,8 @@ def main():
     if pinned:
         cands = [(pinned["seed"], pinned["sc_target"], pinned.get("tabulate_bits", 4))]
     else:
-        cands = [(1, 10, 4), (2, 10, 4), (3, 10, 4), (1, 9, 4), (3, None, 4), (2, 9, 4), (1, None, 4), (1, None, 3),
-                 (3, None, 3), (1, None, 2)]
+        cands = [(1, 10, 4), (3, 10, 4), (1, 9, 4), (3, None, 4), (1, None, 4), (1, None, 3), (3, None, 3),
+                 (2, None, 3), (4, None, 3), (1, None, 2)]
     outs = [torch.empty((B, 4), dtype=torch.float64, device=dev), torch.empty(B, dtype=torch.int32, device=dev),
             torch.empty(B, dtype=torch.uint8, device=dev)]
 
