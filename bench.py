@@ -317,6 +317,29 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
     }
 
 
+def mlp_leg(torch, dev, seed, n_in=48, batch=65536, reps=20):
+    """Inference of the learned front-end alone: the 48-512-512-512-4 MLP
+    (random weights) on a batch of syndromes, forward + confidence gate."""
+    from tensorqec.learned import HipMLP
+    mlp = HipMLP(n_in, 4, seed=seed % 1000)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    S = (torch.rand((batch, n_in), device=dev, generator=g) < 0.1).to(torch.uint8)
+    z = torch.empty((batch, 4), dtype=torch.float32, device=dev)
+    for _ in range(3):
+        mlp.forward(S.data_ptr(), batch, z.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mlp.forward(S.data_ptr(), batch, z.data_ptr())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    flops = 2.0 * batch * (mlp.kin * 512 + 2 * 512 * 512 + 512 * 4)
+    return {"workload": f"MLP {n_in}-512-512-512-4 forward, BF16 weights, FP32 accumulation, {batch} syndromes",
+            "ms_per_batch": dt * 1e3, "syndromes_per_s": batch / dt, "tflops": flops / dt / 1e12,
+            "frac_of_bf16_peak": flops / dt / 2.5e15}
+
+
 def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
     """Circuit-level memory-Z experiment (rotated surface code, depolarizing
     p after every 1- and 2-qubit gate, reset and measurement flips p),
@@ -409,7 +432,7 @@ def main():
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
-    ap.add_argument("--only", default=None, choices=["stream", "learned", "dem"],
+    ap.add_argument("--only", default=None, choices=["stream", "learned", "dem", "mlp"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--dem-d", type=int, default=3, help="distance of the circuit-level DEM workload")
     ap.add_argument("--dem-rounds", type=int, default=3, help="syndrome rounds of the circuit-level DEM workload")
@@ -426,7 +449,7 @@ def main():
 
     if args.only:
         kw = {"dem": {"d": args.dem_d, "rounds": args.dem_rounds}}.get(args.only, {})
-        fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg}[args.only]
+        fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg, "mlp": mlp_leg}[args.only]
         print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 

@@ -44,8 +44,8 @@ size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg);
 // Entries of one workgroup's slab (slab configurations).
 size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg);
 
-// HIP source of the kernel `tqec_spec_f32` / `tqec_spec_f64` (extern "C",
-// see spec_kernel_name).  Parameters:
+// HIP source of the kernel `tqec_spec_f32` / `tqec_spec_bf16` /
+// `tqec_spec_f64` (extern "C", see spec_kernel_name).  Parameters:
 //   (const uint8_t *syn, int64_t n, const int32_t *list, const int32_t *list_count,
 //    const T *tab, double *mar, int32_t *cls, uint8_t *tie,
 //    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, T *slab)
@@ -73,6 +73,8 @@ struct MixedSpecPlan {
 size_t spec_mixed_lds_bytes(const std::vector<MixedSpecPlan> &plans);
 std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threads, size_t slab32_entries,
                               size_t slab64_entries);
-inline const char *spec_kernel_name(const SpecCfg &cfg) { return cfg.fp64 ? "tqec_spec_f64" : "tqec_spec_f32"; }
+inline const char *spec_kernel_name(const SpecCfg &cfg) {
+    return cfg.fp64 ? "tqec_spec_f64" : (cfg.mfma ? "tqec_spec_bf16" : "tqec_spec_f32");
+}
 
 }  // namespace tqec
