@@ -317,6 +317,107 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
     }
 
 
+def pipeline_leg(torch, dev, seed, ct=None, mode="spec_f32", d=7, p=0.05, batch=65536, steps=10):
+    """The whole decode loop on the device at d=7, p=0.05: Philox sampling of
+    errors and syndromes, TN decoding, recovery of the error pattern (GF(2)
+    pure error + logical fix, the DecodingResult payload), and the logical
+    error count against the sampled class; nothing crosses to the host but
+    the final counts."""
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.codes import CSSTannerGraph, SurfaceCode
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+    t = CSSTannerGraph.from_code(SurfaceCode(d, d))
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    if ct is None:
+        ct = D.compile(D.TNMMAP(mode=mode), t, em)
+    lx, lz = logical_operator(t)
+    smp = L.HipSampler.css(t, em, lx, lz)
+    rec = L.HipRecovery.css(t, ct.pure_x, ct.pure_z, ct.lx, ct.lz)
+    S = torch.empty((batch, t.ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(batch, dtype=torch.int32, device=dev)
+    mar = torch.empty((batch, 4), dtype=torch.float64, device=dev)
+    cls = torch.empty(batch, dtype=torch.int32, device=dev)
+    tie = torch.empty(batch, dtype=torch.uint8, device=dev)
+    ex = torch.empty((batch, t.nq), dtype=torch.uint8, device=dev)
+    ez = torch.empty((batch, t.nq), dtype=torch.uint8, device=dev)
+    errors = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def one(k):
+        smp.sample(seed + 11, k * batch, batch, S.data_ptr(), truth.data_ptr(), stream=stream)
+        ct.hip.decode_device(S.data_ptr(), batch, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), stream)
+        rec.recover(S.data_ptr(), cls.data_ptr(), batch, ex.data_ptr(), ez.data_ptr(), stream=stream)
+        errors.add_((cls != truth).sum())
+
+    one(0)
+    torch.cuda.synchronize()
+    errors.zero_()
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        one(k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = steps * batch
+    e = int(errors.item())
+    return {"workload": f"device loop at d={d}, p={p}: sample, decode ({mode}), recover error patterns, count "
+                        f"logical errors; {steps} x {batch} syndromes",
+            "syndromes_per_s": n / dt, "ms_per_batch": dt / steps * 1e3, "logical_errors": e,
+            "ler": e / n, "ler_ci95": wilson(e, n)}
+
+
+def tnmap_leg(torch, dev, seed, d=5, p=0.05, batch=16384, reps=3):
+    """TNMAP (most probable error configuration, max-product contraction with
+    an arg-max backtrace) at code capacity: throughput and LER against the
+    sampled class, with the TNMMAP (marginal) decision on the same syndromes
+    for comparison."""
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.codes import CSSTannerGraph, SurfaceCode
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+    from tensorqec.tnmap import TNMAP, compile_tnmap
+    t = CSSTannerGraph.from_code(SurfaceCode(d, d))
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    t0 = time.time()
+    cm = compile_tnmap(TNMAP(), t, em)
+    compile_s = time.time() - t0
+    lx, lz = logical_operator(t)
+    smp = L.HipSampler.css(t, em, lx, lz)
+    S = torch.empty((batch, t.ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(batch, dtype=torch.int32, device=dev)
+    smp.sample(seed + 21, 0, batch, S.data_ptr(), truth.data_ptr())
+    nconf = cm.qubit_num
+    cfg = torch.empty((batch, nconf), dtype=torch.uint8, device=dev)
+    lp = torch.empty(batch, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    cm.hip.map_decode_device(S.data_ptr(), batch, nconf, cfg.data_ptr(), lp.data_ptr(), stream)
+    torch.cuda.synchronize()
+    h0 = time.perf_counter()
+    for _ in range(reps):
+        cm.hip.map_decode_device(S.data_ptr(), batch, nconf, cfg.data_ptr(), lp.data_ptr(), stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - h0) / reps
+    # class of the most probable configuration: logical parities of (ex, ez)
+    import numpy as np
+    c = cfg.cpu().numpy().astype(np.int64)
+    n = t.nq
+    exx, ezz = c[:, :n], c[:, n:2 * n]
+    k = lx.shape[0]
+    cls = np.zeros(batch, np.int64)
+    for i in range(k):
+        cls |= ((ezz @ lx[i].astype(np.int64)) & 1) << i
+        cls |= ((exx @ lz[i].astype(np.int64)) & 1) << (k + i)
+    tr = truth.cpu().numpy()
+    ct = D.compile(D.TNMMAP(mode="spec_f32"), t, em)
+    r = D.decode_batch(ct, S.cpu().numpy(), error_patterns=False)
+    return {"workload": f"TNMAP, rotated surface code d={d}, code-capacity p={p}, {batch} syndromes",
+            "compile_s": compile_s, "syndromes_per_s": batch / dt, "ler_tnmap": float((cls != tr).mean()),
+            "ler_tnmmap": float((r.classes != tr).mean()),
+            "class_agreement_with_tnmmap": float((cls == r.classes).mean())}
+
+
 def mlp_leg(torch, dev, seed, n_in=48, batch=65536, reps=20):
     """Inference of the learned front-end alone: the 48-512-512-512-4 MLP
     (random weights) on a batch of syndromes, forward + confidence gate."""
@@ -432,7 +533,7 @@ def main():
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
-    ap.add_argument("--only", default=None, choices=["stream", "learned", "dem", "mlp"],
+    ap.add_argument("--only", default=None, choices=["stream", "learned", "dem", "mlp", "pipeline", "tnmap"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--dem-d", type=int, default=3, help="distance of the circuit-level DEM workload")
     ap.add_argument("--dem-rounds", type=int, default=3, help="syndrome rounds of the circuit-level DEM workload")
@@ -449,7 +550,8 @@ def main():
 
     if args.only:
         kw = {"dem": {"d": args.dem_d, "rounds": args.dem_rounds}}.get(args.only, {})
-        fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg, "mlp": mlp_leg}[args.only]
+        fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg, "mlp": mlp_leg, "pipeline": pipeline_leg,
+              "tnmap": tnmap_leg}[args.only]
         print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 
@@ -647,6 +749,13 @@ def main():
         t0 = time.time()
         res["learned"] = learned_leg(torch, dev, args.seed, ct=chosen, mode=mode_c, d=d, p=p)
         print(f"[bench] learned leg {time.time() - t0:.1f}s: {res['learned']}", file=sys.stderr, flush=True)
+        t0 = time.time()
+        res["pipeline"] = pipeline_leg(torch, dev, args.seed, ct=chosen, mode=mode_c, d=d, p=p)
+        print(f"[bench] pipeline leg {time.time() - t0:.1f}s: {res['pipeline']['syndromes_per_s']:.0f} syn/s",
+              file=sys.stderr, flush=True)
+        t0 = time.time()
+        res["tnmap"] = tnmap_leg(torch, dev, args.seed)
+        print(f"[bench] tnmap leg {time.time() - t0:.1f}s: {res['tnmap']}", file=sys.stderr, flush=True)
         t0 = time.time()
         res["dem"] = dem_leg(torch, dev, args.seed)
         print(f"[bench] dem leg {time.time() - t0:.1f}s: {res['dem']['modes']}", file=sys.stderr, flush=True)

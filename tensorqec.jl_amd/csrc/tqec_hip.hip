@@ -1278,16 +1278,20 @@ int spec_code(const std::string &src, std::vector<char> &code) {
 
 bool spec_fits(const tqec_plan *p, const SpecCfg &c) { return spec_lds_bytes(p->cp, c) <= p->lds_limit; }
 
-// Default before tuning: as choose_fused, preferring tables in LDS.
+// Default before tuning: the largest tile whose LDS image (tables
+// included when they fit) lets three workgroups share a CU (else two, else
+// one), 16 threads per syndrome within [64, 128], blocks of 4 outputs per
+// lane.  On the d=7 code-capacity plans this is what the autotuner picks.
 SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
-    for (int div : {4, 2, 1}) {
+    for (int div : {3, 2, 1}) {
         for (int gl = 6; gl >= 0; --gl) {
             for (bool tab : {true, false}) {
                 SpecCfg c;
                 c.glog = gl;
-                c.threads = std::max(64, std::min(128, 1 << gl));
+                c.threads = std::max(64, std::min(128, 16 << gl));
                 c.tab_lds = tab;
                 c.fp64 = fp64;
+                c.unroll = 4;
                 if (spec_lds_bytes(p->cp, c) <= p->lds_limit / div) return c;
             }
         }

@@ -283,6 +283,13 @@ class HipPlan:
                                            _ptr(cfg, _u8p) if cfg.size else None, _ptr(lp, _f64p), None))
         return cfg, lp
 
+    def map_decode_device(self, d_syn: int, n: int, n_config: int, d_config: int, d_logp: int, stream: int = 0) -> None:
+        """Device-pointer max-product decode; enqueues only."""
+        check(load().tqec_map_decode_batch_device(self._h, ctypes.c_void_p(d_syn), int(n), int(n_config),
+                                                  ctypes.c_void_p(d_config) if d_config else None,
+                                                  ctypes.c_void_p(d_logp) if d_logp else None,
+                                                  ctypes.c_void_p(stream) if stream else None))
+
     def set_tolerances(self, tie_tol: float, recheck_tol: float) -> None:
         check(load().tqec_plan_set_tolerances(self._h, float(tie_tol), float(recheck_tol)))
 
