@@ -515,8 +515,8 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--d", type=int, default=7)
     ap.add_argument("--p", type=float, default=0.05)
@@ -527,9 +527,9 @@ def main():
     ap.add_argument("--config", default=None,
                     help="JSON line of an earlier run: reuse its plan and tile choice (no autotuning)")
     ap.add_argument("--plan", default=None,
-                    help="SEED,SC_TARGET,MODE,TILE,THREADS[,TABLES_IN_LDS,WAVES[,UNROLL[,TABULATE_BITS]]]: pin the plan, "
-                         "executor and tile "
-                         "(no autotuning); SC_TARGET may be none; MODE fused_f32 or spec_f32")
+                    help="SEED,SC_TARGET,MODE,TILE,THREADS[,TABLES_IN_LDS,WAVES[,UNROLL[,TABULATE_BITS[,REG]]]]: pin the "
+                         "plan, executor and tile (no autotuning); SC_TARGET may be none; MODE fused_f32 or spec_f32; "
+                         "REG 1 = register-resident specialised kernel (TILE = THREADS / 4)")
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
@@ -587,6 +587,8 @@ def main():
             pinned["unroll"] = int(f[7])
         if len(f) > 8:
             pinned["tabulate_bits"] = int(f[8])
+        if len(f) > 9:
+            pinned["reg"] = bool(int(f[9]))
     if pinned:
         cands = [(pinned["seed"], pinned["sc_target"], pinned.get("tabulate_bits", 4))]
     else:
@@ -596,6 +598,10 @@ def main():
             torch.empty(B, dtype=torch.uint8, device=dev)]
 
     def time_mode(ct, mode):
+        # one untimed call first: it compiles a specialised kernel on first use
+        ct.hip.decode_device(S_dev[0].data_ptr(), B, mode, outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr(),
+                             stream)
+        torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record()
         for _ in range(3):
@@ -605,9 +611,12 @@ def main():
         torch.cuda.synchronize()
         return ev0.elapsed_time(ev1) / 3
 
-    # Candidate contraction trees (seed, width target): each is autotuned with
-    # the generic fused kernel; the two fastest are also autotuned with the
-    # plan-specialised kernel; the fastest (plan, executor, tile) is kept.
+    # Candidate contraction trees (seed, width target, tabulation depth): each
+    # is timed with the plan-specialised kernel in its default configuration
+    # (the register-resident kernel where the plan's live tensors fit), the
+    # three fastest are autotuned, and the fastest (plan, configuration) is
+    # kept.  Without the specialised kernel the generic fused kernel is
+    # autotuned instead.
     tried = []
     for seed, sct, tab in cands:
         ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.TreeSA(seed=seed, sc_target=sct), tabulate_bits=tab), t,
@@ -618,19 +627,26 @@ def main():
                    "threads": pinned["threads"]}
             if pinned["mode"] == "spec_f32":
                 ct.hip.set_spec(pinned["tile"], pinned["threads"], pinned.get("tables_in_lds", False),
-                                pinned.get("waves", 0), pinned.get("slab", False), pinned.get("unroll", 1))
+                                pinned.get("waves", 0), pinned.get("slab", False), pinned.get("unroll", 1),
+                                pinned.get("reg", False))
                 cfg.update(ct.hip.spec_config())
             else:
                 ct.hip.set_tile("fused_f32", pinned["tile"], pinned["threads"])
             tried.append((0.0, ct, cfg))
             continue
-        tile, thr = ct.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
-        ms = time_mode(ct, "fused_f32")
+        if args.no_spec:
+            tile, thr = ct.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
+            ms = time_mode(ct, "fused_f32")
+            cfg = {"seed": seed, "sc_target": sct, "tabulate_bits": tab, "mode": "fused_f32", "tile": tile,
+                   "threads": thr}
+        else:
+            ms = time_mode(ct, "spec_f32")
+            cfg = dict({"seed": seed, "sc_target": sct, "tabulate_bits": tab, "mode": "spec_f32"},
+                       **ct.hip.spec_config())
         print(f"[bench] candidate seed={seed} sc_target={sct} tabulate_bits={tab}: steps={st['n_dep_steps']} "
               f"macs={st['dep_tc']:.0f} lds/syn={st['fused_lds_entries']} tables={st['table_entries']} "
-              f"fused_f32 tile={tile} threads={thr} {ms:.3f} ms/batch", file=sys.stderr, flush=True)
-        tried.append((ms, ct, {"seed": seed, "sc_target": sct, "tabulate_bits": tab, "mode": "fused_f32", "tile": tile,
-                               "threads": thr}))
+              f"{cfg} {ms:.3f} ms/batch", file=sys.stderr, flush=True)
+        tried.append((ms, ct, cfg))
     tried.sort(key=lambda x: x[0])
     best = tried[0]
     if not pinned and not args.no_spec:
@@ -645,6 +661,12 @@ def main():
             if ms < best[0]:
                 best = (ms, ct, dict(cfg, mode="spec_f32", **sc))
     chosen, plan_cfg = best[1], best[2]
+    fused_ms = None
+    if plan_cfg["mode"] == "spec_f32":
+        plan_cfg = dict(plan_cfg, **chosen.hip.spec_config())   # the configuration the plan now holds
+        # the generic (descriptor-interpreting) FP32 kernel on the chosen plan, for comparison
+        chosen.hip.autotune("fused_f32", S_dev[0].data_ptr(), B, stream)
+        fused_ms = time_mode(chosen, "fused_f32")
     print(f"[bench] plans compiled and tuned in {time.time() - t0:.1f}s: {plan_cfg}", file=sys.stderr, flush=True)
     truth = [true_classes(e, chosen.lx, chosen.lz) for e in eps]
     nout = 1 << chosen.hip.n_out
@@ -680,11 +702,13 @@ def main():
             i = args.warmup + k
             ct.hip.decode_device(S_dev[i].data_ptr(), B, mode, mar_all[k].data_ptr(), cls_all[k].data_ptr(),
                                  tie_all[k].data_ptr(), stream)
+        t_enq = time.perf_counter() - t0
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         ct.hip.set_timing(False)
         kmain, krec, calls = ct.hip.kernel_time_totals(stream)
-        print(f"[bench] {label}: {args.steps} steps in {dt * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] {label}: {args.steps} steps in {dt * 1e3:.2f} ms (enqueued in {t_enq * 1e3:.2f} ms; kernels "
+              f"{kmain:.2f} ms + re-check {krec:.2f} ms)", file=sys.stderr, flush=True)
         return dt, kmain, krec, calls, cls_all.cpu().numpy(), tie_all.cpu().numpy(), mar_all.cpu().numpy()
 
     mode_c = plan_cfg["mode"]
@@ -716,6 +740,8 @@ def main():
     avg_ms = km_c / max(calls_c, 1)
     achieved = flops_per_syn * B / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     res["recheck_ms_per_step"] = kr_c / max(calls_c, 1)
+    if fused_ms:
+        res["generic_fused_f32_syndromes_per_s"] = B / (fused_ms * 1e-3)
     res["recheck_fraction"] = None
 
     cpu = None
@@ -776,9 +802,12 @@ def main():
         "config": {"workload": f"rotated surface code d={d}, code-capacity depolarizing p={p} (p/3 each), "
                                f"{B} syndromes per step, TNMMAP marginals over 4 logical classes",
                    "batch": B, "d": d, "p": p, "seed": args.seed,
-                   "chosen": f"{mode_c} (chain-factorized network, evidence as slices, subtrees on <= 4 "
-                             "syndrome bits tabulated, FP32 with per-tensor power-of-two scaling, FP64 re-check"
+                   "chosen": f"{mode_c} (chain-factorized network, evidence as slices, subtrees on <= "
+                             f"{plan_cfg.get('tabulate_bits', 4)} syndrome bits tabulated, FP32 with per-tensor "
+                             "power-of-two scaling, FP64 re-check"
                              + (", plan-specialised kernel compiled with hiprtc" if mode_c == "spec_f32" else "")
+                             + (", one syndrome per quad of lanes with every tensor in registers"
+                                if plan_cfg.get("reg") else "")
                              + ")",
                    "plan": plan_cfg,
                    "plan_stats": {k: st[k] for k in ("n_dep_steps", "dep_tc", "dep_rw", "fused_lds_entries",
@@ -789,7 +818,10 @@ def main():
                      "kernel": "tqec_spec_f32" if mode_c == "spec_f32" else "fused_kernel<float, true>",
                      "kernel_ms_avg": avg_ms,
                      "flops_per_syndrome": flops_per_syn,
-                     "compute": "FP32 vector ALU with LDS-resident operands (no MFMA on this path)"},
+                     "compute": ("FP32 vector ALU, tensors in registers, DPP quad exchanges, tables in "
+                                 + ("LDS" if plan_cfg.get("tables_in_lds") else "global memory")
+                                 + " (no MFMA on this path)") if plan_cfg.get("reg") else
+                                "FP32 vector ALU with LDS-resident operands (no MFMA on this path)"},
         "cpu_baseline": cpu,
         "extra": res,
     }

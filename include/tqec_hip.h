@@ -293,6 +293,10 @@ void tqec_mixed_destroy(tqec_mixed *m);
 #define TQEC_SPEC_MFMA 8           /* GEMM-shaped steps on BF16 MFMA (tile 1); with
                                       tqec_plan_set_spec this pins the SPEC_BF16
                                       configuration                                   */
+#define TQEC_SPEC_REG 16           /* FP32, one syndrome per quad of lanes with every
+                                      tensor in registers (no LDS intermediates, no
+                                      barriers); tile = threads / 4; results
+                                      bit-identical to the other FP32 kernels       */
 
 /* Source of the plan-specialised kernel for a tile configuration
  * (inspection and offline compilation checks); `flags` as above, with the
@@ -303,7 +307,7 @@ int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, 
                           size_t cap, size_t *len);
 
 /* Pin / read the TQEC_MODE_SPEC_F32 configuration: tile, threads,
- * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB flags, and `waves` = minimum
+ * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB | TQEC_SPEC_REG flags, and `waves` = minimum
  * waves per SIMD requested from the compiler (bits 0-7, 0 = none) | unroll
  * of the per-lane output loop << 8 (1, 2, 4 or 8; 0 = 1).  No reference
  * counterpart. */

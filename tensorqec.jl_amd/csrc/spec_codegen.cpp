@@ -384,9 +384,449 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
     o << "  __syncthreads();\n";
 }
 
+// ---- register-resident variant ------------------------------------------
+//
+// A quad of lanes owns one syndrome.  Tensor X of rank r keeps entry i in
+// lane sum_t bit(i, pos[t]) << t of the quad and in register pext(i, other
+// bits) of that lane; a missing pos[t] replicates X along lane bit t.  The
+// consumer of X chooses X's lane bits: an output lane bit that feeds X is
+// placed in the same lane slot of X, so those index bits never cross lanes
+// and the register an operand entry lives in never depends on the lane.  A
+// fetch is then one DPP quad permutation (or a plain register read when the
+// permutation is the identity).  Layouts are decided from the root down.
+
+struct RLay {
+    int pos[2] = {-1, -1};   // tensor index bit held by lane bit t (-1: replicated along t)
+};
+
+int seg_image(const OperandDesc &op, int u) {
+    for (int i = 0; i < op.nseg; ++i) {
+        const Seg &g = op.seg[i];
+        if (g.width > 0 && u >= g.src && u < g.src + g.width) return g.dst + (u - g.src);
+    }
+    return -1;
+}
+
+uint32_t gather_host(uint32_t c, const OperandDesc &op) {
+    uint32_t r = 0;
+    for (int i = 0; i < op.nseg; ++i) {
+        const Seg &g = op.seg[i];
+        if (g.width <= 0) continue;
+        const uint32_t w = (g.width >= 32) ? 0xffffffffu : ((1u << g.width) - 1u);
+        r |= ((c >> g.src) & w) << g.dst;
+    }
+    return r;
+}
+
+int lane_bits(const RLay &l) { return (l.pos[0] >= 0) + (l.pos[1] >= 0); }
+
+std::vector<int> reg_bits(const RLay &l, int rank) {
+    std::vector<int> v;
+    for (int b = 0; b < rank; ++b)
+        if (b != l.pos[0] && b != l.pos[1]) v.push_back(b);
+    return v;
+}
+
+uint32_t pext(uint32_t x, const std::vector<int> &bits) {
+    uint32_t r = 0;
+    for (size_t j = 0; j < bits.size(); ++j) r |= ((x >> bits[j]) & 1u) << j;
+    return r;
+}
+
+uint32_t pdep(uint32_t x, const std::vector<int> &bits) {
+    uint32_t r = 0;
+    for (size_t j = 0; j < bits.size(); ++j) r |= ((x >> j) & 1u) << bits[j];
+    return r;
+}
+
+// Registers per lane above which the variant is not offered (512 VGPRs and
+// AGPRs per lane at one wave per SIMD, less the step temporaries).
+constexpr int kRegBudget = 400;
+
+struct RegPlan {
+    std::vector<int> prod[2];   // producing step of each step's intermediate A / B, or -1
+    std::vector<RLay> lay;      // per step output
+    int root = -1;
+    int peak = 0;               // live tensor registers per lane
+    std::string err;
+};
+
+bool reg_plan(const CompiledPlan &cp, RegPlan &rp) {
+    const int ns = (int)cp.steps.size();
+    if (ns == 0) return rp.err = "no per-syndrome steps", false;
+    if (cp.n_out > 4) return rp.err = "more than 16 classes", false;
+    rp.prod[0].assign(ns, -1);
+    rp.prod[1].assign(ns, -1);
+    rp.lay.assign(ns, RLay());
+    std::vector<int> uses(ns, 0);
+    for (int s = 0; s < ns; ++s) {
+        const StepDesc &st = cp.steps[s];
+        if (st.rc > 16) return rp.err = "tensor too wide for registers", false;
+        for (int side = 0; side < 2; ++side) {
+            const OperandDesc &op = side ? st.b : st.a;
+            if (op.kind != kInter) continue;
+            int p = -1;
+            for (int j = s - 1; j >= 0; --j)
+                if (cp.steps[j].c_off == op.off) {
+                    p = j;
+                    break;
+                }
+            if (p < 0 || cp.steps[p].rc != op.rank) return rp.err = "operand without a producing step", false;
+            rp.prod[side][s] = p;
+            ++uses[p];
+        }
+        if (st.is_root) rp.root = s;
+    }
+    if (rp.root < 0 || cp.steps[rp.root].c_off != cp.root_off || cp.steps[rp.root].rc != cp.n_out)
+        return rp.err = "root step not found", false;
+    for (int s = 0; s < ns; ++s)
+        if (uses[s] != (s == rp.root ? 0 : 1)) return rp.err = "step output not used exactly once", false;
+    // layouts, consumers first
+    RLay &lr = rp.lay[rp.root];
+    for (int t = 0; t < 2; ++t)
+        if (cp.steps[rp.root].rc > t) lr.pos[t] = t;
+    for (int s = ns - 1; s >= 0; --s) {
+        const StepDesc &st = cp.steps[s];
+        const RLay LC = rp.lay[s];
+        for (int side = 0; side < 2; ++side) {
+            const int p = rp.prod[side][s];
+            if (p < 0) continue;
+            const OperandDesc &op = side ? st.b : st.a;
+            RLay L;
+            for (int t = 0; t < 2; ++t)
+                if (LC.pos[t] >= 0) L.pos[t] = seg_image(op, LC.pos[t]);
+            // free slots: the bit that feeds most of the producer's intermediate operands
+            const StepDesc &ps = cp.steps[p];
+            for (int t = 0; t < 2; ++t) {
+                if (L.pos[t] >= 0) continue;
+                int best = -1, best_score = -1;
+                for (int v = 0; v < op.rank; ++v) {
+                    if (v == L.pos[0] || v == L.pos[1]) continue;
+                    int score = 0;
+                    if (ps.a.kind == kInter && seg_image(ps.a, v) >= 0) ++score;
+                    if (ps.b.kind == kInter && seg_image(ps.b, v) >= 0) ++score;
+                    if (score > best_score) {
+                        best = v;
+                        best_score = score;
+                    }
+                }
+                L.pos[t] = best;
+            }
+            rp.lay[p] = L;
+        }
+    }
+    // live registers per lane in execution order
+    int live = 0;
+    for (int s = 0; s < ns; ++s) {
+        const StepDesc &st = cp.steps[s];
+        live += 1 << (st.rc - lane_bits(rp.lay[s]));
+        rp.peak = std::max(rp.peak, live);
+        for (int side = 0; side < 2; ++side) {
+            const int p = rp.prod[side][s];
+            if (p >= 0) live -= 1 << (cp.steps[p].rc - lane_bits(rp.lay[p]));
+        }
+    }
+    if (rp.peak > kRegBudget) return rp.err = "more live tensor entries per lane than registers", false;
+    return true;
+}
+
+// quad_perm control: destination lane d of each quad reads lane src[d]
+int quad_ctrl(const int src[4]) { return src[0] | (src[1] << 2) | (src[2] << 4) | (src[3] << 6); }
+
+// Expression for entry `idx` (lane-independent bits) of intermediate X, made
+// by step p with layout LX, read by a consumer with output layout LC.
+bool reg_fetch(const OperandDesc &op, const RLay &LC, const RLay &LX, int p, uint32_t idx, std::string &out,
+               std::string &err) {
+    int img[2];
+    for (int t = 0; t < 2; ++t) img[t] = LC.pos[t] >= 0 ? seg_image(op, LC.pos[t]) : -1;
+    for (int t = 0; t < 2; ++t)
+        if (img[t] >= 0 && img[t] != LX.pos[t]) return err = "lane bit of an operand moved", false;
+    int src[4];
+    for (int d = 0; d < 4; ++d) {
+        src[d] = 0;
+        for (int t = 0; t < 2; ++t) {
+            int bit;
+            if (LX.pos[t] < 0 || img[t] >= 0)
+                bit = (d >> t) & 1;
+            else
+                bit = (idx >> LX.pos[t]) & 1;
+            src[d] |= bit << t;
+        }
+    }
+    const int ctrl = quad_ctrl(src);
+    const std::string name = "x" + std::to_string(p) + "_" + std::to_string(pext(idx, reg_bits(LX, op.rank)));
+    out = (ctrl == 0xE4) ? name : "QP(" + name + ", " + std::to_string(ctrl) + ")";
+    return true;
+}
+
+bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan &rp, int s, size_t const_n,
+                   std::string &err) {
+    const StepDesc &st = cp.steps[s];
+    const RLay &LC = rp.lay[s];
+    const std::vector<int> RB = reg_bits(LC, st.rc);
+    const uint32_t nreg = 1u << RB.size();
+    const uint32_t nkk = 1u << st.nk;
+    o << "    float m" << s << ";\n    int e" << s << ";\n    float";
+    for (uint32_t r = 0; r < nreg; ++r) o << (r ? ", " : " ") << "x" << s << "_" << r;
+    o << ";\n";
+    o << "    { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << ", " << nreg
+      << " registers per lane\n";
+    const char *ab[2] = {"a", "b"};
+    for (int side = 0; side < 2; ++side) {
+        const OperandDesc &op = side ? st.b : st.a;
+        const int p = rp.prod[side][s];
+        if (op.kind == kInter)
+            o << "      const int n" << ab[side] << " = (m" << p << " > 0.f) ? ilogbf(m" << p << ") : 0, e" << ab[side]
+              << " = e" << p << " + n" << ab[side] << ";\n";
+        else
+            o << "      const int n" << ab[side] << " = 0, e" << ab[side] << " = " << op.cexp << ";\n";
+    }
+    o << "      const int shift = -(na + nb);\n";
+    {
+        std::string cond;
+        if (st.a.kind == kInter) cond += "na < -100";
+        if (st.b.kind == kInter) cond += std::string(cond.empty() ? "" : " || ") + "nb < -100";
+        if (!cond.empty()) o << "      if (" << cond << ") lowf = 1;\n";
+    }
+    bool has_lane_off[2] = {false, false};
+    for (int side = 0; side < 2; ++side) {
+        const OperandDesc &op = side ? st.b : st.a;
+        if (op.kind == kInter) continue;
+        if (op.kind == kConst) {
+            o << "      const float *p" << ab[side] << " = tab + " << (long long)op.off << ";\n";
+        } else {
+            o << "      const unsigned v" << ab[side] << " = 0u";
+            for (int j = 0; j < op.nev; ++j) o << " | (s" << op.ev_bit[j] << " << " << j << ")";
+            o << ";\n      const float *p" << ab[side] << " = tab + " << (long long)(const_n + (size_t)op.off) << " + (v"
+              << ab[side] << " << " << op.rank << ");\n";
+        }
+        std::string lo;
+        for (int t = 0; t < 2; ++t) {
+            if (LC.pos[t] < 0) continue;
+            const uint32_t gbit = gather_host(1u << LC.pos[t], op);
+            if (!gbit) continue;
+            int sh = 0;
+            while ((1u << sh) != gbit) ++sh;
+            lo += std::string(lo.empty() ? "" : " | ") + "((((unsigned)q >> " + std::to_string(t) + ") & 1u) << " +
+                  std::to_string(sh) + ")";
+        }
+        if (!lo.empty()) {
+            o << "      const unsigned l" << ab[side] << " = " << lo << ";\n";
+            has_lane_off[side] = true;
+        }
+    }
+    o << "      float";
+    for (uint32_t r = 0; r < nreg; ++r) o << (r ? ", " : " ") << "acc" << r << " = 0.f";
+    o << ";\n";
+    int ntmp = 0;
+    for (uint32_t k = 0; k < nkk; ++k) {
+        o << "      {\n";
+        std::vector<std::pair<std::string, std::string>> cache;   // expression -> temporary
+        auto value = [&](int side, uint32_t cst, std::string &out) -> bool {
+            const OperandDesc &op = side ? st.b : st.a;
+            const uint32_t idx = gather_host(cst, op) | (k << op.kshift);
+            std::string expr;
+            if (op.kind != kInter) {
+                expr = std::string("p") + ab[side] + "[" + (has_lane_off[side] ? std::string("l") + ab[side] + " + " : "") +
+                       std::to_string(idx) + "u]";
+            } else {
+                const int p = rp.prod[side][s];
+                if (op.nsingle == 0) {
+                    if (!reg_fetch(op, LC, rp.lay[p], p, idx, expr, err)) return false;
+                    if (expr.compare(0, 3, "QP(") != 0) {
+                        out = expr;
+                        return true;
+                    }
+                } else {
+                    // single-side labels summed first, ascending (as the generic kernel)
+                    expr = "S";
+                    std::vector<std::string> terms;
+                    for (uint32_t qq = 0; qq < (1u << op.nsingle); ++qq) {
+                        std::string t1;
+                        if (!reg_fetch(op, LC, rp.lay[p], p, idx | (qq << op.sshift), t1, err)) return false;
+                        terms.push_back(t1);
+                        expr += "|" + t1;
+                    }
+                    for (auto &c : cache)
+                        if (c.first == expr) {
+                            out = c.second;
+                            return true;
+                        }
+                    const std::string nm = "f" + std::to_string(ntmp++);
+                    o << "        float " << nm << " = 0.f;\n";
+                    for (auto &t1 : terms) o << "        " << nm << " += " << t1 << ";\n";
+                    cache.emplace_back(expr, nm);
+                    out = nm;
+                    return true;
+                }
+            }
+            for (auto &c : cache)
+                if (c.first == expr) {
+                    out = c.second;
+                    return true;
+                }
+            const std::string nm = "f" + std::to_string(ntmp++);
+            o << "        const float " << nm << " = " << expr << ";\n";
+            cache.emplace_back(expr, nm);
+            out = nm;
+            return true;
+        };
+        for (uint32_t r = 0; r < nreg; ++r) {
+            const uint32_t cst = pdep(r, RB);
+            std::string va, vb;
+            if (!value(0, cst, va) || !value(1, cst, vb)) return false;
+            o << "        acc" << r << " = fmaf(" << va << ", " << vb << ", acc" << r << ");\n";
+        }
+        o << "      }\n";
+    }
+    // Entries are >= 0 or NaN, and NaN-ignoring fmaxf over them equals the
+    // generic kernel's running `x > m ? x : m` from 0 (max is exact and order
+    // free); a tree of three-input maxima halves the instructions.
+    std::vector<std::string> mx;
+    for (uint32_t r = 0; r < nreg; ++r) {
+        o << "      x" << s << "_" << r << " = ldexpf(acc" << r << ", shift);\n";
+        mx.push_back("x" + std::to_string(s) + "_" + std::to_string(r));
+    }
+    mx.push_back("0.f");
+    int nm = 0;
+    while (mx.size() > 1) {
+        std::vector<std::string> nx;
+        for (size_t i = 0; i < mx.size(); i += 3) {
+            if (i + 1 >= mx.size()) {
+                nx.push_back(mx[i]);
+                continue;
+            }
+            const std::string t = "mx" + std::to_string(nm++);
+            o << "      const float " << t << " = ";
+            if (i + 2 < mx.size())
+                o << "fmaxf(fmaxf(" << mx[i] << ", " << mx[i + 1] << "), " << mx[i + 2] << ");\n";
+            else
+                o << "fmaxf(" << mx[i] << ", " << mx[i + 1] << ");\n";
+            nx.push_back(t);
+        }
+        mx.swap(nx);
+    }
+    o << "      float lmax = " << mx[0] << ";\n";
+    o << "      lmax = fmaxf(QP(lmax, 177), lmax);\n      lmax = fmaxf(QP(lmax, 78), lmax);\n";
+    o << "      m" << s << " = lmax;\n      e" << s << " = ea + eb;\n    }\n";
+    return true;
+}
+
+std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &err) {
+    RegPlan rp;
+    if (!reg_plan(cp, rp)) {
+        err = rp.err;
+        return "";
+    }
+    const size_t const_n = cp.const_pool.size();
+    const size_t tab_n = cp.const_pool.size() + cp.sliced_pool.size();
+    const int G = cfg.threads / 4;
+    const int nout = 1 << cp.n_out;
+    std::ostringstream o;
+    o << "// plan-specialised FP32 TNMMAP kernel, one syndrome per quad of lanes, tensors in registers (generated)\n";
+    o << "#define QP(x, c) __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), (c), 0xf, 0xf, false))\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << cfg.threads;
+    if (cfg.waves > 0) o << ", " << cfg.waves;
+    o << ") tqec_spec_f32(\n"
+         "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
+         "    const int *__restrict__ list_count, const float *__restrict__ gtab, double *__restrict__ mar,\n"
+         "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
+         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, float *__restrict__ gslab) {\n";
+    o << "  constexpr int G = " << G << ", NBITS = " << std::max(1, cp.n_syn) << ", NOUT = " << nout << ";\n";
+    o << "  (void)gslab;\n";
+    if (cfg.tab_lds) {
+        o << "  __shared__ float tab[" << std::max<size_t>(1, tab_n) << "];\n";
+        o << "  for (int i = threadIdx.x; i < " << tab_n << "; i += " << cfg.threads << ") tab[i] = gtab[i];\n";
+        o << "  __syncthreads();\n";
+    } else {
+        o << "  const float *tab = gtab;\n";
+    }
+    o << "  const int q = threadIdx.x & 3;\n"
+         "  const long long count = list ? (long long)(*list_count) : n;\n"
+         "  const long long ntiles = (count + G - 1) / G;\n"
+         "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {\n"
+         "    const long long qi = t * G + (threadIdx.x >> 2);\n"
+         "    const bool valid = qi < count;\n"
+         "    const long long b = valid ? (list ? (long long)list[qi] : qi) : 0ll;\n"
+         "    const unsigned char *row = syn + b * NBITS;\n";
+    std::vector<char> used(std::max(1, cp.n_syn), 0);
+    for (const StepDesc &st : cp.steps)
+        for (const OperandDesc *op : {&st.a, &st.b})
+            if (op->kind == kSliced)
+                for (int j = 0; j < op->nev; ++j) used[op->ev_bit[j]] = 1;
+    for (int j = 0; j < cp.n_syn; ++j)
+        if (used[j]) o << "    const unsigned s" << j << " = valid ? (unsigned)(row[" << j << "] & 1u) : 0u;\n";
+    o << "    int lowf = 0;\n";
+    for (int s = 0; s < (int)cp.steps.size(); ++s)
+        if (!emit_reg_step(o, cp, rp, s, const_n, err)) return "";
+    // every lane gathers the root's entries; the quad's lane 0 finishes the row
+    const RLay &LR = rp.lay[rp.root];
+    const std::vector<int> RR = reg_bits(LR, cp.steps[rp.root].rc);
+    o << "    float rv[NOUT];\n";
+    for (int j = 0; j < nout; ++j) {
+        int src[4];
+        for (int d = 0; d < 4; ++d) {
+            src[d] = 0;
+            for (int t = 0; t < 2; ++t) {
+                const int bit = LR.pos[t] >= 0 ? ((j >> LR.pos[t]) & 1) : ((d >> t) & 1);
+                src[d] |= bit << t;
+            }
+        }
+        const int ctrl = quad_ctrl(src);
+        const std::string nm = "x" + std::to_string(rp.root) + "_" + std::to_string(pext((uint32_t)j, RR));
+        o << "    rv[" << j << "] = " << (ctrl == 0xE4 ? nm : "QP(" + nm + ", " + std::to_string(ctrl) + ")") << ";\n";
+    }
+    o << "    if (q == 0 && valid) {\n"
+         "      const int e = e" << rp.root << ";\n"
+         "      float best = 0, second = -1;\n"
+         "      int bi = 0;\n"
+         "      for (int j = 0; j < NOUT; ++j) {\n"
+         "        const float x = rv[j];\n"
+         "        if (mar) mar[b * NOUT + j] = ldexp((double)x, e);\n"
+         "        if (j == 0 || x > best) {\n"
+         "          if (j > 0) second = best;\n"
+         "          best = x;\n"
+         "          bi = j;\n"
+         "        } else if (x > second) {\n"
+         "          second = x;\n"
+         "        }\n"
+         "      }\n"
+         "      if (cls) cls[b] = bi;\n"
+         "      const bool low = NOUT > 1 && second >= 0 && (double)(best - second) <= tie_tol * (double)best;\n"
+         "      if (ties) ties[b] = low ? 1 : 0;\n"
+         "      if (rl) {\n"
+         "        const bool rc = !(best > 0) || lowf != 0 ||\n"
+         "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
+         "        if (rc) {\n"
+         "          const int slot = atomicAdd(rcount, 1);\n"
+         "          if (slot < rcap) rl[slot] = (int)b;\n"
+         "        }\n"
+         "      }\n"
+         "    }\n"
+         "  }\n"
+         "}\n";
+    return o.str();
+}
+
 }  // namespace
 
-size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg) { return layout(cp, cfg).total; }
+size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg) {
+    if (cfg.reg) return cfg.tab_lds ? round_up(std::max<size_t>(1, cp.const_pool.size() + cp.sliced_pool.size()) * 4, 16) : 0;
+    return layout(cp, cfg).total;
+}
+
+bool spec_reg_supported(const CompiledPlan &cp, std::string *why) {
+    RegPlan rp;
+    const bool ok = reg_plan(cp, rp);
+    if (!ok && why) *why = rp.err;
+    return ok;
+}
+
+int spec_reg_peak(const CompiledPlan &cp) {
+    RegPlan rp;
+    reg_plan(cp, rp);
+    return rp.peak;
+}
 
 size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg) {
     return cfg.slab ? (size_t)std::max<int64_t>(1, cp.ws_entries) << cfg.glog : 0;
@@ -512,6 +952,10 @@ bool spec_step_uses_mfma(const StepDesc &st, const SpecCfg &cfg) {
 }
 
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
+    if (cfg.reg) {
+        std::string err;
+        return reg_source(cp, cfg, err);
+    }
     const Layout L = layout(cp, cfg);
     const int nsteps = (int)cp.steps.size();
     std::ostringstream o;

@@ -30,7 +30,20 @@ struct SpecCfg {
     bool slab = false;   // intermediates in a per-workgroup slab of global memory instead of LDS
     int unroll = 1;      // output entries per lane in flight (unroll of the output loop)
     bool mfma = false;   // FP32 tiles of one syndrome: GEMM-shaped steps on BF16 MFMA (FP32 accumulation)
+    bool reg = false;    // FP32, one syndrome per quad of lanes, every tensor in registers (glog = log2(threads / 4))
 };
+
+// Register-resident variant (cfg.reg): each syndrome is owned by a quad of
+// lanes; a tensor's entries are spread over the quad by two of its index bits
+// and over registers by the rest, operands cross lanes with DPP quad
+// permutations and per-tensor maxima with two DPP steps, so the step program
+// touches no LDS and needs no barrier.  Returns false (with the reason) when
+// the plan does not suit it: more than 16 classes, or more live tensor entries
+// per lane than the register file holds.
+bool spec_reg_supported(const CompiledPlan &cp, std::string *why = nullptr);
+
+// Peak live tensor registers per lane of the register-resident variant.
+int spec_reg_peak(const CompiledPlan &cp);
 
 // A step runs on MFMA (v_mfma_f32_32x32x16_bf16) when the configuration asks
 // for it, the tile holds one syndrome, and per syndrome the step is a GEMM

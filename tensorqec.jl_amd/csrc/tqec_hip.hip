@@ -1276,13 +1276,31 @@ int spec_code(const std::string &src, std::vector<char> &code) {
     return TQEC_OK;
 }
 
-bool spec_fits(const tqec_plan *p, const SpecCfg &c) { return spec_lds_bytes(p->cp, c) <= p->lds_limit; }
+bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
+    if (c.reg) {
+        if (c.fp64 || c.mfma || c.slab || c.threads % 64 != 0 || (1 << c.glog) * 4 != c.threads) return false;
+        if (!spec_reg_supported(p->cp)) return false;
+    }
+    return spec_lds_bytes(p->cp, c) <= p->lds_limit;
+}
 
-// Default before tuning: the largest tile whose LDS image (tables
-// included when they fit) lets three workgroups share a CU (else two, else
-// one), 16 threads per syndrome within [64, 128], blocks of 4 outputs per
-// lane.  On the d=7 code-capacity plans this is what the autotuner picks.
+// Default before tuning: the register-resident FP32 kernel when the plan's
+// live tensors fit a quad's registers (256 threads, tables in LDS, two waves
+// per SIMD when the live set leaves room); else the largest tile whose LDS
+// image (tables included when they fit) lets three workgroups share a CU
+// (else two, else one), 16 threads per syndrome within [64, 128], blocks of
+// 4 outputs per lane.
 SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
+    if (!fp64 && spec_reg_supported(p->cp)) {
+        // every tensor of a syndrome fits a quad's registers: no LDS intermediates
+        SpecCfg c;
+        c.reg = true;
+        c.threads = 256;
+        c.glog = 6;
+        c.tab_lds = true;
+        c.waves = spec_reg_peak(p->cp) <= 200 ? 2 : 0;
+        if (spec_fits(p, c)) return c;
+    }
     for (int div : {3, 2, 1}) {
         for (int gl = 6; gl >= 0; --gl) {
             for (bool tab : {true, false}) {
@@ -1314,7 +1332,7 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
 
 int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
-                                     (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.unroll << 2));
+                                     (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.reg ? 4 : 0) | (cfg.unroll << 3));
     auto it = p->spec.find(key);
     if (it != p->spec.end()) {
         *out = &it->second;
@@ -1323,7 +1341,9 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     if (cfg.glog < 0 || !spec_fits(p, cfg)) return set_error(TQEC_EUNSUPPORTED, "specialised tile does not fit in LDS");
     std::vector<char> code;
     int rc;
-    if ((rc = spec_code(spec_source(p->cp, cfg), code))) return rc;
+    const std::string src = spec_source(p->cp, cfg);
+    if (src.empty()) return set_error(TQEC_EUNSUPPORTED, "plan does not suit this specialised kernel");
+    if ((rc = spec_code(src, code))) return rc;
     SpecKernel k;
     k.cfg = cfg;
     k.lds = spec_lds_bytes(p->cp, cfg);
@@ -1334,6 +1354,13 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
         return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
     }
     k.blocks = p->n_cu * resident_per_cu(k.lds, cfg.threads, p->lds_limit);
+    if (cfg.reg) {
+        // register-bound: the runtime's occupancy of the compiled kernel
+        int per_cu = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, cfg.threads, 0) == hipSuccess &&
+            per_cu > 0)
+            k.blocks = p->n_cu * per_cu;
+    }
     p->spec[key] = k;
     *out = &p->spec[key];
     return TQEC_OK;
@@ -1375,12 +1402,13 @@ int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const
     return TQEC_OK;
 }
 
-// FP64 re-check kernel: one syndrome per 64-thread workgroup (the list is
-// short, so latency, not throughput, matters), in a slab when too wide for LDS.
+// FP64 re-check kernel: one syndrome per 256-thread workgroup (the list is
+// short, so the latency of one syndrome's step chain matters: one output
+// per thread on the widest steps), in a slab when too wide for LDS.
 int spec_recheck_kernel(tqec_plan *p, int dev, SpecKernel **out) {
     SpecCfg c;
     c.glog = 0;
-    c.threads = 64;
+    c.threads = 256;
     c.tab_lds = false;
     c.fp64 = true;
     if (!spec_fits(p, c)) {
@@ -1482,8 +1510,11 @@ void spec_precompile(const tqec_plan *p, const std::vector<SpecCfg> &cands) {
 }
 
 // Autotuner of the specialised kernel, in two stages: (tile 1..64, 64/128/256
-// threads) with tables in global memory, then the best pair with tables in
-// LDS, a 4-waves-per-SIMD register cap and 1/2/4/8 outputs per lane block.  Each configuration is timed
+// threads) with tables in global memory, and the register-resident kernel at
+// 64/128/256 threads with tables in LDS; then, for an LDS tile, the best pair
+// with tables in LDS, a 4-waves-per-SIMD register cap and 1/2/4/8 outputs per
+// lane block, or, for the register-resident kernel, tables in global memory
+// and a 2-waves-per-SIMD register cap.  Each configuration is timed
 // twice on the trial batch and the fastest is kept.
 int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st, int32_t *tile, int32_t *threads,
                   bool bf16) {
@@ -1544,6 +1575,15 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
             c.mfma = bf16;
             if (spec_fits(p, c)) stage1.push_back(c);
         }
+    if (!bf16)
+        for (int thr : {64, 128, 256}) {
+            SpecCfg c;
+            c.reg = true;
+            c.threads = thr;
+            c.glog = thr == 64 ? 4 : (thr == 128 ? 5 : 6);
+            c.tab_lds = true;
+            if (spec_fits(p, c)) stage1.push_back(c);
+        }
     if (stage1.empty()) {
         // slab configurations (per-workgroup intermediates of at most 8 MiB)
         for (int gl = 0; gl <= (bf16 ? 0 : 4); ++gl)
@@ -1562,17 +1602,29 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
     if (!rc) time_all(stage1);
     if (!rc && best.glog >= 0) {
         std::vector<SpecCfg> stage2;
-        for (bool tab : {false, true})
-            for (int wv : {0, 4})
-                for (int un : {1, 2, 4, 8}) {
-                    if (!tab && wv == 0 && un == 1) continue;
-                    if (tab && best.slab) continue;
+        if (best.reg) {
+            // register-resident: table placement and a two-waves-per-SIMD register cap
+            for (bool tab : {false, true})
+                for (int wv : {0, 2}) {
+                    if (tab && wv == 0) continue;
                     SpecCfg c = best;
                     c.tab_lds = tab;
                     c.waves = wv;
-                    c.unroll = un;
                     if (spec_fits(p, c)) stage2.push_back(c);
                 }
+        } else {
+            for (bool tab : {false, true})
+                for (int wv : {0, 4})
+                    for (int un : {1, 2, 4, 8}) {
+                        if (!tab && wv == 0 && un == 1) continue;
+                        if (tab && best.slab) continue;
+                        SpecCfg c = best;
+                        c.tab_lds = tab;
+                        c.waves = wv;
+                        c.unroll = un;
+                        if (spec_fits(p, c)) stage2.push_back(c);
+                    }
+        }
         time_all(stage2);
     }
     p->timing = saved_timing;
@@ -1744,7 +1796,8 @@ void mixed_drop_graphs(tqec_mixed *m) {
 }
 
 // Per-plan tiles of the generated mixed kernel: each plan's current SPEC_F32
-// configuration (or the default) with the kernel's thread count and no
+// LDS-tile configuration (or the default: a register-resident configuration
+// does not apply here) with the kernel's thread count and no
 // tables in LDS, its tile shrunk until it fits half the LDS (else a slab),
 // and an FP64 tile of one syndrome for the re-checks.
 int mixed_spec_layout(const std::vector<tqec_plan *> &plans, int nt, size_t lds_limit, std::vector<MixedSpecPlan> &sp,
@@ -1759,7 +1812,7 @@ int mixed_spec_layout(const std::vector<tqec_plan *> &plans, int nt, size_t lds_
         if (p->cp.semiring != TQEC_SEMIRING_SUM)
             return set_error(TQEC_EINVAL, "plan " + std::to_string(i) + " is a max-product plan");
         SpecCfg c;
-        if (p->spec_cfg_set && p->spec_cfg.glog >= 0) {
+        if (p->spec_cfg_set && p->spec_cfg.glog >= 0 && !p->spec_cfg.reg) {
             c = p->spec_cfg;
         } else {
             // the default tile of the largest size whose LDS image fits the budget
@@ -2520,13 +2573,20 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     c.fp64 = (flags & 2) != 0;
     c.slab = (flags & 4) != 0;
     c.mfma = (flags & 8) != 0;
+    c.reg = (flags & 16) != 0;
     c.unroll = std::max(1, (flags >> 8) & 0xff);
+    if (c.reg && (c.fp64 || c.mfma || c.slab || tile * 4 != threads))
+        return set_error(TQEC_EINVAL, "register-resident kernels are FP32 without slab or MFMA, tile = threads / 4");
     std::string src;
     try {
         src = spec_source(p->cp, c);
     } catch (const std::exception &e) {
         return set_error(TQEC_ENOMEM, e.what());
     }
+    std::string why;
+    if (src.empty() && c.reg && !spec_reg_supported(p->cp, &why))
+        return set_error(TQEC_EUNSUPPORTED, "plan does not suit the register-resident kernel: " + why);
+    if (src.empty()) return set_error(TQEC_EUNSUPPORTED, "plan does not suit this specialised kernel");
     if (len) *len = src.size();
     if (buf && cap > 0) {
         const size_t m = std::min(cap - 1, src.size());
@@ -2558,8 +2618,14 @@ int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t flag
     c.tab_lds = (flags & 1) != 0;
     c.slab = (flags & 4) != 0;
     c.mfma = (flags & 8) != 0;
+    c.reg = (flags & 16) != 0;
     c.waves = waves;
     c.unroll = unroll ? unroll : 1;
+    if (c.reg && (c.mfma || c.slab || tile * 4 != threads))
+        return set_error(TQEC_EINVAL, "register-resident kernels are FP32 without slab or MFMA, tile = threads / 4");
+    std::string why;
+    if (c.reg && !spec_reg_supported(p->cp, &why))
+        return set_error(TQEC_EUNSUPPORTED, "plan does not suit the register-resident kernel: " + why);
     if (!spec_fits(p, c)) return set_error(TQEC_EUNSUPPORTED, "tile does not fit in LDS");
     if (c.mfma) {
         if (c.glog != 0) return set_error(TQEC_EINVAL, "MFMA steps need tiles of one syndrome");
@@ -2584,7 +2650,7 @@ int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t
     if (p->spec_cfg.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the specialised kernel");
     if (tile) *tile = 1 << p->spec_cfg.glog;
     if (threads) *threads = p->spec_cfg.threads;
-    if (flags) *flags = (p->spec_cfg.tab_lds ? 1 : 0) | (p->spec_cfg.slab ? 4 : 0);
+    if (flags) *flags = (p->spec_cfg.tab_lds ? 1 : 0) | (p->spec_cfg.slab ? 4 : 0) | (p->spec_cfg.reg ? 16 : 0);
     if (waves) *waves = p->spec_cfg.waves | (p->spec_cfg.unroll << 8);
     return TQEC_OK;
 }

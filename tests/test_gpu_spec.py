@@ -75,6 +75,63 @@ def test_spec_bit_identical_to_generic(name, code, p, n):
             assert np.array_equal(a, b), (name, cfg)
 
 
+@pytest.mark.parametrize("name,code,p,n", [("surface3", SurfaceCode(3, 3), 0.05, 777),
+                                           ("steane", SteaneCode(), 0.08, 300), ("shor", ShorCode(), 0.03, 129),
+                                           ("surface5", SurfaceCode(5, 5), 0.06, 4097),
+                                           ("surface7", SurfaceCode(7, 7), 0.05, 20001),
+                                           ("toric3", ToricCode(3, 3), 0.07, 65)])
+def test_spec_reg_bit_identical_to_generic(name, code, p, n):
+    """Register-resident kernel (one syndrome per quad of lanes, DPP
+    exchanges): same operations in the same order as the generic FP32
+    kernel, so bit-identical, for every thread count, table placement and
+    register cap; also through the list entry point."""
+    torch = pytest.importorskip("torch")
+    say(f"case spec reg {name} n={n}")
+    t = CSSTannerGraph.from_code(code)
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="fused_f32"), t, em)
+    S = css_syndrome_rows(sample_depolarizing(em, seed=777 + t.nq, count=n), t)
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to(dev)
+    try:
+        ref = run(torch, ct, dS, n, "fused_f32")
+    except L.TqecError as e:
+        assert e.code == L.TQEC_EUNSUPPORTED
+        pytest.skip(f"{name}: plan does not fit the fused executors")
+    done = 0
+    for thr, tab, wv in ((64, False, 0), (128, True, 0), (256, True, 0), (256, False, 2)):
+        try:
+            ct.hip.set_spec(thr // 4, thr, tab, wv, reg=True)
+        except L.TqecError as e:
+            assert e.code == L.TQEC_EUNSUPPORTED
+            continue
+        assert ct.hip.spec_config()["reg"]
+        say(f"  threads {thr} tables_in_lds {tab} waves {wv}")
+        got = run(torch, ct, dS, n, "spec_f32")
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (name, thr, tab, wv)
+        done += 1
+    if done == 0:
+        pytest.skip(f"{name}: too many live entries per lane for registers")
+    say("  list entry point")
+    rows = np.arange(1, n, 3, dtype=np.int32)
+    dl = torch.from_numpy(rows).to(dev)
+    dc = torch.tensor([len(rows)], dtype=torch.int32, device=dev)
+    nout = 1 << ct.hip.n_out
+    mar = torch.zeros((n, nout), dtype=torch.float64, device=dev)
+    cls = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    tie = torch.zeros(n, dtype=torch.uint8, device=dev)
+    ct.hip.decode_list_device(dS.data_ptr(), dl.data_ptr(), dc.data_ptr(), len(rows), "spec_f32", mar.data_ptr(),
+                              cls.data_ptr(), tie.data_ptr())
+    torch.cuda.synchronize()
+    c = cls.cpu().numpy()
+    assert np.array_equal(c[rows], ref[1][rows])
+    untouched = np.ones(n, bool)
+    untouched[rows] = False
+    assert np.all(c[untouched] == -1)
+    assert np.array_equal(mar.cpu().numpy()[rows], ref[0][rows])
+
+
 def test_spec_list_decode_and_autotune():
     torch = pytest.importorskip("torch")
     t = CSSTannerGraph.from_code(SurfaceCode(5, 5))

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from tensorqec import lib as L
-from tensorqec.codes import CSSTannerGraph, SurfaceCode
+from tensorqec.codes import CSSTannerGraph, SurfaceCode, ToricCode
 from tensorqec.errors import iid_error
 from tensorqec import decoders as D
 
@@ -182,6 +182,44 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
     assert m and int(m.group(1)) == 0
     with pytest.raises(L.TqecError):
         ct.hip.spec_source(3, 64, True)          # not a power of two
+
+
+@pytest.mark.parametrize("threads,tab", [(64, False), (256, True)])
+def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, tab):
+    """The register-resident variant (one syndrome per quad of lanes, every
+    tensor in registers, DPP quad exchanges) compiles for gfx950 without
+    scratch on the d=5 plan; configurations it cannot take are refused."""
+    import shutil
+    import subprocess
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not available")
+    t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
+    p = 0.05 / 3
+    ct = D.compile(D.TNMMAP(mode="fused_f32"), t, iid_error(p, p, p, t.nq))
+    src = ct.hip.spec_source(threads // 4, threads, tab, reg=True)
+    assert "tqec_spec_f32" in src and f"constexpr int G = {threads // 4}," in src
+    assert "__builtin_amdgcn_mov_dpp" in src
+    assert ("__syncthreads" in src) == tab          # only the table staging synchronises
+    f = tmp_path / "reg.hip"
+    f.write_text("#include <hip/hip_runtime.h>\n" + src)
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
+                        str(f), "-o", str(tmp_path / "reg.o"), "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
+    assert m and int(m.group(1)) == 0
+    with pytest.raises(L.TqecError):
+        ct.hip.spec_source(threads // 2, threads, tab, reg=True)   # tile must be threads / 4
+    with pytest.raises(L.TqecError):
+        ct.hip.spec_source(threads // 4, threads, tab, fp64=True, reg=True)
+    # a plan whose live tensors exceed the register file is refused
+    t3 = CSSTannerGraph.from_code(ToricCode(3, 3))
+    ct3 = D.compile(D.TNMMAP(mode="fused_f32"), t3, iid_error(p, p, p, t3.nq))
+    with pytest.raises(L.TqecError) as e:
+        ct3.hip.spec_source(16, 64, True, reg=True)
+    assert e.value.code == L.TQEC_EUNSUPPORTED
 
 
 def test_mixed_kernel_source_compiles_for_gfx950(tmp_path):
