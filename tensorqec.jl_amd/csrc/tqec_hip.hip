@@ -1285,8 +1285,8 @@ bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
 }
 
 // Default before tuning: the register-resident FP32 kernel when the plan's
-// live tensors fit a quad's registers (256 threads, tables in LDS, two waves
-// per SIMD when the live set leaves room); else the largest tile whose LDS
+// live tensors fit a quad's registers (256 threads, tables in global memory,
+// two waves per SIMD when the live set leaves room); else the largest tile whose LDS
 // image (tables included when they fit) lets three workgroups share a CU
 // (else two, else one), 16 threads per syndrome within [64, 128], blocks of
 // 4 outputs per lane.
@@ -1297,7 +1297,7 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
         c.reg = true;
         c.threads = 256;
         c.glog = 6;
-        c.tab_lds = true;
+        c.tab_lds = false;
         c.waves = spec_reg_peak(p->cp) <= 200 ? 2 : 0;
         if (spec_fits(p, c)) return c;
     }

@@ -132,6 +132,35 @@ def test_spec_reg_bit_identical_to_generic(name, code, p, n):
     assert np.array_equal(mar.cpu().numpy()[rows], ref[0][rows])
 
 
+@pytest.mark.parametrize("d,tab,n", [(5, 5, 3001), (5, 6, 3001), (7, 6, 8192)])
+def test_deep_tabulation_matches_plain_f64(d, tab, n):
+    """Subtrees on up to 6 syndrome bits tabulated at plan creation (64
+    variants each): the register-resident kernel stays bit-identical to the
+    generic FP32 kernel, and both agree with the FP64 reference-network
+    executor (classes on every non-tie syndrome, marginals to 2e-5 of the row
+    maximum)."""
+    torch = pytest.importorskip("torch")
+    say(f"case tabulate_bits={tab} d={d} n={n}")
+    t = CSSTannerGraph.from_code(SurfaceCode(d, d))
+    p = 0.05
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.TreeSA(seed=14), tabulate_bits=tab), t, em)
+    assert ct.hip.stats()["tabulate_bits"] == tab
+    plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
+    S = css_syndrome_rows(sample_depolarizing(em, seed=606 + d + tab, count=n), t)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+    ref = run(torch, ct, dS, n, "fused_f32")
+    ct.hip.set_spec(64, 256, False, 2, reg=True)
+    got = run(torch, ct, dS, n, "spec_f32")
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    m64, c64, t64 = run(torch, plain, dS, n, "plain_f64")
+    ok = t64 == 0
+    assert np.array_equal(got[1][ok], c64[ok])
+    err = np.abs(got[0] - m64) / m64.max(axis=1, keepdims=True)
+    assert err.max() < 2e-5, err.max()
+
+
 def test_spec_list_decode_and_autotune():
     torch = pytest.importorskip("torch")
     t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
