@@ -52,6 +52,8 @@ struct CompiledPlan {
     int64_t root_off = 0;
     int root_slot = 0;
 
+    std::vector<int> reg_lay;                   // register-resident kernel: two lane bits per step (or empty)
+
     // ---- statistics ------------------------------------------------------
     double tc = 0, rw = 0, dep_tc = 0, dep_rw = 0;
     int sc = 0;

@@ -7,6 +7,7 @@
 #include "spec_codegen.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cctype>
 #include <sstream>
@@ -451,6 +452,163 @@ struct RegPlan {
     std::string err;
 };
 
+// Live tensor registers per lane in execution order; false above the budget.
+bool reg_peak_ok(const CompiledPlan &cp, RegPlan &rp) {
+    int live = 0;
+    rp.peak = 0;
+    for (int s = 0; s < (int)cp.steps.size(); ++s) {
+        const StepDesc &st = cp.steps[s];
+        live += 1 << (st.rc - lane_bits(rp.lay[s]));
+        rp.peak = std::max(rp.peak, live);
+        for (int side = 0; side < 2; ++side) {
+            const int p = rp.prod[side][s];
+            if (p >= 0) live -= 1 << (cp.steps[p].rc - lane_bits(rp.lay[p]));
+        }
+    }
+    if (rp.peak > kRegBudget) return rp.err = "more live tensor entries per lane than registers", false;
+    return true;
+}
+
+// quad_perm control: destination lane d of each quad reads lane src[d]
+int quad_ctrl(const int src[4]) { return src[0] | (src[1] << 2) | (src[2] << 4) | (src[3] << 6); }
+
+constexpr int kIdentityCtrl = 0xE4;   // quad_perm [0, 1, 2, 3]
+
+// DPP control with which a consumer writing layout LC reads entry `idx`
+// (lane-independent bits) of operand `op` stored in layout LX; -1 when an
+// output lane bit that feeds the operand does not sit in the same lane slot
+// of LX (the register would then depend on the lane).
+int fetch_ctrl(const OperandDesc &op, const RLay &LC, const RLay &LX, uint32_t idx) {
+    int img[2];
+    for (int t = 0; t < 2; ++t) {
+        img[t] = LC.pos[t] >= 0 ? seg_image(op, LC.pos[t]) : -1;
+        if (img[t] >= 0 && img[t] != LX.pos[t]) return -1;
+    }
+    int src[4];
+    for (int d = 0; d < 4; ++d) {
+        src[d] = 0;
+        for (int t = 0; t < 2; ++t) {
+            const int bit = (LX.pos[t] < 0 || img[t] >= 0) ? (d >> t) & 1 : (int)((idx >> LX.pos[t]) & 1u);
+            src[d] |= bit << t;
+        }
+    }
+    return quad_ctrl(src);
+}
+
+// Per-lane instruction estimate of one operand's reads in a step writing
+// layout LC: distinct DPP moves (intermediate stored in *LX) or distinct
+// table loads (LX null), summed over the contraction index as the emitter
+// caches them.  -1 when LX is not a legal layout for this consumer.
+long fetch_cost(const StepDesc &st, int side, const RLay &LC, const RLay *LX) {
+    const OperandDesc &op = side ? st.b : st.a;
+    if (LX)
+        for (int t = 0; t < 2; ++t)
+            if (LC.pos[t] >= 0) {
+                const int img = seg_image(op, LC.pos[t]);
+                if (img >= 0 && img != LX->pos[t]) return -1;
+            }
+    const std::vector<int> RB = reg_bits(LC, st.rc);
+    const std::vector<int> RX = LX ? reg_bits(*LX, op.rank) : std::vector<int>();
+    const uint32_t nreg = 1u << RB.size(), nkk = 1u << st.nk;
+    const uint32_t nq = (LX && op.nsingle) ? (1u << op.nsingle) : 1u;
+    long cost = 0;
+    std::vector<uint64_t> keys;
+    for (uint32_t k = 0; k < nkk; ++k) {
+        keys.clear();
+        for (uint32_t r = 0; r < nreg; ++r) {
+            const uint32_t base = gather_host(pdep(r, RB), op) | (k << op.kshift);
+            if (!LX) {
+                keys.push_back(base);
+                continue;
+            }
+            for (uint32_t q = 0; q < nq; ++q) {
+                const uint32_t idx = base | (q << op.sshift);
+                const int ctrl = fetch_ctrl(op, LC, *LX, idx);
+                if (ctrl != kIdentityCtrl) keys.push_back(((uint64_t)pext(idx, RX) << 8) | (uint64_t)ctrl);
+            }
+        }
+        std::sort(keys.begin(), keys.end());
+        cost += (long)(std::unique(keys.begin(), keys.end()) - keys.begin());
+    }
+    return cost;
+}
+
+std::vector<RLay> layout_candidates(int rank) {
+    std::vector<RLay> v;
+    for (int a = -1; a < rank; ++a)
+        for (int b = -1; b < rank; ++b) {
+            if (a >= 0 && a == b) continue;
+            RLay L;
+            L.pos[0] = a;
+            L.pos[1] = b;
+            v.push_back(L);
+        }
+    return v;
+}
+
+// Lane layouts minimising the per-lane instruction estimate of the whole
+// step program: for every step and output layout, its multiply-adds, scale
+// and maximum per output register and table loads, plus for each
+// intermediate operand the cheapest legal layout of it (its DPP moves here
+// plus the best cost of its own subtree in that layout).  Exact over the
+// tree (each tensor has one consumer); replication (a missing lane bit) is
+// one of the options.  Only for tensors of rank <= 9.
+bool reg_layout_dp(const CompiledPlan &cp, RegPlan &rp) {
+    const int ns = (int)cp.steps.size();
+    for (const StepDesc &st : cp.steps)
+        if (st.rc > 9) return false;
+    std::vector<std::vector<RLay>> cand(ns);
+    std::vector<std::vector<long>> best(ns);
+    std::vector<std::vector<std::array<int, 2>>> arg(ns);
+    for (int s = 0; s < ns; ++s) {
+        const StepDesc &st = cp.steps[s];
+        cand[s] = layout_candidates(st.rc);
+        best[s].assign(cand[s].size(), -1);
+        arg[s].assign(cand[s].size(), {-1, -1});
+        const long nkk = 1l << st.nk;
+        for (size_t i = 0; i < cand[s].size(); ++i) {
+            const RLay &L = cand[s][i];
+            long total = (1l << (st.rc - lane_bits(L))) * (nkk + 2);
+            bool ok = true;
+            for (int side = 0; side < 2 && ok; ++side) {
+                const int p = rp.prod[side][s];
+                if (p < 0) {
+                    total += fetch_cost(st, side, L, nullptr);
+                    continue;
+                }
+                long bc = -1;
+                int bj = -1;
+                for (size_t j = 0; j < cand[p].size(); ++j) {
+                    if (best[p][j] < 0) continue;
+                    const long c = fetch_cost(st, side, L, &cand[p][j]);
+                    if (c < 0) continue;
+                    if (bc < 0 || c + best[p][j] < bc) {
+                        bc = c + best[p][j];
+                        bj = (int)j;
+                    }
+                }
+                if (bj < 0) ok = false;
+                total += bc;
+                arg[s][i][side] = bj;
+            }
+            if (ok) best[s][i] = total;
+        }
+    }
+    int bi = -1;
+    for (size_t i = 0; i < cand[rp.root].size(); ++i)
+        if (best[rp.root][i] >= 0 && (bi < 0 || best[rp.root][i] < best[rp.root][bi])) bi = (int)i;
+    if (bi < 0) return false;
+    std::vector<int> choice(ns, -1);
+    choice[rp.root] = bi;
+    for (int s = ns - 1; s >= 0; --s) {
+        if (choice[s] < 0) return false;
+        rp.lay[s] = cand[s][choice[s]];
+        for (int side = 0; side < 2; ++side)
+            if (rp.prod[side][s] >= 0) choice[rp.prod[side][s]] = arg[s][choice[s]][side];
+    }
+    return true;
+}
+
 bool reg_plan(const CompiledPlan &cp, RegPlan &rp) {
     const int ns = (int)cp.steps.size();
     if (ns == 0) return rp.err = "no per-syndrome steps", false;
@@ -481,7 +639,18 @@ bool reg_plan(const CompiledPlan &cp, RegPlan &rp) {
         return rp.err = "root step not found", false;
     for (int s = 0; s < ns; ++s)
         if (uses[s] != (s == rp.root ? 0 : 1)) return rp.err = "step output not used exactly once", false;
-    // layouts, consumers first
+    if (cp.reg_lay.size() == 2 * (size_t)ns) {
+        // prepared with the plan (spec_reg_prepare)
+        for (int s = 0; s < ns; ++s) {
+            rp.lay[s].pos[0] = cp.reg_lay[2 * s];
+            rp.lay[s].pos[1] = cp.reg_lay[2 * s + 1];
+        }
+        return reg_peak_ok(cp, rp);
+    }
+    if (reg_layout_dp(cp, rp) && reg_peak_ok(cp, rp)) return true;
+    rp.err.clear();
+    rp.peak = 0;
+    // greedy layouts, consumers first
     RLay &lr = rp.lay[rp.root];
     for (int t = 0; t < 2; ++t)
         if (cp.steps[rp.root].rc > t) lr.pos[t] = t;
@@ -515,47 +684,17 @@ bool reg_plan(const CompiledPlan &cp, RegPlan &rp) {
             rp.lay[p] = L;
         }
     }
-    // live registers per lane in execution order
-    int live = 0;
-    for (int s = 0; s < ns; ++s) {
-        const StepDesc &st = cp.steps[s];
-        live += 1 << (st.rc - lane_bits(rp.lay[s]));
-        rp.peak = std::max(rp.peak, live);
-        for (int side = 0; side < 2; ++side) {
-            const int p = rp.prod[side][s];
-            if (p >= 0) live -= 1 << (cp.steps[p].rc - lane_bits(rp.lay[p]));
-        }
-    }
-    if (rp.peak > kRegBudget) return rp.err = "more live tensor entries per lane than registers", false;
-    return true;
+    return reg_peak_ok(cp, rp);
 }
-
-// quad_perm control: destination lane d of each quad reads lane src[d]
-int quad_ctrl(const int src[4]) { return src[0] | (src[1] << 2) | (src[2] << 4) | (src[3] << 6); }
 
 // Expression for entry `idx` (lane-independent bits) of intermediate X, made
 // by step p with layout LX, read by a consumer with output layout LC.
 bool reg_fetch(const OperandDesc &op, const RLay &LC, const RLay &LX, int p, uint32_t idx, std::string &out,
                std::string &err) {
-    int img[2];
-    for (int t = 0; t < 2; ++t) img[t] = LC.pos[t] >= 0 ? seg_image(op, LC.pos[t]) : -1;
-    for (int t = 0; t < 2; ++t)
-        if (img[t] >= 0 && img[t] != LX.pos[t]) return err = "lane bit of an operand moved", false;
-    int src[4];
-    for (int d = 0; d < 4; ++d) {
-        src[d] = 0;
-        for (int t = 0; t < 2; ++t) {
-            int bit;
-            if (LX.pos[t] < 0 || img[t] >= 0)
-                bit = (d >> t) & 1;
-            else
-                bit = (idx >> LX.pos[t]) & 1;
-            src[d] |= bit << t;
-        }
-    }
-    const int ctrl = quad_ctrl(src);
+    const int ctrl = fetch_ctrl(op, LC, LX, idx);
+    if (ctrl < 0) return err = "lane bit of an operand moved", false;
     const std::string name = "x" + std::to_string(p) + "_" + std::to_string(pext(idx, reg_bits(LX, op.rank)));
-    out = (ctrl == 0xE4) ? name : "QP(" + name + ", " + std::to_string(ctrl) + ")";
+    out = (ctrl == kIdentityCtrl) ? name : "QP(" + name + ", " + std::to_string(ctrl) + ")";
     return true;
 }
 
@@ -774,7 +913,7 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
         }
         const int ctrl = quad_ctrl(src);
         const std::string nm = "x" + std::to_string(rp.root) + "_" + std::to_string(pext((uint32_t)j, RR));
-        o << "    rv[" << j << "] = " << (ctrl == 0xE4 ? nm : "QP(" + nm + ", " + std::to_string(ctrl) + ")") << ";\n";
+        o << "    rv[" << j << "] = " << (ctrl == kIdentityCtrl ? nm : "QP(" + nm + ", " + std::to_string(ctrl) + ")") << ";\n";
     }
     o << "    if (q == 0 && valid) {\n"
          "      const int e = e" << rp.root << ";\n"
@@ -820,6 +959,16 @@ bool spec_reg_supported(const CompiledPlan &cp, std::string *why) {
     const bool ok = reg_plan(cp, rp);
     if (!ok && why) *why = rp.err;
     return ok;
+}
+
+void spec_reg_prepare(CompiledPlan &cp) {
+    cp.reg_lay.clear();
+    RegPlan rp;
+    if (!reg_plan(cp, rp)) return;
+    for (const RLay &L : rp.lay) {
+        cp.reg_lay.push_back(L.pos[0]);
+        cp.reg_lay.push_back(L.pos[1]);
+    }
 }
 
 int spec_reg_peak(const CompiledPlan &cp) {

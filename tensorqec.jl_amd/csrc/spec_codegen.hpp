@@ -45,6 +45,11 @@ bool spec_reg_supported(const CompiledPlan &cp, std::string *why = nullptr);
 // Peak live tensor registers per lane of the register-resident variant.
 int spec_reg_peak(const CompiledPlan &cp);
 
+// Choose the register-resident variant's lane layouts once (a tree search
+// over every tensor's lane bits) and keep them in cp.reg_lay; later calls
+// for this plan reuse them.
+void spec_reg_prepare(CompiledPlan &cp);
+
 // A step runs on MFMA (v_mfma_f32_32x32x16_bf16) when the configuration asks
 // for it, the tile holds one syndrome, and per syndrome the step is a GEMM
 // with M, N >= 32 (output bits feeding only A / only B) and K >= 16

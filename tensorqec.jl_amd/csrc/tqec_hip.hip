@@ -1885,6 +1885,7 @@ int tqec_plan_create(const tqec_plan_desc *desc, tqec_plan **out) {
     int rc;
     try {
         rc = compile_plan(desc, p->cp, err);
+        if (rc == TQEC_OK && p->cp.semiring == TQEC_SEMIRING_SUM) spec_reg_prepare(p->cp);
     } catch (const std::bad_alloc &) {
         return set_error(TQEC_ENOMEM, "out of host memory while compiling the plan");
     } catch (const std::exception &e) {
