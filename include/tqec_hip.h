@@ -106,7 +106,7 @@ typedef struct tqec_plan_desc {
     const int32_t *sliced_bit;      /* [n_sliced] syndrome bit              */
     int32_t tabulate_bits;          /* 0: only syndrome-independent subtrees
                                        are contracted at plan creation;
-                                       1..6: subtrees that depend on at most
+                                       1..8: subtrees that depend on at most
                                        this many syndrome bits are also
                                        contracted there, once per bit
                                        pattern, and looked up per syndrome */

@@ -16,7 +16,7 @@
 namespace tqec {
 
 constexpr int kMaxSeg = 12;   // bit-field moves per operand
-constexpr int kMaxEv = 6;     // syndrome bits selecting a sliced-leaf variant
+constexpr int kMaxEv = 8;     // syndrome bits selecting a sliced-leaf variant
 constexpr int kMaxRank = 24;  // widest per-syndrome tensor
 constexpr int kMaxOut = 16;   // result rank
 

@@ -1510,11 +1510,11 @@ void spec_precompile(const tqec_plan *p, const std::vector<SpecCfg> &cands) {
 }
 
 // Autotuner of the specialised kernel, in two stages: (tile 1..64, 64/128/256
-// threads) with tables in global memory, and the register-resident kernel at
-// 64/128/256 threads with tables in LDS; then, for an LDS tile, the best pair
-// with tables in LDS, a 4-waves-per-SIMD register cap and 1/2/4/8 outputs per
-// lane block, or, for the register-resident kernel, tables in global memory
-// and a 2-waves-per-SIMD register cap.  Each configuration is timed
+// threads) and the register-resident kernel at 64/128/256 threads, all with
+// tables in global memory; then, for an LDS tile, the best pair with tables
+// in LDS, a 4-waves-per-SIMD register cap and 1/2/4/8 outputs per lane block,
+// or, for the register-resident kernel, tables in LDS (when they fit) and a
+// 2-waves-per-SIMD register cap.  Each configuration is timed
 // twice on the trial batch and the fastest is kept.
 int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st, int32_t *tile, int32_t *threads,
                   bool bf16) {
@@ -1581,7 +1581,7 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
             c.reg = true;
             c.threads = thr;
             c.glog = thr == 64 ? 4 : (thr == 128 ? 5 : 6);
-            c.tab_lds = true;
+            c.tab_lds = false;
             if (spec_fits(p, c)) stage1.push_back(c);
         }
     if (stage1.empty()) {
@@ -1606,7 +1606,7 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
             // register-resident: table placement and a two-waves-per-SIMD register cap
             for (bool tab : {false, true})
                 for (int wv : {0, 2}) {
-                    if (tab && wv == 0) continue;
+                    if (!tab && wv == 0) continue;
                     SpecCfg c = best;
                     c.tab_lds = tab;
                     c.waves = wv;
