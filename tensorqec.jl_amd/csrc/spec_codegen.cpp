@@ -870,9 +870,11 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
          "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
          "    const int *__restrict__ list_count, const float *__restrict__ gtab, double *__restrict__ mar,\n"
          "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
-         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, float *__restrict__ gslab) {\n";
+         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, float *__restrict__ gslab,\n"
+         "    int nseg, int segcap) {\n";
     o << "  constexpr int G = " << G << ", NBITS = " << std::max(1, cp.n_syn) << ", NOUT = " << nout << ";\n";
-    o << "  (void)gslab;\n";
+    o << "  (void)gslab; (void)nseg; (void)segcap;\n";
+    o << "  __shared__ int seg_n;\n";
     if (cfg.tab_lds) {
         o << "  __shared__ float tab[" << std::max<size_t>(1, tab_n) << "];\n";
         o << "  for (int i = threadIdx.x; i < " << tab_n << "; i += " << cfg.threads << ") tab[i] = gtab[i];\n";
@@ -880,10 +882,18 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
     } else {
         o << "  const float *tab = gtab;\n";
     }
+    // Re-check list in per-tile segments: tile t's flagged rows go to
+    // rl[t * rcap ..] and their number to rcount[t], written by every tile
+    // the launch may own (those beyond a listed count write 0), so the list
+    // needs no clearing between calls.
     o << "  const int q = threadIdx.x & 3;\n"
          "  const long long count = list ? (long long)(*list_count) : n;\n"
          "  const long long ntiles = (count + G - 1) / G;\n"
-         "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {\n"
+         "  const long long nseg_all = (n + G - 1) / G;\n"
+         "  for (long long t = blockIdx.x; t < nseg_all; t += gridDim.x) {\n"
+         "   if (threadIdx.x == 0) seg_n = 0;\n"
+         "   __syncthreads();\n"
+         "   if (t < ntiles) {\n"
          "    const long long qi = t * G + (threadIdx.x >> 2);\n"
          "    const bool valid = qi < count;\n"
          "    const long long b = valid ? (list ? (long long)list[qi] : qi) : 0ll;\n"
@@ -937,11 +947,14 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
          "        const bool rc = !(best > 0) || lowf != 0 ||\n"
          "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
          "        if (rc) {\n"
-         "          const int slot = atomicAdd(rcount, 1);\n"
-         "          if (slot < rcap) rl[slot] = (int)b;\n"
+         "          const int slot = atomicAdd(&seg_n, 1);\n"
+         "          if (slot < rcap) rl[t * rcap + slot] = (int)b;\n"
          "        }\n"
          "      }\n"
          "    }\n"
+         "   }\n"
+         "   __syncthreads();\n"
+         "   if (threadIdx.x == 0 && rcount) rcount[t] = min(seg_n, (int)rcap);\n"
          "  }\n"
          "}\n";
     return o.str();
@@ -1120,7 +1133,7 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
       << "    const int *__restrict__ list_count, const " << tn << " *__restrict__ gtab, double *__restrict__ mar,\n"
       << "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
       << "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, " << tn
-      << " *__restrict__ gslab) {\n";
+      << " *__restrict__ gslab, int nseg, int segcap) {\n";
     o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
     if (cfg.tab_lds) {
         o << "  " << tn << " *tab = reinterpret_cast<" << tn << " *>(smem + " << L.tab << ");\n";
@@ -1132,9 +1145,24 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
         o << "  " << tn << " *slabw = gslab + (unsigned long long)blockIdx.x * " << spec_slab_entries(cp, cfg) << "ull;\n";
     else
         o << "  " << tn << " *slabw = nullptr;\n";
+    o << "  unsigned iter = 0;\n";
+    // nseg > 0: the rows listed in per-tile segments (the register-resident
+    // kernel's re-check list): segment w holds min(list_count[w], segcap)
+    // rows at list[w * segcap ..]; each is decoded as row b0 of the syndrome
+    // array (any row: a listed call's rows may lie beyond its count)
+    o << "  if (nseg > 0) {\n"
+         "    for (long long w = blockIdx.x; w < nseg; w += gridDim.x) {\n"
+         "      const int c = min(list_count[w], segcap);\n"
+         "      for (int e = 0; e < c; ++e, ++iter) {\n"
+         "        const long long row = (long long)list[w * segcap + e];\n"
+         "        tile(syn, row + 1, nullptr, row, tab, mar, cls, ties, rl, rcount, rcap, tie_tol, recheck_tol, smem, slabw,"
+         " iter * " << nsteps << "u + 1u);\n"
+         "      }\n"
+         "    }\n"
+         "    return;\n"
+         "  }\n";
     o << "  const long long count = list ? (long long)(*list_count) : n;\n";
     o << "  const long long ntiles = (count + " << G - 1 << ") >> " << cfg.glog << ";\n";
-    o << "  unsigned iter = 0;\n";
     o << "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x, ++iter)\n";
     o << "    tile(syn, count, list, t << " << cfg.glog << ", tab, mar, cls, ties, rl, rcount, rcap, tie_tol, recheck_tol,"
          " smem, slabw, iter * " << nsteps << "u + 1u);\n";

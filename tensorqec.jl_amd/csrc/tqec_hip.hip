@@ -809,6 +809,9 @@ struct Workspace {
     int64_t list_cap = 0;
     int32_t *count = nullptr;
     int32_t *host_count = nullptr;
+    int32_t *segc = nullptr;        // register-resident kernel: re-check rows per tile segment
+    int64_t segc_cap = 0;
+    int64_t count_nseg = 0;         // > 0: the last call's re-check count is the sum of segc[0 .. count_nseg)
     uint8_t *d_syn = nullptr; size_t d_syn_bytes = 0;   // staging for the host API
     double *d_mar = nullptr; size_t d_mar_bytes = 0;
     int32_t *d_cls = nullptr; size_t d_cls_bytes = 0;
@@ -1016,6 +1019,9 @@ int record(tqec_plan *p, Workspace &w, int i, hipStream_t stream) {
     return TQEC_OK;
 }
 
+int ensure_count(Workspace &w, hipStream_t stream);
+int read_recheck_count(Workspace &w, int64_t *out);
+
 int run_plain(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, int64_t n, double *d_mar,
               int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
     const CompiledPlan &cp = p->cp;
@@ -1072,11 +1078,9 @@ int run_wide(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
         TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
         w.list_cap = n;
     }
-    if (!w.count) {
-        TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
-        TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
-    }
+    if ((rc = ensure_count(w, stream))) return rc;
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
+    w.count_nseg = 0;
     Pools<float> pools{dd->cpool32, dd->spool32};
     PlainScale sc{(uint32_t *)w.smx, (int32_t *)w.sex};
     const int nout = 1 << cp.n_out;
@@ -1204,11 +1208,34 @@ int prepare_recheck(Workspace &w, int64_t n, hipStream_t stream) {
         TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
         w.list_cap = n;
     }
-    if (!w.count) {
-        TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
-        TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
-    }
+    int rc;
+    if ((rc = ensure_count(w, stream))) return rc;
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, w.count, 1);
+    w.count_nseg = 0;
+    return TQEC_OK;
+}
+
+// The single re-check counter: allocated on first use.
+int ensure_count(Workspace &w, hipStream_t stream) {
+    (void)stream;
+    if (w.count) return TQEC_OK;
+    TQ_HIP(hipMalloc((void **)&w.count, sizeof(int32_t)));
+    TQ_HIP(hipHostMalloc((void **)&w.host_count, sizeof(int32_t)));
+    return TQEC_OK;
+}
+
+// The last call's re-check count (after the stream's work has finished).
+int read_recheck_count(Workspace &w, int64_t *out) {
+    if (w.count_nseg > 0) {
+        std::vector<int32_t> c((size_t)w.count_nseg);
+        TQ_HIP(hipMemcpy(c.data(), w.segc, c.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        int64_t t = 0;
+        for (int32_t x : c) t += x;
+        *out = t;
+        return TQEC_OK;
+    }
+    TQ_HIP(hipMemcpy(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *out = *w.host_count;
     return TQEC_OK;
 }
 
@@ -1377,8 +1404,9 @@ size_t spec_slab_need(const tqec_plan *p, const SpecKernel *k, int64_t n) {
 // Launch a specialised kernel over n syndromes (or the listed ones).
 int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n,
                 const int32_t *in_list, const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie,
-                int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream) {
-    const int64_t ntiles = (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
+                int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream,
+                int64_t nseg = 0, int32_t segcap = 0) {
+    const int64_t ntiles = nseg > 0 ? nseg : (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
     void *a_slab = nullptr;
     if (k->cfg.slab) {
@@ -1396,8 +1424,9 @@ int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const
     int *a_rl = rl, *a_rc = rcount;
     long long a_cap = (long long)rcap;
     double a_tt = tie_tol, a_rt = recheck_tol;
+    int a_nseg = (int)nseg, a_segcap = segcap;
     void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt,
-                    &a_slab};
+                    &a_slab, &a_nseg, &a_segcap};
     TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
     return TQEC_OK;
 }
@@ -1480,9 +1509,43 @@ int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
     if ((rc = grow((char **)&w.spec_slab, &w.spec_slab_bytes,
                    std::max(spec_slab_need(p, k, n), spec_slab_need(p, k64, n)))))
         return rc;
+    const double rtol = bf16 ? std::max(p->recheck_tol, kBf16RecheckTol) : p->recheck_tol;
+    if (cfg.reg) {
+        // Re-check list in per-tile segments, each written whole by the tile
+        // that owns it: no clearing launch before the call.
+        const int32_t G = 1 << cfg.glog;
+        const int64_t nseg = (n + G - 1) / G;
+        if (w.list_cap < nseg * G) {
+            if (w.list) TQ_HIP(hipFree(w.list));
+            w.list = nullptr;
+            w.list_cap = 0;
+            TQ_HIP(hipMalloc((void **)&w.list, (size_t)std::max<int64_t>(nseg * G, 1) * sizeof(int32_t)));
+            w.list_cap = nseg * G;
+        }
+        if (w.segc_cap < nseg) {
+            if (w.segc) TQ_HIP(hipFree(w.segc));
+            w.segc = nullptr;
+            w.segc_cap = 0;
+            TQ_HIP(hipMalloc((void **)&w.segc, (size_t)std::max<int64_t>(nseg, 1) * sizeof(int32_t)));
+            w.segc_cap = nseg;
+        }
+        if ((rc = ensure_count(w, stream))) return rc;
+        if ((rc = record(p, w, 0, stream))) return rc;
+        if ((rc = spec_launch(p, w, k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, w.list, w.segc, G,
+                              p->tie_tol, rtol, stream)))
+            return rc;
+        if ((rc = record(p, w, 1, stream))) return rc;
+        if ((rc = record(p, w, 2, stream))) return rc;
+        if ((rc = spec_launch(p, w, k64, dd, d_syn, n, w.list, w.segc, d_mar, d_cls, d_tie, nullptr, nullptr, 0,
+                              p->tie_tol, p->recheck_tol, stream, nseg, G)))
+            return rc;
+        if ((rc = record(p, w, 3, stream))) return rc;
+        w.count_nseg = nseg;
+        w.count_on_device = true;   // read on demand by tqec_last_recheck_count
+        return TQEC_OK;
+    }
     if ((rc = prepare_recheck(w, n, stream))) return rc;
     if ((rc = record(p, w, 0, stream))) return rc;
-    const double rtol = bf16 ? std::max(p->recheck_tol, kBf16RecheckTol) : p->recheck_tol;
     if ((rc = spec_launch(p, w, k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, w.list, w.count, w.list_cap,
                           p->tie_tol, rtol, stream)))
         return rc;
@@ -1906,6 +1969,7 @@ void tqec_plan_destroy(tqec_plan *p) {
         if (w.synT) (void)hipFree(w.synT);
         if (w.list) (void)hipFree(w.list);
         if (w.count) (void)hipFree(w.count);
+        if (w.segc) (void)hipFree(w.segc);
         if (w.host_count) (void)hipHostFree(w.host_count);
         if (w.d_syn) (void)hipFree(w.d_syn);
         if (w.d_mar) (void)hipFree(w.d_mar);
@@ -2012,8 +2076,9 @@ int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode,
     if (tie) TQ_HIP(hipMemcpyAsync(tie, w.d_tie, (size_t)n, hipMemcpyDeviceToHost, st));
     TQ_HIP(hipStreamSynchronize(st));
     if (wp && wp->count_on_device && wp->host_count) {
-        TQ_HIP(hipMemcpy(wp->host_count, wp->count, sizeof(int32_t), hipMemcpyDeviceToHost));
-        wp->last_recheck = *wp->host_count;
+        int64_t m;
+        if ((rc = read_recheck_count(*wp, &m))) return rc;
+        wp->last_recheck = m;
         wp->count_on_device = false;
     }
     return TQEC_OK;
@@ -2032,8 +2097,10 @@ int tqec_last_recheck_count(tqec_plan *p, void *stream, int64_t *count) {
     Workspace &w = it->second;
     if (w.count_on_device) {
         TQ_HIP(hipStreamSynchronize((hipStream_t)stream));
-        TQ_HIP(hipMemcpy(w.host_count, w.count, sizeof(int32_t), hipMemcpyDeviceToHost));
-        w.last_recheck = *w.host_count;
+        int64_t m;
+        int rc;
+        if ((rc = read_recheck_count(w, &m))) return rc;
+        w.last_recheck = m;
         w.count_on_device = false;
     }
     *count = w.last_recheck;

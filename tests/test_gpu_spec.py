@@ -98,6 +98,7 @@ def test_spec_reg_bit_identical_to_generic(name, code, p, n):
     except L.TqecError as e:
         assert e.code == L.TQEC_EUNSUPPORTED
         pytest.skip(f"{name}: plan does not fit the fused executors")
+    ref_rechecks = ct.hip.last_recheck_count()
     done = 0
     for thr, tab, wv in ((64, False, 0), (128, True, 0), (256, True, 0), (256, False, 2)):
         try:
@@ -110,6 +111,8 @@ def test_spec_reg_bit_identical_to_generic(name, code, p, n):
         got = run(torch, ct, dS, n, "spec_f32")
         for a, b in zip(got, ref):
             assert np.array_equal(a, b), (name, thr, tab, wv)
+        # the same syndromes were re-checked in FP64 (per-tile re-check segments)
+        assert ct.hip.last_recheck_count() == ref_rechecks, (name, thr, ref_rechecks)
         done += 1
     if done == 0:
         pytest.skip(f"{name}: too many live entries per lane for registers")
