@@ -74,10 +74,10 @@ def measured_traffic(mode, plan_cfg, batch):
         return None
     if plan_cfg.get("reg"):
         path = os.path.join(REPO, "profiles", "spec_f32_reg_d7_p005_counters.md")
-        if plan_cfg.get("tables_in_lds") or plan_cfg.get("tabulate_bits") != 6 or not os.path.exists(path):
+        if plan_cfg.get("tables_in_lds") or not os.path.exists(path):
             return None
-        m = re.search(r"seed=%d: fetch ([0-9.]+) MB \+ write ([0-9.]+) MB" % plan_cfg.get("seed", -1),
-                      open(path).read())
+        m = re.search(r"seed=%d tab=%d: fetch ([0-9.]+) MB \+ write ([0-9.]+) MB"
+                      % (plan_cfg.get("seed", -1), plan_cfg.get("tabulate_bits", -1)), open(path).read())
     else:
         path = os.path.join(REPO, "profiles", "spec_f32_d7_p005_counters.md")
         if (plan_cfg.get("seed"), plan_cfg.get("tabulate_bits"), plan_cfg.get("slab")) != (3, 3, False) or \
