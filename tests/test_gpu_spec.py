@@ -164,6 +164,59 @@ def test_deep_tabulation_matches_plain_f64(d, tab, n):
     assert err.max() < 2e-5, err.max()
 
 
+@pytest.mark.parametrize("reg", [True, False])
+def test_spec_call_replays_from_a_captured_graph(reg):
+    """A SPEC_F32 call captured into a HIP graph (torch.cuda.graph on a side
+    stream) replays correctly several times in a row, with eager work on
+    other batches between the replays: the register-resident kernel's
+    per-tile re-check segments need no clearing, and the LDS kernel's list
+    length is cleared by a kernel, not a memset node."""
+    torch = pytest.importorskip("torch")
+    t = CSSTannerGraph.from_code(SurfaceCode(7, 7))
+    p = 0.05
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="fused_f32"), t, em)
+    if reg:
+        ct.hip.set_spec(64, 256, False, 0, reg=True)
+    else:
+        ct.hip.set_spec(8, 128, True, 0)
+    n = 16384
+    dev = torch.device("cuda:0")
+    S = [torch.from_numpy(np.ascontiguousarray(css_syndrome_rows(sample_depolarizing(em, seed=90 + i, count=n), t)))
+         .to(dev) for i in range(2)]
+    s = torch.cuda.Stream()
+    mar = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    cls = torch.zeros(n, dtype=torch.int32, device=dev)
+    tie = torch.zeros(n, dtype=torch.uint8, device=dev)
+
+    def call(i):
+        ct.hip.decode_device(S[i].data_ptr(), n, "spec_f32", mar.data_ptr(), cls.data_ptr(), tie.data_ptr(),
+                             s.cuda_stream)
+
+    with torch.cuda.stream(s):
+        call(0)
+    torch.cuda.synchronize()
+    ref = (mar.cpu().numpy().copy(), cls.cpu().numpy().copy(), tie.cpu().numpy().copy())
+    ref_rechecks = ct.hip.last_recheck_count(s.cuda_stream)
+    assert ref_rechecks > 0            # the re-check list is exercised
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        call(0)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        say(f"case graph replay reg={reg} rep {rep}")
+        mar.zero_()
+        cls.fill_(-1)
+        g.replay()
+        torch.cuda.synchronize()
+        got = (mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy())
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (reg, rep)
+        with torch.cuda.stream(s):
+            call(1)                    # eager work on another batch between replays
+        torch.cuda.synchronize()
+
+
 def test_spec_list_decode_and_autotune():
     torch = pytest.importorskip("torch")
     t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
