@@ -762,6 +762,14 @@ def main():
         res["max_rel_marginal_err_fp32_vs_fp64"] = float(
             np.max(np.abs(mar_c - mar_p) / mar_p.max(axis=2, keepdims=True)))
         res["plain_ms_per_step"] = dt_p / args.steps * 1e3
+        if mode_c == "spec_f32":
+            # the exact FP64 specialised kernel on the chosen plan (register-resident when it fits)
+            dt_6, _, _, _, cls_6, tie_6, mar_6 = run(chosen, "spec_f64", "chosen spec_f64", warmup=20, instrument=False)
+            ok6 = (tie_p == 0) & (tie_6 == 0)
+            res["spec_f64_syndromes_per_s"] = args.steps * B / dt_6
+            res["spec_f64_class_agreement_with_plain_f64"] = float((cls_6[ok6] == cls_p[ok6]).mean())
+            res["spec_f64_max_rel_marginal_err_vs_plain_f64"] = float(
+                np.max(np.abs(mar_6 - mar_p) / mar_p.max(axis=2, keepdims=True)))
     st = chosen.hip.stats()
     traffic = measured_traffic(mode_c, plan_cfg, B)
     flops_per_syn = 2.0 * st["dep_tc"]

@@ -293,10 +293,13 @@ void tqec_mixed_destroy(tqec_mixed *m);
 #define TQEC_SPEC_MFMA 8           /* GEMM-shaped steps on BF16 MFMA (tile 1); with
                                       tqec_plan_set_spec this pins the SPEC_BF16
                                       configuration                                   */
-#define TQEC_SPEC_REG 16           /* FP32, one syndrome per quad of lanes with every
+#define TQEC_SPEC_REG 16           /* one syndrome per quad of lanes with every
                                       tensor in registers (no LDS intermediates, no
-                                      barriers); tile = threads / 4; results
-                                      bit-identical to the other FP32 kernels       */
+                                      barriers); tile = threads / 4; FP32 (default
+                                      SPEC_F32 kernel) or, with TQEC_SPEC_FP64 in
+                                      tqec_plan_spec_source, FP64 (default SPEC_F64
+                                      kernel); results bit-identical to the other
+                                      kernels of the same precision                 */
 
 /* Source of the plan-specialised kernel for a tile configuration
  * (inspection and offline compilation checks); `flags` as above, with the

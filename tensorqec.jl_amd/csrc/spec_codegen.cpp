@@ -698,20 +698,27 @@ bool reg_fetch(const OperandDesc &op, const RLay &LC, const RLay &LX, int p, uin
     return true;
 }
 
+// One step of the register-resident kernel.  FP32: per-tensor power-of-two
+// scaling (the generic FP32 kernel's); FP64 (`fp64`): no scaling, the
+// constant exponents folded out of table data restored per entry (the
+// generic FP64 kernel's).
 bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan &rp, int s, size_t const_n,
-                   std::string &err) {
+                   bool fp64, std::string &err) {
     const StepDesc &st = cp.steps[s];
     const RLay &LC = rp.lay[s];
     const std::vector<int> RB = reg_bits(LC, st.rc);
     const uint32_t nreg = 1u << RB.size();
     const uint32_t nkk = 1u << st.nk;
-    o << "    float m" << s << ";\n    int e" << s << ";\n    float";
+    const char *T = fp64 ? "double" : "float";
+    const char *zero = fp64 ? "0.0" : "0.f";
+    if (!fp64) o << "    float m" << s << ";\n    int e" << s << ";\n";
+    o << "    " << T;
     for (uint32_t r = 0; r < nreg; ++r) o << (r ? ", " : " ") << "x" << s << "_" << r;
     o << ";\n";
     o << "    { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << ", " << nreg
       << " registers per lane\n";
     const char *ab[2] = {"a", "b"};
-    for (int side = 0; side < 2; ++side) {
+    for (int side = 0; side < 2 && !fp64; ++side) {
         const OperandDesc &op = side ? st.b : st.a;
         const int p = rp.prod[side][s];
         if (op.kind == kInter)
@@ -720,8 +727,8 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
         else
             o << "      const int n" << ab[side] << " = 0, e" << ab[side] << " = " << op.cexp << ";\n";
     }
-    o << "      const int shift = -(na + nb);\n";
-    {
+    if (!fp64) o << "      const int shift = -(na + nb);\n";
+    if (!fp64) {
         std::string cond;
         if (st.a.kind == kInter) cond += "na < -100";
         if (st.b.kind == kInter) cond += std::string(cond.empty() ? "" : " || ") + "nb < -100";
@@ -732,11 +739,11 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
         const OperandDesc &op = side ? st.b : st.a;
         if (op.kind == kInter) continue;
         if (op.kind == kConst) {
-            o << "      const float *p" << ab[side] << " = tab + " << (long long)op.off << ";\n";
+            o << "      const " << T << " *p" << ab[side] << " = tab + " << (long long)op.off << ";\n";
         } else {
             o << "      const unsigned v" << ab[side] << " = 0u";
             for (int j = 0; j < op.nev; ++j) o << " | (s" << op.ev_bit[j] << " << " << j << ")";
-            o << ";\n      const float *p" << ab[side] << " = tab + " << (long long)(const_n + (size_t)op.off) << " + (v"
+            o << ";\n      const " << T << " *p" << ab[side] << " = tab + " << (long long)(const_n + (size_t)op.off) << " + (v"
               << ab[side] << " << " << op.rank << ");\n";
         }
         std::string lo;
@@ -754,8 +761,8 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
             has_lane_off[side] = true;
         }
     }
-    o << "      float";
-    for (uint32_t r = 0; r < nreg; ++r) o << (r ? ", " : " ") << "acc" << r << " = 0.f";
+    o << "      " << T;
+    for (uint32_t r = 0; r < nreg; ++r) o << (r ? ", " : " ") << "acc" << r << " = " << zero;
     o << ";\n";
     int ntmp = 0;
     for (uint32_t k = 0; k < nkk; ++k) {
@@ -792,7 +799,7 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
                             return true;
                         }
                     const std::string nm = "f" + std::to_string(ntmp++);
-                    o << "        float " << nm << " = 0.f;\n";
+                    o << "        " << T << " " << nm << " = " << zero << ";\n";
                     for (auto &t1 : terms) o << "        " << nm << " += " << t1 << ";\n";
                     cache.emplace_back(expr, nm);
                     out = nm;
@@ -805,7 +812,7 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
                     return true;
                 }
             const std::string nm = "f" + std::to_string(ntmp++);
-            o << "        const float " << nm << " = " << expr << ";\n";
+            o << "        const " << T << " " << nm << " = " << expr << ";\n";
             cache.emplace_back(expr, nm);
             out = nm;
             return true;
@@ -814,9 +821,21 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
             const uint32_t cst = pdep(r, RB);
             std::string va, vb;
             if (!value(0, cst, va) || !value(1, cst, vb)) return false;
-            o << "        acc" << r << " = fmaf(" << va << ", " << vb << ", acc" << r << ");\n";
+            o << "        acc" << r << " = " << (fp64 ? "fma(" : "fmaf(") << va << ", " << vb << ", acc" << r << ");\n";
         }
         o << "      }\n";
+    }
+    if (fp64) {
+        const int cexp = st.a.cexp + st.b.cexp;
+        for (uint32_t r = 0; r < nreg; ++r) {
+            o << "      x" << s << "_" << r << " = ";
+            if (cexp != 0)
+                o << "ldexp(acc" << r << ", " << cexp << ");\n";
+            else
+                o << "acc" << r << ";\n";
+        }
+        o << "    }\n";
+        return true;
     }
     // Entries are >= 0 or NaN, and NaN-ignoring fmaxf over them equals the
     // generic kernel's running `x > m ? x : m` from 0 (max is exact and order
@@ -861,26 +880,34 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
     const size_t tab_n = cp.const_pool.size() + cp.sliced_pool.size();
     const int G = cfg.threads / 4;
     const int nout = 1 << cp.n_out;
+    const bool fp64 = cfg.fp64;
+    const char *T = fp64 ? "double" : "float";
     std::ostringstream o;
-    o << "// plan-specialised FP32 TNMMAP kernel, one syndrome per quad of lanes, tensors in registers (generated)\n";
-    o << "#define QP(x, c) __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), (c), 0xf, 0xf, false))\n";
+    o << "// plan-specialised " << (fp64 ? "FP64" : "FP32")
+      << " TNMMAP kernel, one syndrome per quad of lanes, tensors in registers (generated)\n";
+    if (fp64)
+        o << "#define QP(x, c) __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), (c), 0xf, 0xf, false), "
+             "__builtin_amdgcn_mov_dpp(__double2loint(x), (c), 0xf, 0xf, false))\n";
+    else
+        o << "#define QP(x, c) __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), (c), 0xf, 0xf, false))\n";
     o << "extern \"C\" __global__ void __launch_bounds__(" << cfg.threads;
     if (cfg.waves > 0) o << ", " << cfg.waves;
-    o << ") tqec_spec_f32(\n"
+    o << ") " << spec_kernel_name(cfg) << "(\n"
          "    const unsigned char *__restrict__ syn, long long n, const int *__restrict__ list,\n"
-         "    const int *__restrict__ list_count, const float *__restrict__ gtab, double *__restrict__ mar,\n"
+         "    const int *__restrict__ list_count, const " << T << " *__restrict__ gtab, double *__restrict__ mar,\n"
          "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
-         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, float *__restrict__ gslab,\n"
+         "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, " << T
+      << " *__restrict__ gslab,\n"
          "    int nseg, int segcap) {\n";
     o << "  constexpr int G = " << G << ", NBITS = " << std::max(1, cp.n_syn) << ", NOUT = " << nout << ";\n";
     o << "  (void)gslab; (void)nseg; (void)segcap;\n";
     o << "  __shared__ int seg_n;\n";
     if (cfg.tab_lds) {
-        o << "  __shared__ float tab[" << std::max<size_t>(1, tab_n) << "];\n";
+        o << "  __shared__ " << T << " tab[" << std::max<size_t>(1, tab_n) << "];\n";
         o << "  for (int i = threadIdx.x; i < " << tab_n << "; i += " << cfg.threads << ") tab[i] = gtab[i];\n";
         o << "  __syncthreads();\n";
     } else {
-        o << "  const float *tab = gtab;\n";
+        o << "  const " << T << " *tab = gtab;\n";
     }
     // Re-check list in per-tile segments: tile t's flagged rows go to
     // rl[t * rcap ..] and their number to rcount[t], written by every tile
@@ -905,13 +932,13 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
                 for (int j = 0; j < op->nev; ++j) used[op->ev_bit[j]] = 1;
     for (int j = 0; j < cp.n_syn; ++j)
         if (used[j]) o << "    const unsigned s" << j << " = valid ? (unsigned)(row[" << j << "] & 1u) : 0u;\n";
-    o << "    int lowf = 0;\n";
+    if (!fp64) o << "    int lowf = 0;\n";
     for (int s = 0; s < (int)cp.steps.size(); ++s)
-        if (!emit_reg_step(o, cp, rp, s, const_n, err)) return "";
+        if (!emit_reg_step(o, cp, rp, s, const_n, fp64, err)) return "";
     // every lane gathers the root's entries; the quad's lane 0 finishes the row
     const RLay &LR = rp.lay[rp.root];
     const std::vector<int> RR = reg_bits(LR, cp.steps[rp.root].rc);
-    o << "    float rv[NOUT];\n";
+    o << "    " << T << " rv[NOUT];\n";
     for (int j = 0; j < nout; ++j) {
         int src[4];
         for (int d = 0; d < 4; ++d) {
@@ -926,11 +953,11 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
         o << "    rv[" << j << "] = " << (ctrl == kIdentityCtrl ? nm : "QP(" + nm + ", " + std::to_string(ctrl) + ")") << ";\n";
     }
     o << "    if (q == 0 && valid) {\n"
-         "      const int e = e" << rp.root << ";\n"
-         "      float best = 0, second = -1;\n"
+         "      const int e = " << (fp64 ? std::string("0") : "e" + std::to_string(rp.root)) << ";\n"
+         "      " << T << " best = 0, second = -1;\n"
          "      int bi = 0;\n"
          "      for (int j = 0; j < NOUT; ++j) {\n"
-         "        const float x = rv[j];\n"
+         "        const " << T << " x = rv[j];\n"
          "        if (mar) mar[b * NOUT + j] = ldexp((double)x, e);\n"
          "        if (j == 0 || x > best) {\n"
          "          if (j > 0) second = best;\n"
@@ -942,16 +969,19 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
          "      }\n"
          "      if (cls) cls[b] = bi;\n"
          "      const bool low = NOUT > 1 && second >= 0 && (double)(best - second) <= tie_tol * (double)best;\n"
-         "      if (ties) ties[b] = low ? 1 : 0;\n"
-         "      if (rl) {\n"
-         "        const bool rc = !(best > 0) || lowf != 0 ||\n"
-         "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
-         "        if (rc) {\n"
-         "          const int slot = atomicAdd(&seg_n, 1);\n"
-         "          if (slot < rcap) rl[t * rcap + slot] = (int)b;\n"
-         "        }\n"
-         "      }\n"
-         "    }\n"
+         "      if (ties) ties[b] = low ? 1 : 0;\n";
+    if (!fp64)
+        o << "      if (rl) {\n"
+             "        const bool rc = !(best > 0) || lowf != 0 ||\n"
+             "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
+             "        if (rc) {\n"
+             "          const int slot = atomicAdd(&seg_n, 1);\n"
+             "          if (slot < rcap) rl[t * rcap + slot] = (int)b;\n"
+             "        }\n"
+             "      }\n";
+    else
+        o << "      (void)recheck_tol;\n";
+    o << "    }\n"
          "   }\n"
          "   __syncthreads();\n"
          "   if (threadIdx.x == 0 && rcount) rcount[t] = min(seg_n, (int)rcap);\n"

@@ -1303,10 +1303,16 @@ int spec_code(const std::string &src, std::vector<char> &code) {
     return TQEC_OK;
 }
 
+// Live FP64 tensor registers per lane up to which the register-resident
+// FP64 kernel is offered (512 VGPRs and AGPRs at one wave per SIMD).
+constexpr int kRegBudgetF64 = 440;
+
 bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
     if (c.reg) {
-        if (c.fp64 || c.mfma || c.slab || c.threads % 64 != 0 || (1 << c.glog) * 4 != c.threads) return false;
+        if (c.mfma || c.slab || c.threads % 64 != 0 || (1 << c.glog) * 4 != c.threads) return false;
         if (!spec_reg_supported(p->cp)) return false;
+        // FP64 entries take two registers each
+        if (c.fp64 && 2 * spec_reg_peak(p->cp) > kRegBudgetF64) return false;
     }
     return spec_lds_bytes(p->cp, c) <= p->lds_limit;
 }
@@ -1318,14 +1324,15 @@ bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
 // (else two, else one), 16 threads per syndrome within [64, 128], blocks of
 // 4 outputs per lane.
 SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
-    if (!fp64 && spec_reg_supported(p->cp)) {
+    if (spec_reg_supported(p->cp)) {
         // every tensor of a syndrome fits a quad's registers: no LDS intermediates
         SpecCfg c;
         c.reg = true;
+        c.fp64 = fp64;
         c.threads = 256;
         c.glog = 6;
         c.tab_lds = false;
-        c.waves = spec_reg_peak(p->cp) <= 200 ? 2 : 0;
+        c.waves = (!fp64 && spec_reg_peak(p->cp) <= 200) ? 2 : 0;
         if (spec_fits(p, c)) return c;
     }
     for (int div : {3, 2, 1}) {
@@ -2643,8 +2650,8 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     c.mfma = (flags & 8) != 0;
     c.reg = (flags & 16) != 0;
     c.unroll = std::max(1, (flags >> 8) & 0xff);
-    if (c.reg && (c.fp64 || c.mfma || c.slab || tile * 4 != threads))
-        return set_error(TQEC_EINVAL, "register-resident kernels are FP32 without slab or MFMA, tile = threads / 4");
+    if (c.reg && (c.mfma || c.slab || tile * 4 != threads))
+        return set_error(TQEC_EINVAL, "register-resident kernels run without slab or MFMA, tile = threads / 4");
     std::string src;
     try {
         src = spec_source(p->cp, c);
