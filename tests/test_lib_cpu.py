@@ -213,8 +213,18 @@ def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, 
     assert m and int(m.group(1)) == 0
     with pytest.raises(L.TqecError):
         ct.hip.spec_source(threads // 2, threads, tab, reg=True)   # tile must be threads / 4
-    with pytest.raises(L.TqecError):
-        ct.hip.spec_source(threads // 4, threads, tab, fp64=True, reg=True)
+    # the FP64 form (SPEC_F64): doubles in register pairs, no scaling, no re-check list
+    src64 = ct.hip.spec_source(threads // 4, threads, tab, fp64=True, reg=True)
+    assert "tqec_spec_f64" in src64 and "ldexpf" not in src64 and "atomicAdd" not in src64
+    f64 = tmp_path / "reg64.hip"
+    f64.write_text("#include <hip/hip_runtime.h>\n" + src64)
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
+                        str(f64), "-o", str(tmp_path / "reg64.o"), "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
+    assert m and int(m.group(1)) == 0
     # a plan whose live tensors exceed the register file is refused
     t3 = CSSTannerGraph.from_code(ToricCode(3, 3))
     ct3 = D.compile(D.TNMMAP(mode="fused_f32"), t3, iid_error(p, p, p, t3.nq))
