@@ -1412,11 +1412,9 @@ size_t spec_slab_need(const tqec_plan *p, const SpecKernel *k, int64_t n) {
 int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n,
                 const int32_t *in_list, const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie,
                 int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream,
-                int64_t nseg = 0, int32_t segcap = 0, int64_t max_grid = 0) {
+                int64_t nseg = 0, int32_t segcap = 0) {
     const int64_t ntiles = nseg > 0 ? nseg : (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
-    int64_t g = std::min<int64_t>(ntiles, k->blocks);
-    if (max_grid > 0) g = std::min(g, max_grid);
-    const unsigned grid = (unsigned)std::max<int64_t>(1, g);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
     void *a_slab = nullptr;
     if (k->cfg.slab) {
         // the caller sized the slab before enqueueing anything that uses it
@@ -1545,7 +1543,6 @@ int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
             return rc;
         if ((rc = record(p, w, 1, stream))) return rc;
         if ((rc = record(p, w, 2, stream))) return rc;
-        // (a grid capped at one workgroup per CU measured slower)
         if (nseg > 0 && (rc = spec_launch(p, w, k64, dd, d_syn, n, w.list, w.segc, d_mar, d_cls, d_tie, nullptr,
                                           nullptr, 0, p->tie_tol, p->recheck_tol, stream, nseg, G)))
             return rc;
