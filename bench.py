@@ -233,14 +233,16 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
 
 
 def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_train=10_000_000, train_seconds=90.0,
-                batch=65536, eval_batches=8):
+                batch=65536, eval_batches=8, ref_mode="plain_f64"):
     """Learned front-end at d=7 code capacity p=0.05: a 48-512-512-512-4 MLP
     (BF16 weights, FP32 accumulation) trained by the library's own kernels on
     TN-labelled syndromes, then gated: confident syndromes keep the MLP's
     class, the rest go to the TN decoder on the device.  The gate threshold
     is the smallest of a fixed ladder whose decisions differ from the TN's on
     at most 1e-4 of a calibration batch; the evaluation batches are held
-    out.  Reported against plain FP64 TN decisions on the same syndromes."""
+    out.  Reported against FP64 TN decisions on the same syndromes: the plain
+    executor, or (``ref_mode="spec_f64"``, for codes where the plain one is
+    too slow) the exact FP64 specialised kernel on the same plan."""
     import numpy as np
     from tensorqec import decoders as D
     from tensorqec import lib as L
@@ -253,7 +255,9 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
     em = iid_error(p / 3, p / 3, p / 3, t.nq)
     if ct is None:
         ct = D.compile(D.TNMMAP(mode=mode), t, em)
-    plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
+    plain = None
+    if ref_mode == "plain_f64":
+        plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
     lx, lz = logical_operator(t)
     smp = L.HipSampler.css(t, em, lx, lz)
     mlp = HipMLP(t.ns, 4, seed=seed % 1000)
@@ -296,7 +300,13 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
         if k > 0:                       # first batch warms up
             ttn += h1 - h0
             tg += h2 - h1
-        m64, c64, t64 = plain.hip.decode(S.cpu().numpy(), "plain_f64")
+        if plain is not None:
+            m64, c64, t64 = plain.hip.decode(S.cpu().numpy(), "plain_f64")
+        else:
+            ct.hip.decode_device(S.data_ptr(), batch, ref_mode, mar.data_ptr(), cls_tn.data_ptr(), tie.data_ptr(),
+                                 stream)
+            c64 = cls_tn.cpu().numpy()
+            ct.hip.decode_device(S.data_ptr(), batch, mode, mar.data_ptr(), cls_tn.data_ptr(), tie.data_ptr(), stream)
         tr = truth.cpu().numpy()
         a, b = cls_tn.cpu().numpy(), cls_g.cpu().numpy()
         res["fallback"] += gd.fallbacks()
@@ -325,6 +335,27 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
         "gated_syndromes_per_s": batch * (eval_batches - 1) / max(tg, 1e-9),
         "tn_only_syndromes_per_s": batch * (eval_batches - 1) / max(ttn, 1e-9),
     }
+
+
+def learned_d9_leg(torch, dev, seed, p=0.05):
+    """The learned front-end where the exact contraction is expensive: d=9
+    (80 syndrome bits, 400 K multiply-adds per syndrome, LDS-tile specialised
+    kernel, autotuned), MLP trained on 40 M TN-labelled syndromes (at most
+    90 s), FP64 reference from the exact specialised kernel."""
+    from tensorqec import decoders as D
+    from tensorqec.codes import CSSTannerGraph, SurfaceCode
+    from tensorqec.errors import css_syndrome_rows, iid_error, sample_depolarizing
+    d = 9
+    t = CSSTannerGraph.from_code(SurfaceCode(d, d))
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="spec_f32"), t, em)
+    S = torch.from_numpy(css_syndrome_rows(sample_depolarizing(em, seed + 9, 65536), t)).to(dev)
+    ct.hip.autotune("spec_f32", S.data_ptr(), 65536)
+    r = learned_leg(torch, dev, seed, ct=ct, mode="spec_f32", d=d, p=p, n_train=40_000_000, train_seconds=90.0,
+                    eval_batches=4, ref_mode="spec_f64")
+    r["tn_config"] = ct.hip.spec_config()
+    r["tn_macs_per_syndrome"] = ct.hip.stats()["dep_tc"]
+    return r
 
 
 def pipeline_leg(torch, dev, seed, ct=None, mode="spec_f32", d=7, p=0.05, batch=65536, steps=10):
@@ -543,7 +574,8 @@ def main():
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
-    ap.add_argument("--only", default=None, choices=["stream", "learned", "dem", "mlp", "pipeline", "tnmap"],
+    ap.add_argument("--only", default=None,
+                    choices=["stream", "learned", "learned_d9", "dem", "mlp", "pipeline", "tnmap"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--dem-d", type=int, default=3, help="distance of the circuit-level DEM workload")
     ap.add_argument("--dem-rounds", type=int, default=3, help="syndrome rounds of the circuit-level DEM workload")
@@ -560,8 +592,8 @@ def main():
 
     if args.only:
         kw = {"dem": {"d": args.dem_d, "rounds": args.dem_rounds}}.get(args.only, {})
-        fn = {"stream": stream_leg, "learned": learned_leg, "dem": dem_leg, "mlp": mlp_leg, "pipeline": pipeline_leg,
-              "tnmap": tnmap_leg}[args.only]
+        fn = {"stream": stream_leg, "learned": learned_leg, "learned_d9": learned_d9_leg, "dem": dem_leg,
+              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg}[args.only]
         print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 
@@ -811,6 +843,10 @@ def main():
         t0 = time.time()
         res["learned"] = learned_leg(torch, dev, args.seed, ct=chosen, mode=mode_c, d=d, p=p)
         print(f"[bench] learned leg {time.time() - t0:.1f}s: {res['learned']}", file=sys.stderr, flush=True)
+        t0 = time.time()
+        res["learned_d9"] = learned_d9_leg(torch, dev, args.seed)
+        print(f"[bench] learned d=9 leg {time.time() - t0:.1f}s: gated {res['learned_d9']['gated_syndromes_per_s']:.0f} "
+              f"vs TN {res['learned_d9']['tn_only_syndromes_per_s']:.0f} syn/s", file=sys.stderr, flush=True)
         t0 = time.time()
         res["pipeline"] = pipeline_leg(torch, dev, args.seed, ct=chosen, mode=mode_c, d=d, p=p)
         print(f"[bench] pipeline leg {time.time() - t0:.1f}s: {res['pipeline']['syndromes_per_s']:.0f} syn/s",
