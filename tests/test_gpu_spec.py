@@ -135,6 +135,33 @@ def test_spec_reg_bit_identical_to_generic(name, code, p, n):
     assert np.array_equal(mar.cpu().numpy()[rows], ref[0][rows])
 
 
+@pytest.mark.parametrize("name,code,tab,factorize", [("surface3x5", SurfaceCode(3, 5), 0, False),
+                                                     ("steane", SteaneCode(), 8, True),
+                                                     ("toric2", ToricCode(2, 2), 2, True),
+                                                     ("surface5", SurfaceCode(5, 5), 8, False),
+                                                     ("shor", ShorCode(), 1, False)])
+def test_spec_reg_plan_variants_bit_identical(name, code, tab, factorize):
+    """Register-resident kernels on plans of other shapes (rectangular code,
+    no chain factorization, tabulation depth 0-8, 16 classes for the toric
+    code): FP32 bit-identical to the generic FP32 kernel, FP64 to the
+    generic FP64 kernel."""
+    torch = pytest.importorskip("torch")
+    say(f"case spec reg variant {name} tab={tab} factorize={factorize}")
+    t = CSSTannerGraph.from_code(code)
+    em = iid_error(0.02, 0.03, 0.01, t.nq)
+    ct = D.compile(D.TNMMAP(mode="fused_f32", factorize=factorize, tabulate_bits=tab), t, em)
+    n = 2500
+    S = css_syndrome_rows(sample_depolarizing(em, seed=1234 + t.nq + tab, count=n), t)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+    ct.hip.set_spec(16, 64, False, 0, reg=True)
+    assert ct.hip.spec_config()["reg"]
+    for mode, ref_mode in (("spec_f32", "fused_f32"), ("spec_f64", "fused_f64")):
+        ref = run(torch, ct, dS, n, ref_mode)
+        got = run(torch, ct, dS, n, mode)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (name, mode)
+
+
 @pytest.mark.parametrize("d,tab,n", [(5, 5, 3001), (5, 6, 3001), (7, 6, 8192)])
 def test_deep_tabulation_matches_plain_f64(d, tab, n):
     """Subtrees on up to 6 syndrome bits tabulated at plan creation (64
