@@ -53,6 +53,8 @@ struct CompiledPlan {
     int root_slot = 0;
 
     std::vector<int> reg_lay;                   // register-resident kernel: two lane bits per step (or empty)
+    bool reg_checked = false;                   // reg_lay was decided at plan creation (empty: not offered)
+    std::string reg_err;                        // why not, when reg_lay is empty
 
     // ---- statistics ------------------------------------------------------
     double tc = 0, rw = 0, dep_tc = 0, dep_rw = 0;

@@ -639,6 +639,7 @@ bool reg_plan(const CompiledPlan &cp, RegPlan &rp) {
         return rp.err = "root step not found", false;
     for (int s = 0; s < ns; ++s)
         if (uses[s] != (s == rp.root ? 0 : 1)) return rp.err = "step output not used exactly once", false;
+    if (cp.reg_checked && cp.reg_lay.empty()) return rp.err = cp.reg_err, false;
     if (cp.reg_lay.size() == 2 * (size_t)ns) {
         // prepared with the plan (spec_reg_prepare)
         for (int s = 0; s < ns; ++s) {
@@ -1006,12 +1007,16 @@ bool spec_reg_supported(const CompiledPlan &cp, std::string *why) {
 
 void spec_reg_prepare(CompiledPlan &cp) {
     cp.reg_lay.clear();
+    cp.reg_checked = false;
     RegPlan rp;
-    if (!reg_plan(cp, rp)) return;
-    for (const RLay &L : rp.lay) {
-        cp.reg_lay.push_back(L.pos[0]);
-        cp.reg_lay.push_back(L.pos[1]);
-    }
+    const bool ok = reg_plan(cp, rp);
+    if (ok)
+        for (const RLay &L : rp.lay) {
+            cp.reg_lay.push_back(L.pos[0]);
+            cp.reg_lay.push_back(L.pos[1]);
+        }
+    cp.reg_err = ok ? std::string() : rp.err;
+    cp.reg_checked = true;
 }
 
 int spec_reg_peak(const CompiledPlan &cp) {
