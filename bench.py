@@ -583,8 +583,9 @@ def main():
                     help="MiB copied once by a plain device copy before the timed steps (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (syndrome batches shard "
-              "across GPUs with no exchange, see the design notes).", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (syndromes are independent: "
+              "a batch splits into per-GPU shards with no exchange, so N GPUs give N times the one-GPU rate).",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
