@@ -317,6 +317,12 @@ int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, 
 int tqec_plan_set_spec(tqec_plan *plan, int32_t tile, int32_t threads, int32_t flags, int32_t waves);
 int tqec_plan_spec_config(tqec_plan *plan, int32_t *tile, int32_t *threads, int32_t *flags, int32_t *waves);
 
+/* Max-product plans: run tqec_map_decode_batch[_device] on a kernel
+ * generated for the plan (1, default; compiled with hiprtc at first use,
+ * bit-identical results) or on the generic max-product kernel (0).  No
+ * reference counterpart. */
+int tqec_plan_set_map_kernel(tqec_plan *plan, int32_t specialised);
+
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);
 

@@ -1205,6 +1205,113 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     return o.str();
 }
 
+std::string spec_map_source(const CompiledPlan &cp, int glog, int threads) {
+    const int G = 1 << glog;
+    const int ns = (int)cp.steps.size();
+    const size_t const_n = cp.const_pool.size();
+    (void)const_n;
+    const int words = (cp.n_vars + 31) / 32;
+    const size_t work_b = (size_t)std::max<int64_t>(1, cp.ws_entries) * G * 8;
+    const size_t arg_off = round_up(work_b, 16);
+    const size_t syn_off = arg_off + round_up((size_t)std::max<int64_t>(1, cp.arg_entries) * G * 2, 16);
+    const size_t total = round_up(syn_off + (size_t)std::max(1, cp.n_syn) * G, 16);
+    std::ostringstream o;
+    o << "// plan-specialised max-product (TNMAP) kernel (generated)\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << threads << ") tqec_map_spec(\n"
+         "    const unsigned char *__restrict__ syn, long long n, const double *__restrict__ cpool,\n"
+         "    const double *__restrict__ spool, unsigned char *__restrict__ config, int n_config,\n"
+         "    double *__restrict__ logp) {\n";
+    o << "  constexpr int G = " << G << ", GLOG = " << glog << ", NT = " << threads << ", CSTEP = " << (threads >> glog)
+      << ", NBITS = " << std::max(1, cp.n_syn) << ", NV = " << cp.n_vars << ";\n";
+    o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << total << "];\n";
+    o << "  double *work = reinterpret_cast<double *>(smem);\n";
+    o << "  unsigned short *arg = reinterpret_cast<unsigned short *>(smem + " << arg_off << ");\n";
+    o << "  unsigned char *synl = smem + " << syn_off << ";\n";
+    o << "  const int tid = threadIdx.x, g = tid & (G - 1), c0 = tid >> GLOG;\n";
+    o << "  const long long ntiles = (n + G - 1) >> GLOG;\n";
+    o << "  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {\n"
+         "    const long long b0 = t << GLOG;\n"
+         "    const long long rows = min((long long)G, n - b0);\n"
+         "    const unsigned char *src = syn + b0 * NBITS;\n"
+         "    for (int i = tid; i < G * NBITS; i += NT) {\n"
+         "      const int r = i / NBITS, bit = i - r * NBITS;\n"
+         "      synl[(bit << GLOG) + r] = (r < rows) ? (src[i] & 1u) : 0u;\n"
+         "    }\n"
+         "    __syncthreads();\n";
+    // forward: every output keeps the first maximum over (k, sa, sb), as the generic kernel
+    for (int s = 0; s < ns; ++s) {
+        const StepDesc &st = cp.steps[s];
+        const MapStep &ms = cp.map_steps[s];
+        o << "    { // step " << s << ": rank " << st.rc << ", eliminated " << (st.nk + st.a.nsingle + st.b.nsingle)
+          << "\n";
+        const char *nm[2] = {"pa", "pb"};
+        for (int side = 0; side < 2; ++side) {
+            const OperandDesc &op = side ? st.b : st.a;
+            if (op.kind == kInter) {
+                o << "      const double *" << nm[side] << " = work + (" << (long long)op.off << "ll << GLOG) + g;\n";
+            } else if (op.kind == kConst) {
+                o << "      const double *" << nm[side] << " = cpool + " << (long long)op.off << ";\n";
+            } else {
+                o << "      unsigned v" << nm[side] << " = 0u;\n";
+                for (int j = 0; j < op.nev; ++j)
+                    o << "      v" << nm[side] << " |= (unsigned)synl[(" << op.ev_bit[j] << " << GLOG) + g] << " << j
+                      << ";\n";
+                o << "      const double *" << nm[side] << " = spool + " << (long long)op.off << " + ((long long)v"
+                  << nm[side] << " << " << op.rank << ");\n";
+            }
+        }
+        auto ld = [&](const OperandDesc &op, const char *base, const std::string &idx) {
+            return op.kind == kInter ? std::string(base) + "[(" + idx + ") << GLOG]" : std::string(base) + "[" + idx + "]";
+        };
+        const int cexp = st.a.cexp + st.b.cexp;
+        o << "      for (int c = c0; c < " << (1 << st.rc) << "; c += CSTEP) {\n";
+        o << "        const unsigned ar = " << subst_c(gather_expr(st.a), "(unsigned)c") << ";\n";
+        o << "        const unsigned br = " << subst_c(gather_expr(st.b), "(unsigned)c") << ";\n";
+        o << "        double best = -1.0;\n        unsigned bi = 0u;\n";
+        o << "        for (unsigned k = 0; k < " << (1u << st.nk) << "u; ++k)\n";
+        o << "          for (unsigned qa = 0; qa < " << (1u << st.a.nsingle) << "u; ++qa) {\n";
+        o << "            const double va = "
+          << ld(st.a, "pa", "ar | (qa << " + std::to_string(st.a.sshift) + ") | (k << " + std::to_string(st.a.kshift) + ")")
+          << ";\n";
+        o << "            for (unsigned qb = 0; qb < " << (1u << st.b.nsingle) << "u; ++qb) {\n";
+        o << "              const double vb = "
+          << ld(st.b, "pb", "br | (qb << " + std::to_string(st.b.sshift) + ") | (k << " + std::to_string(st.b.kshift) + ")")
+          << ";\n";
+        o << "              const double v = va * vb;\n";
+        o << "              if (v > best) {\n                best = v;\n                bi = k | (qa << " << st.nk
+          << ") | (qb << " << (st.nk + st.a.nsingle) << ");\n              }\n";
+        o << "            }\n          }\n";
+        if (cexp != 0) o << "        best = ldexp(best, " << cexp << ");\n";
+        o << "        work[((" << (long long)st.c_off << "ll + c) << GLOG) + g] = best;\n";
+        o << "        arg[((" << (long long)ms.arg_off << "ll + c) << GLOG) + g] = (unsigned short)bi;\n";
+        o << "      }\n    }\n    __syncthreads();\n";
+    }
+    // backtrace: one thread per syndrome, from the root down, labels as constants
+    o << "    if (tid < G && b0 + tid < n) {\n";
+    o << "      unsigned as[" << std::max(1, words) << "];\n";
+    o << "      #pragma unroll\n      for (int w = 0; w < " << std::max(1, words) << "; ++w) as[w] = 0u;\n";
+    for (int s = ns - 1; s >= 0; --s) {
+        const MapStep &ms = cp.map_steps[s];
+        const int32_t *L = cp.map_labels.data() + ms.off;
+        o << "      {\n        unsigned c = 0u;\n";
+        for (int j = 0; j < ms.nc; ++j)
+            o << "        c |= ((as[" << (L[j] >> 5) << "] >> " << (L[j] & 31) << ") & 1u) << " << j << ";\n";
+        o << "        const unsigned rec = arg[((" << (long long)ms.arg_off << "ll + c) << GLOG) + tid];\n";
+        const int ne = ms.nk + ms.nsa + ms.nsb;
+        for (int j = 0; j < ne; ++j) {
+            const int l = L[ms.nc + j];
+            o << "        as[" << (l >> 5) << "] |= ((rec >> " << j << ") & 1u) << " << (l & 31) << ";\n";
+        }
+        o << "      }\n";
+    }
+    o << "      const long long b = b0 + tid;\n";
+    o << "      #pragma unroll\n      for (int v = 0; v < NV; ++v)\n"
+         "        if (v < n_config) config[b * n_config + v] = (unsigned char)((as[v >> 5] >> (v & 31)) & 1u);\n";
+    o << "      if (logp) logp[b] = log(work[(" << (long long)cp.root_off << "ll << GLOG) + tid]);\n";
+    o << "    }\n    __syncthreads();\n  }\n}\n";
+    return o.str();
+}
+
 size_t spec_mixed_lds_bytes(const std::vector<MixedSpecPlan> &plans) {
     size_t m = 16;
     for (const MixedSpecPlan &q : plans) {

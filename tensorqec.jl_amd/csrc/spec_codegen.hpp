@@ -91,6 +91,14 @@ struct MixedSpecPlan {
 size_t spec_mixed_lds_bytes(const std::vector<MixedSpecPlan> &plans);
 std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threads, size_t slab32_entries,
                               size_t slab64_entries);
+// Max-product (TNMAP) plan: kernel `tqec_map_spec(const unsigned char *syn,
+// long long n, const double *cpool, const double *spool, unsigned char
+// *config, int n_config, double *logp)`, G = 2^glog syndromes per
+// workgroup, the generic max-product kernel's LDS layout, order of
+// comparisons and backtrace with every descriptor value substituted (the
+// backtrace's label words in registers).  Bit-identical to it.
+std::string spec_map_source(const CompiledPlan &cp, int glog, int threads);
+
 inline const char *spec_kernel_name(const SpecCfg &cfg) {
     return cfg.fp64 ? "tqec_spec_f64" : (cfg.mfma ? "tqec_spec_bf16" : "tqec_spec_f32");
 }

@@ -865,7 +865,8 @@ struct tqec_plan {
     int n_cu = 0;
     SpecCfg spec_cfg;
     bool spec_cfg_set = false;
-    std::map<std::tuple<int, int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64, slab)
+    std::map<std::tuple<int, int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64, flags)
+    bool map_spec = true;               // max-product plans: plan-specialised kernel (else the generic one)
     SpecCfg spec64_cfg;                 // TQEC_MODE_SPEC_F64
     bool spec64_cfg_set = false;
     SpecCfg specb_cfg;                  // TQEC_MODE_SPEC_BF16
@@ -2261,10 +2262,38 @@ int tqec_map_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, 
     a.n_config = n_config;
     a.config = d_config;
     a.logp = d_logp;
-    TQ_HIP(hipFuncSetAttribute((const void *)map_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_limit));
     const int64_t ntiles = (n + (1 << gl) - 1) >> gl;
     const int per_cu = resident_per_cu(lds, threads, p->lds_limit);
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)p->n_cu * per_cu));
+    if (p->map_spec) {
+        // plan-specialised max-product kernel (hiprtc, cached per device and tile)
+        const auto key = std::make_tuple(dev, gl, threads, 0, 0, 1, 1 << 20);
+        auto it = p->spec.find(key);
+        if (it == p->spec.end()) {
+            std::vector<char> code;
+            if ((rc = spec_code(spec_map_source(cp, gl, threads), code))) return rc;
+            SpecKernel k;
+            k.lds = lds;
+            TQ_HIP(hipModuleLoadData(&k.mod, code.data()));
+            const hipError_t e = hipModuleGetFunction(&k.fn, k.mod, "tqec_map_spec");
+            if (e != hipSuccess) {
+                (void)hipModuleUnload(k.mod);
+                return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
+            }
+            it = p->spec.emplace(key, k).first;
+        }
+        const unsigned char *a_syn = d_syn;
+        long long a_n = (long long)n;
+        const double *a_cp = dd->cpool64, *a_sp = dd->spool64;
+        unsigned char *a_cfg = d_config;
+        int a_nc = n_config;
+        double *a_lp = d_logp;
+        void *args[] = {&a_syn, &a_n, &a_cp, &a_sp, &a_cfg, &a_nc, &a_lp};
+        TQ_HIP(hipModuleLaunchKernel(it->second.fn, grid, 1, 1, (unsigned)threads, 1, 1, 0, (hipStream_t)stream, args,
+                                     nullptr));
+        return TQEC_OK;
+    }
+    TQ_HIP(hipFuncSetAttribute((const void *)map_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_limit));
     hipLaunchKernelGGL(map_kernel, dim3(grid), dim3(threads), lds, (hipStream_t)stream, a);
     TQ_HIP(hipGetLastError());
     return TQEC_OK;
@@ -2710,6 +2739,14 @@ int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t flag
         p->spec_cfg = c;
         p->spec_cfg_set = true;
     }
+    return TQEC_OK;
+}
+
+int tqec_plan_set_map_kernel(tqec_plan *p, int32_t specialised) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (p->cp.semiring != TQEC_SEMIRING_MAX) return set_error(TQEC_EINVAL, "not a max-product plan");
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->map_spec = specialised != 0;
     return TQEC_OK;
 }
 

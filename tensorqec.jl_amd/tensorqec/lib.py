@@ -180,6 +180,7 @@ EXPORTS = [
     ("tqec_plan_set_spec", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            ctypes.c_int32]),
     ("tqec_plan_spec_config", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p, _i32p]),
+    ("tqec_plan_set_map_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("tqec_last_error", ctypes.c_char_p, []),
 ]
 
@@ -338,6 +339,10 @@ class HipPlan:
                  unroll: int = 1, reg: bool = False) -> None:
         flags = int(bool(tables_in_lds)) | (4 if slab else 0) | (16 if reg else 0)
         check(load().tqec_plan_set_spec(self._h, int(tile), int(threads), flags, int(waves) | (int(unroll) << 8)))
+
+    def set_map_kernel(self, specialised: bool) -> None:
+        """Max-product plans: plan-specialised (default) or generic kernel."""
+        check(load().tqec_plan_set_map_kernel(self._h, 1 if specialised else 0))
 
     def spec_config(self) -> dict:
         v = [ctypes.c_int32() for _ in range(4)]
