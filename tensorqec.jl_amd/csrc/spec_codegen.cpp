@@ -1208,8 +1208,6 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
 std::string spec_map_source(const CompiledPlan &cp, int glog, int threads) {
     const int G = 1 << glog;
     const int ns = (int)cp.steps.size();
-    const size_t const_n = cp.const_pool.size();
-    (void)const_n;
     const int words = (cp.n_vars + 31) / 32;
     const size_t work_b = (size_t)std::max<int64_t>(1, cp.ws_entries) * G * 8;
     const size_t arg_off = round_up(work_b, 16);
