@@ -11,17 +11,27 @@ import numpy as np
 
 
 def bf16(x):
+    """Round to the nearest BF16 value, ties to even (finite inputs)."""
     a = np.asarray(x, dtype=np.float32)
     u = a.view(np.uint32).astype(np.uint64)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def forward_backward(params: dict, X: np.ndarray, Q: np.ndarray):
+_bf16 = bf16
+
+
+def forward_backward(params: dict, X: np.ndarray, Q: np.ndarray, round_bf16: bool = True):
     """params: W1 b1 W2 b2 W3 b3 W4 b4 (float arrays, W1 rows = padded input
     width); X: (B, kin) 0/1 features; Q: (B, n_out) target probabilities.
-    Returns (loss, logits, grads dict)."""
+    Returns (loss, logits, grads dict).  With ``round_bf16=False`` every
+    value stays float64 (the exact gradient of the float64 network); with
+    rounding, the gradient is taken through BF16 storage as through the
+    identity (what the GPU's backward pass does).  Shapes are checked."""
     B = X.shape[0]
+    assert Q.shape[0] == B and params["W4"].shape[1] == Q.shape[1]
+    assert params["W1"].shape[0] == X.shape[1]
+    bf16 = _bf16 if round_bf16 else (lambda x: np.asarray(x, dtype=np.float64))
     W = {k: bf16(params[k]) for k in ("W1", "W2", "W3", "W4")}
     b = {k: np.asarray(params[k], dtype=np.float64) for k in ("b1", "b2", "b3", "b4")}
     H0 = np.asarray(X, dtype=np.float64)
