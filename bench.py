@@ -87,6 +87,17 @@ def measured_traffic(mode, plan_cfg, batch):
     return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
@@ -800,9 +811,25 @@ def main():
             dt_6, _, _, _, cls_6, tie_6, mar_6 = run(chosen, "spec_f64", "chosen spec_f64", warmup=20, instrument=False)
             ok6 = (tie_p == 0) & (tie_6 == 0)
             res["spec_f64_syndromes_per_s"] = args.steps * B / dt_6
+            res["speedup_vs_spec_f64"] = thr_c / res["spec_f64_syndromes_per_s"]
             res["spec_f64_class_agreement_with_plain_f64"] = float((cls_6[ok6] == cls_p[ok6]).mean())
             res["spec_f64_max_rel_marginal_err_vs_plain_f64"] = float(
                 np.max(np.abs(mar_6 - mar_p) / mar_p.max(axis=2, keepdims=True)))
+        # the plan as the Julia binding hands it over: the reference network
+        # (no chain factorization), one-hot evidence leaves, default tree
+        # search and tabulation depth, default specialised configuration
+        dropin = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="spec_f32", tabulate_bits=4), t, em)
+        dj = {"plan": "reference network, one-hot evidence leaves, tabulate_bits 4, default spec_f32 configuration",
+              "config": dropin.hip.spec_config(), "dep_tc": dropin.hip.stats()["dep_tc"]}
+        for mode_j in ("spec_f32", "spec_f64"):
+            dt_j, _, _, _, cls_j, tie_j, mar_j = run(dropin, mode_j, f"drop-in plan {mode_j}", warmup=20,
+                                                     instrument=False)
+            okj = (tie_p == 0) & (tie_j == 0)
+            dj[mode_j] = {"syndromes_per_s": args.steps * B / dt_j,
+                          "class_agreement_with_plain_f64": float((cls_j[okj] == cls_p[okj]).mean()),
+                          "max_rel_marginal_err_vs_plain_f64": float(
+                              np.max(np.abs(mar_j - mar_p) / mar_p.max(axis=2, keepdims=True)))}
+        res["dropin_plan"] = dj
     st = chosen.hip.stats()
     traffic = measured_traffic(mode_c, plan_cfg, B)
     flops_per_syn = 2.0 * st["dep_tc"]
@@ -820,21 +847,28 @@ def main():
             cores = len(os.sched_getaffinity(0))
         except AttributeError:
             cores = os.cpu_count() or 1
-        cores = max(1, min(cores, 16))
         op = OraclePlan(plain.plan)
         s_all = np.concatenate(S_host)
-        t0 = time.perf_counter()
-        op.decode(s_all[:512], nthreads=cores)
-        rate0 = 512 / max(time.perf_counter() - t0, 1e-6)
-        ncpu = int(min(len(s_all), max(1024, rate0 * args.cpu_seconds)))
-        t0 = time.perf_counter()
-        _, ccls, _ = op.decode(s_all[:ncpu], nthreads=cores)
-        dt = time.perf_counter() - t0
+
+        def cpu_rate(nthreads, seconds):
+            t0 = time.perf_counter()
+            op.decode(s_all[:256], nthreads=nthreads)
+            rate0 = 256 / max(time.perf_counter() - t0, 1e-6)
+            ncpu = int(min(len(s_all), max(256, rate0 * seconds)))
+            t0 = time.perf_counter()
+            op.decode(s_all[:ncpu], nthreads=nthreads)
+            return ncpu, time.perf_counter() - t0
+
+        ncpu, dt = cpu_rate(cores, args.cpu_seconds)
+        n1, dt1 = cpu_rate(1, min(6.0, args.cpu_seconds))
         cpu = {"value": ncpu / dt, "unit": "syndromes/s", "cores": cores, "kind": "port",
                "sample": f"first {ncpu} syndromes of the same d=7 p=0.05 batches, FP64, reference network "
-                         f"(one-hot evidence, every step per syndrome), OpenMP over syndromes, "
-                         f"{dt:.1f}s wall"}
-        print(f"[bench] cpu baseline {ncpu / dt:.0f} syn/s on {cores} threads", file=sys.stderr, flush=True)
+                         f"(one-hot evidence, every step per syndrome), OpenMP over syndromes on every core of "
+                         f"the process's affinity set, {dt:.1f}s wall; SCIP's pure error not timed",
+               "value_1core": n1 / dt1, "sample_1core": f"first {n1} syndromes, one thread, {dt1:.1f}s wall",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+        print(f"[bench] cpu baseline {ncpu / dt:.0f} syn/s on {cores} threads, {n1 / dt1:.0f} syn/s on 1 "
+              f"({cpu['cpu_model']})", file=sys.stderr, flush=True)
 
     if not args.no_extras:
         t0 = time.time()
@@ -886,7 +920,7 @@ def main():
                    "plan_stats": {k: st[k] for k in ("n_dep_steps", "dep_tc", "dep_rw", "fused_lds_entries",
                                                      "table_entries", "const_entries")},
                    "plain": "plain_f64 (reference network, one-hot evidence, one kernel per step)"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp32_valu" if mode_c in ("spec_f32", "fused_f32") else "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "tqec_spec_f32" if mode_c == "spec_f32" else "fused_kernel<float, true>",
                      "kernel_ms_avg": avg_ms,

@@ -23,9 +23,13 @@
  *   - Node ids: leaves are 0..n_leaves-1, step i produces node n_leaves+i.
  *   - All functions return TQEC_OK (0) or a negative TQEC_E* code and never
  *     abort; tqec_last_error() returns a thread-local message.
- *   - A plan is immutable after creation.  Per-call device workspace is kept
- *     per (plan, stream); calls on different streams may run concurrently,
- *     calls on the same stream are ordered by the stream.
+ *   - The contraction a plan describes is fixed at creation.  Only its
+ *     execution settings change afterwards (tqec_plan_set_tolerances,
+ *     _set_tile, _set_spec, _set_map_kernel, _set_timing, and the
+ *     autotuner); each setter takes the plan's lock and applies to calls
+ *     made after it returns.  Per-call device workspace is kept per (plan,
+ *     stream); calls on different streams may run concurrently, calls on the
+ *     same stream are ordered by the stream.
  *   - tqec_decode_batch takes host pointers and synchronises the stream
  *     before returning; tqec_decode_batch_device takes device pointers and
  *     only enqueues work on the stream.
@@ -33,6 +37,7 @@
 #ifndef TQEC_HIP_H
 #define TQEC_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -249,7 +254,10 @@ typedef struct {
 } tqec_mixed_batch;
 
 /* Binds the plans to the current device; at most `max_batches` batches and
- * `max_rows` syndromes in total per call.  The plans must outlive it. */
+ * `max_rows` syndromes in total per call.  The plans must outlive it.  Each
+ * plan's tie and re-check tolerances are copied to the device here: a later
+ * tqec_plan_set_tolerances on a plan does not reach an executor created
+ * before it. */
 int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batches, int64_t max_rows,
                       tqec_mixed **out);
 /* Graph capture on (default) or off. */
@@ -318,10 +326,17 @@ int tqec_plan_set_spec(tqec_plan *plan, int32_t tile, int32_t threads, int32_t f
 int tqec_plan_spec_config(tqec_plan *plan, int32_t *tile, int32_t *threads, int32_t *flags, int32_t *waves);
 
 /* Max-product plans: run tqec_map_decode_batch[_device] on a kernel
- * generated for the plan (1, default; compiled with hiprtc at first use,
- * bit-identical results) or on the generic max-product kernel (0).  No
- * reference counterpart. */
+ * generated for the plan (1, default; compiled with hiprtc at the first call
+ * on a device, so that call pays the compile time and should not be the
+ * first one inside a stream capture; bit-identical results) or on the
+ * generic max-product kernel (0).  No reference counterpart. */
 int tqec_plan_set_map_kernel(tqec_plan *plan, int32_t specialised);
+/* Kernel the next max-product call will use (1 = plan-specialised).  When
+ * the generated kernel could not be built (hiprtc or module-load failure)
+ * the plan falls back to the generic kernel by itself and `note` (NULL to
+ * skip; at most cap-1 bytes, NUL-terminated) receives the reason; it is
+ * empty otherwise. */
+int tqec_plan_map_kernel(tqec_plan *plan, int32_t *specialised, char *note, size_t cap);
 
 /* Tunables of a plan (defaults: tie_tol 1e-12, recheck_tol 1e-3). */
 int tqec_plan_set_tolerances(tqec_plan *plan, double tie_tol, double recheck_tol);
@@ -398,6 +413,32 @@ int tqec_recovery_create_dem(int32_t nm, int32_t nd, const uint8_t *R, int32_t n
 int tqec_recover(tqec_recovery *rec, const uint8_t *d_syndromes, const int32_t *d_classes, int64_t n,
                  uint8_t *d_err_a, uint8_t *d_err_b, void *stream);
 void tqec_recovery_destroy(tqec_recovery *rec);
+
+/* ---------------------------------------------------------------------------
+ * Logical-error-rate loop on the device.  Replaces the per-sample harness
+ * loop `random_error_pattern` -> `syndrome_extraction` -> `decode` ->
+ * `check_logical_error` (src/decoding/error_model.jl:97-181, driven by
+ * src/decoding/threshold.jl:1-32 and the test loops of test/decoding/):
+ * samples first .. first+n-1 of the sampler's stream are drawn, decoded
+ * with `plan` in `mode`, their error patterns recovered, and the difference
+ * between the sampled and the recovered pattern checked against every
+ * logical operator, in chunks of `batch` syndromes; nothing crosses to the
+ * host but four counts:
+ *   counts[0]  samples with a logical error (`check_logical_error` true)
+ *   counts[1]  of which in the X-error half (lz . (ex + ex') odd for a row;
+ *              0 for detector error models)
+ *   counts[2]  of which in the Z-error half (lx . (ez + ez') odd)
+ *   counts[3]  recovered patterns that do not reproduce the sampled syndrome
+ * The sampler, recovery and plan must describe the same code (checked) and
+ * outlive the handle; device buffers for one chunk are allocated at
+ * creation.  tqec_ler_run synchronises `stream`.
+ * ------------------------------------------------------------------------- */
+typedef struct tqec_ler tqec_ler;
+int tqec_ler_create(tqec_plan *plan, tqec_sampler *sampler, tqec_recovery *recovery, int64_t batch,
+                    tqec_ler **out);
+int tqec_ler_run(tqec_ler *ler, int32_t mode, uint64_t seed, int64_t first, int64_t n, int64_t *counts,
+                 void *stream);
+void tqec_ler_destroy(tqec_ler *ler);
 
 /* Thread-local description of the last error. */
 const char *tqec_last_error(void);

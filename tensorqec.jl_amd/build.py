@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "tensorqec", "libtqec_hip.so")
 
-SOURCES = ["tqec_hip.hip", "plan_compile.cpp", "spec_codegen.cpp", "tnmap_hip.hip", "mlp_hip.hip", "sampler_hip.hip"]
+SOURCES = ["tqec_hip.hip", "plan_compile.cpp", "spec_codegen.cpp", "mlp_hip.hip", "sampler_hip.hip"]
 
 # Packed FP32 instructions are disabled for all device code: the compiler is
 # free to form them from float2 arithmetic otherwise.
@@ -31,7 +31,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False, extra=None) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
     objs = []
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)

@@ -174,6 +174,67 @@ __global__ void recovery_kernel(RecoveryArgs a) {
         for (int q = 0; q < a.nq; ++q) a.eb[i * a.nq + q] = (uint8_t)((eb[q >> 6] >> (q & 63)) & 1ull);
 }
 
+// Logical-error check of a batch: per sample, the difference d between the
+// sampled and the recovered error pattern; its logical class bits (the XOR of
+// the class columns of the variables where d is set) are `check_logical_error`
+// (src/decoding/error_model.jl:161-181) of every logical row at once, and its
+// syndrome (XOR of the syndrome columns) must be zero when the recovered
+// pattern reproduces the sampled syndrome.  counts: [0] samples with a
+// logical error, [1] with one in the X-error half (lz . dx odd for some row;
+// DEM: 0), [2] in the Z-error half (lx . dz odd), [3] recovered patterns that
+// do not reproduce the syndrome.
+struct CheckArgs {
+    int kind, nvar, synw, k;
+    const unsigned long long *colx, *colz;
+    const uint32_t *logx, *logz;
+    const uint8_t *err;        // sampled: kind 0 [n][2 nvar] (x, z); kind 1 [n][nvar]
+    const uint8_t *ea, *eb;    // recovered: ex / ez (kind 0) or e (kind 1), [n][nvar]
+    int64_t n;
+    unsigned long long *counts;
+};
+
+__global__ void ler_check_kernel(CheckArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool lerr = false, xerr = false, zerr = false, bad = false;
+    if (i < a.n) {
+        unsigned long long syn[kMaxSynWords];
+        for (int w = 0; w < kMaxSynWords; ++w) syn[w] = 0ull;
+        uint32_t lg = 0;
+        const int stride = a.kind == 0 ? 2 * a.nvar : a.nvar;
+        const uint8_t *e = a.err + i * stride;
+        for (int j = 0; j < a.nvar; ++j) {
+            if ((e[j] ^ a.ea[i * a.nvar + j]) & 1) {
+                for (int w = 0; w < a.synw; ++w) syn[w] ^= a.colx[(int64_t)j * a.synw + w];
+                lg ^= a.logx[j];
+            }
+            if (a.kind == 0 && ((e[a.nvar + j] ^ a.eb[i * a.nvar + j]) & 1)) {
+                for (int w = 0; w < a.synw; ++w) syn[w] ^= a.colz[(int64_t)j * a.synw + w];
+                lg ^= a.logz[j];
+            }
+        }
+        unsigned long long any = 0ull;
+        for (int w = 0; w < a.synw; ++w) any |= syn[w];
+        bad = any != 0ull;
+        lerr = lg != 0u;
+        if (a.kind == 0) {
+            zerr = (lg & ((1u << a.k) - 1u)) != 0u;
+            xerr = (lg >> a.k) != 0u;
+        }
+    }
+    // one atomic per wave and counter
+    const unsigned long long bl = __ballot(lerr), bx = __ballot(xerr), bz = __ballot(zerr), bb = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (bl) atomicAdd(&a.counts[0], (unsigned long long)__popcll(bl));
+        if (bx) atomicAdd(&a.counts[1], (unsigned long long)__popcll(bx));
+        if (bz) atomicAdd(&a.counts[2], (unsigned long long)__popcll(bz));
+        if (bb) atomicAdd(&a.counts[3], (unsigned long long)__popcll(bb));
+    }
+}
+
+__global__ void ler_zero_kernel(unsigned long long *c) {
+    if (threadIdx.x < 4) c[threadIdx.x] = 0ull;
+}
+
 #define TQS_HIP(call)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \
@@ -209,6 +270,18 @@ struct tqec_sampler {
 struct tqec_recovery {
     RecoveryArgs a{};
     unsigned long long *Ra = nullptr, *Rb = nullptr, *la = nullptr, *lb = nullptr, *fa = nullptr, *fb = nullptr;
+};
+
+struct tqec_ler {
+    tqec_plan *plan = nullptr;
+    tqec_sampler *smp = nullptr;
+    tqec_recovery *rec = nullptr;
+    int64_t batch = 0;
+    int nout = 0;
+    uint8_t *syn = nullptr, *err = nullptr, *ea = nullptr, *eb = nullptr, *tie = nullptr;
+    int32_t *cls = nullptr;
+    double *mar = nullptr;
+    unsigned long long *counts = nullptr;
 };
 
 extern "C" {
@@ -433,6 +506,92 @@ int tqec_recover(tqec_recovery *r, const uint8_t *d_syndromes, const int32_t *d_
     hipLaunchKernelGGL(recovery_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, (hipStream_t)stream, a);
     TQS_HIP(hipGetLastError());
     return TQEC_OK;
+}
+
+int tqec_ler_create(tqec_plan *plan, tqec_sampler *smp, tqec_recovery *rec, int64_t batch, tqec_ler **out) {
+    if (!out || !plan || !smp || !rec) return fail(TQEC_ENULL, "null argument");
+    *out = nullptr;
+    if (batch <= 0 || batch > (int64_t(1) << 24)) return fail(TQEC_EINVAL, "batch must be in [1, 2^24]");
+    tqec_stats st{};
+    int rc;
+    if ((rc = tqec_plan_stats(plan, &st))) return rc;
+    const RecoveryArgs &ra = rec->a;
+    if (ra.kind != smp->kind || ra.nq != smp->nvar || ra.nsyn != smp->nsyn)
+        return fail(TQEC_EINVAL, "sampler and recovery describe different codes");
+    if (ra.kind == 0 ? (2 * ra.k != smp->nlog) : (ra.k != smp->nlog))
+        return fail(TQEC_EINVAL, "sampler and recovery have different logical counts");
+    if (st.n_out != smp->nlog) return fail(TQEC_EINVAL, "plan and sampler have different logical counts");
+    auto *h = new (std::nothrow) tqec_ler();
+    if (!h) return fail(TQEC_ENOMEM, "out of host memory");
+    h->plan = plan;
+    h->smp = smp;
+    h->rec = rec;
+    h->batch = batch;
+    h->nout = 1 << st.n_out;
+    const size_t B = (size_t)batch, nv = (size_t)smp->nvar;
+    hipError_t e = hipMalloc((void **)&h->syn, B * smp->nsyn);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->err, B * nv * (smp->kind == 0 ? 2 : 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->ea, B * nv);
+    if (e == hipSuccess && smp->kind == 0) e = hipMalloc((void **)&h->eb, B * nv);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->tie, B);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->cls, B * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->mar, B * h->nout * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->counts, 4 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        tqec_ler_destroy(h);
+        return fail(TQEC_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return TQEC_OK;
+}
+
+int tqec_ler_run(tqec_ler *h, int32_t mode, uint64_t seed, int64_t first, int64_t n, int64_t *counts, void *stream) {
+    if (!h || !counts) return fail(TQEC_ENULL, "null argument");
+    if (n < 0 || first < 0) return fail(TQEC_EINVAL, "negative count or offset");
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(ler_zero_kernel, dim3(1), dim3(64), 0, st, h->counts);
+    TQS_HIP(hipGetLastError());
+    int rc;
+    for (int64_t off = 0; off < n; off += h->batch) {
+        const int64_t nb = std::min(h->batch, n - off);
+        if ((rc = tqec_sample(h->smp, seed, first + off, nb, h->syn, nullptr, h->err, stream))) return rc;
+        if ((rc = tqec_decode_batch_device(h->plan, h->syn, nb, mode, h->mar, h->cls, h->tie, stream))) return rc;
+        if ((rc = tqec_recover(h->rec, h->syn, h->cls, nb, h->ea, h->eb, stream))) return rc;
+        CheckArgs a{};
+        a.kind = h->smp->kind;
+        a.nvar = h->smp->nvar;
+        a.synw = h->smp->synw;
+        a.k = h->rec->a.k;
+        a.colx = h->smp->colx;
+        a.colz = h->smp->colz;
+        a.logx = h->smp->logx;
+        a.logz = h->smp->logz;
+        a.err = h->err;
+        a.ea = h->ea;
+        a.eb = h->eb;
+        a.n = nb;
+        a.counts = h->counts;
+        hipLaunchKernelGGL(ler_check_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, a);
+        TQS_HIP(hipGetLastError());
+    }
+    unsigned long long c[4];
+    TQS_HIP(hipMemcpyAsync(c, h->counts, sizeof(c), hipMemcpyDeviceToHost, st));
+    TQS_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < 4; ++i) counts[i] = (int64_t)c[i];
+    return TQEC_OK;
+}
+
+void tqec_ler_destroy(tqec_ler *h) {
+    if (!h) return;
+    (void)hipFree(h->syn);
+    (void)hipFree(h->err);
+    (void)hipFree(h->ea);
+    (void)hipFree(h->eb);
+    (void)hipFree(h->tie);
+    (void)hipFree(h->cls);
+    (void)hipFree(h->mar);
+    (void)hipFree(h->counts);
+    delete h;
 }
 
 void tqec_recovery_destroy(tqec_recovery *r) {

@@ -867,6 +867,7 @@ struct tqec_plan {
     bool spec_cfg_set = false;
     std::map<std::tuple<int, int, int, int, int, int, int>, SpecKernel> spec;   // (device, glog, threads, tab_lds, waves, fp64, flags)
     bool map_spec = true;               // max-product plans: plan-specialised kernel (else the generic one)
+    std::string map_spec_note;          // why the specialised max-product kernel was given up, if it was
     SpecCfg spec64_cfg;                 // TQEC_MODE_SPEC_F64
     bool spec64_cfg_set = false;
     SpecCfg specb_cfg;                  // TQEC_MODE_SPEC_BF16
@@ -1365,9 +1366,20 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
     return c;
 }
 
+std::tuple<int, int, int, int, int, int, int> spec_key(int dev, const SpecCfg &cfg) {
+    return std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
+                           (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.reg ? 4 : 0) | (cfg.unroll << 3));
+}
+
+// The compiled kernel of a configuration on a device, or nullptr when it has
+// not been built there yet (never compiles, never allocates).
+SpecKernel *spec_find(tqec_plan *p, int dev, const SpecCfg &cfg) {
+    auto it = p->spec.find(spec_key(dev, cfg));
+    return it == p->spec.end() ? nullptr : &it->second;
+}
+
 int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
-    const auto key = std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
-                                     (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.reg ? 4 : 0) | (cfg.unroll << 3));
+    const auto key = spec_key(dev, cfg);
     auto it = p->spec.find(key);
     if (it != p->spec.end()) {
         *out = &it->second;
@@ -1442,7 +1454,7 @@ int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const
 // FP64 re-check kernel: one syndrome per 256-thread workgroup (the list is
 // short, so the latency of one syndrome's step chain matters: one output
 // per thread on the widest steps), in a slab when too wide for LDS.
-int spec_recheck_kernel(tqec_plan *p, int dev, SpecKernel **out) {
+SpecCfg spec_recheck_cfg(tqec_plan *p) {
     SpecCfg c;
     c.glog = 0;
     c.threads = 256;
@@ -1453,8 +1465,10 @@ int spec_recheck_kernel(tqec_plan *p, int dev, SpecKernel **out) {
         c.slab = true;
         c.threads = 256;
     }
-    return spec_get(p, dev, c, out);
+    return c;
 }
+
+int spec_recheck_kernel(tqec_plan *p, int dev, SpecKernel **out) { return spec_get(p, dev, spec_recheck_cfg(p), out); }
 
 int spec_recheck(tqec_plan *p, DeviceData *dd, Workspace &w, SpecKernel *k, const uint8_t *d_syn, int64_t n,
                  double *d_mar, int32_t *d_cls, uint8_t *d_tie, hipStream_t stream) {
@@ -2270,18 +2284,34 @@ int tqec_map_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, 
         const auto key = std::make_tuple(dev, gl, threads, 0, 0, 1, 1 << 20);
         auto it = p->spec.find(key);
         if (it == p->spec.end()) {
+            // a plan whose generated kernel cannot be built (hiprtc or module
+            // load failure) falls back to the generic kernel for good; the
+            // reason is kept in tqec_last_error's thread-local text only until
+            // the next error, so it is also recorded on the plan
             std::vector<char> code;
-            if ((rc = spec_code(spec_map_source(cp, gl, threads), code))) return rc;
             SpecKernel k;
             k.lds = lds;
-            TQ_HIP(hipModuleLoadData(&k.mod, code.data()));
-            const hipError_t e = hipModuleGetFunction(&k.fn, k.mod, "tqec_map_spec");
-            if (e != hipSuccess) {
-                (void)hipModuleUnload(k.mod);
-                return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
+            std::string why;
+            if (spec_code(spec_map_source(cp, gl, threads), code)) {
+                why = "hiprtc: " + g_last_error;
+            } else {
+                hipError_t e = hipModuleLoadData(&k.mod, code.data());
+                if (e == hipSuccess) {
+                    e = hipModuleGetFunction(&k.fn, k.mod, "tqec_map_spec");
+                    if (e != hipSuccess) (void)hipModuleUnload(k.mod);
+                }
+                if (e != hipSuccess) why = std::string("module load: ") + hipGetErrorString(e);
             }
-            it = p->spec.emplace(key, k).first;
+            if (why.empty()) {
+                it = p->spec.emplace(key, k).first;
+            } else {
+                p->map_spec = false;
+                p->map_spec_note = why;
+            }
         }
+    }
+    if (p->map_spec) {
+        auto it = p->spec.find(std::make_tuple(dev, gl, threads, 0, 0, 1, 1 << 20));
         const unsigned char *a_syn = d_syn;
         long long a_n = (long long)n;
         const double *a_cp = dd->cpool64, *a_sp = dd->spool64;
@@ -2747,6 +2777,19 @@ int tqec_plan_set_map_kernel(tqec_plan *p, int32_t specialised) {
     if (p->cp.semiring != TQEC_SEMIRING_MAX) return set_error(TQEC_EINVAL, "not a max-product plan");
     std::lock_guard<std::mutex> lk(p->mu);
     p->map_spec = specialised != 0;
+    if (p->map_spec) p->map_spec_note.clear();
+    return TQEC_OK;
+}
+
+int tqec_plan_map_kernel(tqec_plan *p, int32_t *specialised, char *note, size_t cap) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (specialised) *specialised = p->map_spec ? 1 : 0;
+    if (note && cap > 0) {
+        const size_t k = std::min(cap - 1, p->map_spec_note.size());
+        std::memcpy(note, p->map_spec_note.data(), k);
+        note[k] = 0;
+    }
     return TQEC_OK;
 }
 

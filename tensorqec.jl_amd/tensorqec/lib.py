@@ -181,6 +181,12 @@ EXPORTS = [
                                            ctypes.c_int32]),
     ("tqec_plan_spec_config", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p, _i32p]),
     ("tqec_plan_set_map_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("tqec_plan_map_kernel", ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("tqec_ler_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    ("tqec_ler_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]),
+    ("tqec_ler_destroy", None, [ctypes.c_void_p]),
     ("tqec_last_error", ctypes.c_char_p, []),
 ]
 
@@ -343,6 +349,14 @@ class HipPlan:
     def set_map_kernel(self, specialised: bool) -> None:
         """Max-product plans: plan-specialised (default) or generic kernel."""
         check(load().tqec_plan_set_map_kernel(self._h, 1 if specialised else 0))
+
+    def map_kernel(self):
+        """(True if the plan-specialised max-product kernel is in use, reason
+        it was given up or '')."""
+        v = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(512)
+        check(load().tqec_plan_map_kernel(self._h, ctypes.byref(v), buf, 512))
+        return bool(v.value), buf.value.decode()
 
     def spec_config(self) -> dict:
         v = [ctypes.c_int32() for _ in range(4)]
@@ -553,3 +567,29 @@ def mixed_spec_source(plans, threads: int = 128) -> str:
     buf = ctypes.create_string_buffer(n.value + 1)
     check(load().tqec_mixed_spec_source(arr, len(plans), int(threads), buf, n.value + 1, ctypes.byref(n)))
     return buf.value.decode()
+
+
+class HipLER:
+    """Logical-error-rate loop on the device (``tqec_ler``): sample, decode,
+    recover, check against the logical operators, count."""
+
+    def __init__(self, plan: "HipPlan", sampler: "HipSampler", recovery: "HipRecovery", batch: int = 65536):
+        self._keep = (plan, sampler, recovery)
+        h = ctypes.c_void_p()
+        check(load().tqec_ler_create(plan.handle, sampler._h, recovery._h, int(batch), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, mode, seed: int, first: int, n: int, stream: int = 0) -> dict:
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        c = (ctypes.c_int64 * 4)()
+        check(load().tqec_ler_run(self._h, m, int(seed), int(first), int(n), c, _vp(stream)))
+        return {"samples": int(n), "logical_errors": c[0], "x_half": c[1], "z_half": c[2], "inconsistent": c[3]}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().tqec_ler_destroy(h)
+            except Exception:
+                pass
+            self._h = None

@@ -1,0 +1,95 @@
+"""Full-size oracle parity of the configurations the throughput benchmark
+runs, through the C ABI (reference call: `decode(ct, syndrome)`,
+src/decoding/tndecoder.jl:160-165, evaluated for every syndrome).
+
+  * d=7 rotated surface code, code-capacity depolarizing p in {0.02, 0.05,
+    0.08}: whole 65,536-syndrome batches through the register-resident
+    plan-specialised FP32 kernel in the benchmark's configurations (tree
+    seeds 11 and 14; subtrees on up to 6, 7 or 8 syndrome bits tabulated;
+    256 threads; 2 waves per SIMD), and through the exact FP64
+    register-resident kernel, against the CPU oracle's FP64 executor on
+    the same plan and the same syndromes.
+  * d=9 on 2,048 syndromes: every fused executor the plan fits.
+Tolerances: tests/_parity.py (FP64 1e-12 relative with exact zeros, tie
+flags equal; FP32 2e-5 of the row maximum; classes bit-exact on every
+non-tie syndrome).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from _parity import check_f32, check_f64
+from oracle.cexec import OraclePlan
+from tensorqec import decoders as D
+from tensorqec import lib as L
+from tensorqec.codes import CSSTannerGraph, SurfaceCode
+from tensorqec.errors import css_syndrome_rows, iid_error, sample_depolarizing
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(torch, ct, dS, n, mode):
+    dev = torch.device("cuda:0")
+    nout = 1 << ct.hip.n_out
+    mar = torch.full((n, nout), -1.0, dtype=torch.float64, device=dev)
+    cls = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    tie = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    ct.hip.decode_device(dS.data_ptr(), n, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr())
+    torch.cuda.synchronize()
+    return mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy()
+
+
+@pytest.mark.parametrize("p,seed,tab", [(0.02, 11, 7), (0.05, 11, 7), (0.08, 11, 7),
+                                        (0.05, 14, 6), (0.05, 11, 8)])
+def test_d7_benchmark_configuration_full_batch(p, seed, tab):
+    torch = pytest.importorskip("torch")
+    n = 65536
+    say(f"case d7 full batch p={p} tree seed {seed} tabulate {tab}")
+    t = CSSTannerGraph.from_code(SurfaceCode(7, 7))
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="spec_f32", optimizer=D.TreeSA(seed=seed), tabulate_bits=tab), t, em)
+    assert ct.hip.stats()["tabulate_bits"] == tab
+    S = css_syndrome_rows(sample_depolarizing(em, seed=int(p * 1e4) + 31 * seed + tab, count=n), t)
+    ref, _, _ = OraclePlan(ct.plan).decode(S)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+    ct.hip.set_spec(64, 256, False, 2, reg=True)
+    cfg = ct.hip.spec_config()
+    assert cfg["reg"] and cfg["threads"] == 256 and cfg["waves"] == 2
+    m, c, _ = run(torch, ct, dS, n, "spec_f32")
+    err, ties = check_f32(m, c, ref, f"spec_f32 p={p}")
+    say(f"  spec_f32 (register-resident): max err {err:.2e} of the row maximum, {ties} ties, "
+        f"{ct.hip.last_recheck_count()} FP64 re-checks")
+    m, c, tie = run(torch, ct, dS, n, "spec_f64")
+    err = check_f64(m, c, tie, ref, f"spec_f64 p={p}")
+    say(f"  spec_f64: max rel err {err:.2e}")
+
+
+def test_d9_vs_oracle():
+    torch = pytest.importorskip("torch")
+    n = 2048
+    t = CSSTannerGraph.from_code(SurfaceCode(9, 9))
+    p = 0.05
+    em = iid_error(p / 3, p / 3, p / 3, t.nq)
+    ct = D.compile(D.TNMMAP(mode="spec_f32"), t, em)
+    S = css_syndrome_rows(sample_depolarizing(em, seed=909, count=n), t)
+    ref, _, _ = OraclePlan(ct.plan).decode(S)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+    for mode in ("spec_f32", "spec_f64", "fused_f32", "fused_f64"):
+        say(f"case d9 {mode}")
+        try:
+            m, c, tie = run(torch, ct, dS, n, mode)
+        except L.TqecError as e:
+            assert e.code == L.TQEC_EUNSUPPORTED and mode.startswith("fused")
+            say(f"  {mode}: plan does not fit the generic fused executor")
+            continue
+        if mode.endswith("f32"):
+            err, ties = check_f32(m, c, ref, f"d9 {mode}")
+        else:
+            err = check_f64(m, c, tie, ref, f"d9 {mode}")
+        say(f"  max err {err:.2e}")

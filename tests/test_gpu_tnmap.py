@@ -89,18 +89,21 @@ def test_networks_agree_and_configs_are_valid(code, p):
 
 @pytest.mark.parametrize("code,p,fac", [(SurfaceCode(3, 3), 0.1, True), (SurfaceCode(5, 5), 0.05, True),
                                         (SurfaceCode(5, 5), 0.05, False), (ToricCode(3, 3), 0.08, True),
-                                        (SurfaceCode(7, 7), 0.05, True)])
+                                        (SurfaceCode(7, 7), 0.05, True), (SurfaceCode(9, 9), 0.05, True)])
 def test_specialised_map_kernel_bit_identical_to_generic(code, p, fac):
     """The max-product kernel generated for the plan performs the generic
     kernel's comparisons in the same order and the same backtrace: identical
     configurations (tie-breaking included) and log-maxima."""
     t = CSSTannerGraph.from_code(code)
     em = iid_error(p / 3, p / 3, p / 3, t.nq)
-    S = css_syndrome_rows(sample_depolarizing(em, 77 + t.nq, 3000), t)
+    S = css_syndrome_rows(sample_depolarizing(em, 77 + t.nq, 3000 if t.nq < 81 else 1000), t)
     ct = compile_tnmap(TNMAP(factorize=fac), t, em)
     ct.hip.set_map_kernel(False)
+    assert ct.hip.map_kernel() == (False, "")
     cfg0, lp0 = decode_tnmap_batch(ct, S)
     ct.hip.set_map_kernel(True)
     cfg1, lp1 = decode_tnmap_batch(ct, S)
+    # the generated kernel was built and used (no fall-back to the generic one)
+    assert ct.hip.map_kernel() == (True, "")
     assert np.array_equal(cfg0, cfg1)
     assert np.array_equal(lp0, lp1)
