@@ -645,6 +645,8 @@ def main():
             pinned["tabulate_bits"] = int(f[8])
         if len(f) > 9:
             pinned["reg"] = bool(int(f[9]))
+        if len(f) > 10:
+            pinned["dyn"] = bool(int(f[10]))
     if pinned:
         cands = [(pinned["seed"], pinned["sc_target"], pinned.get("tabulate_bits", 4))]
     else:
@@ -684,7 +686,7 @@ def main():
             if pinned["mode"] == "spec_f32":
                 ct.hip.set_spec(pinned["tile"], pinned["threads"], pinned.get("tables_in_lds", False),
                                 pinned.get("waves", 0), pinned.get("slab", False), pinned.get("unroll", 1),
-                                pinned.get("reg", False))
+                                pinned.get("reg", False), pinned.get("dyn", False))
                 cfg.update(ct.hip.spec_config())
             else:
                 ct.hip.set_tile("fused_f32", pinned["tile"], pinned["threads"])

@@ -72,8 +72,12 @@ extern "C" {
 #define TQEC_MODE_SPEC_F32 4  /* FUSED_F32 with a kernel generated for the
                                  plan and compiled at first use (hiprtc):
                                  constant index arithmetic, unrolled
-                                 contractions, tables in LDS; results are
-                                 bit-identical to FUSED_F32               */
+                                 contractions, tables in LDS or registers;
+                                 LDS-tile kernels are bit-identical to
+                                 FUSED_F32, the register-resident kernel
+                                 (default where the plan fits) uses
+                                 plan-time static exponents, see
+                                 TQEC_SPEC_REG                            */
 #define TQEC_MODE_SPEC_F64 5  /* FUSED_F64 with the plan-specialised kernel;
                                  bit-identical to FUSED_F64               */
 #define TQEC_MODE_SPEC_BF16 6 /* SPEC_F32 with one syndrome per tile and the
@@ -306,8 +310,17 @@ void tqec_mixed_destroy(tqec_mixed *m);
                                       barriers); tile = threads / 4; FP32 (default
                                       SPEC_F32 kernel) or, with TQEC_SPEC_FP64 in
                                       tqec_plan_spec_source, FP64 (default SPEC_F64
-                                      kernel); results bit-identical to the other
-                                      kernels of the same precision                 */
+                                      kernel).  FP32: plan-time static exponents
+                                      (no per-step maxima or rescales; syndromes
+                                      whose largest marginal falls below the
+                                      plan's underflow bound are re-checked in
+                                      FP64), within the FP32 tolerance of the
+                                      other kernels; FP64: bit-identical to the
+                                      other FP64 kernels                            */
+#define TQEC_SPEC_DYNSCALE 32      /* with TQEC_SPEC_REG in FP32: per-step scaling
+                                      from each operand's maximum instead, the
+                                      generic FP32 kernel's arithmetic
+                                      (bit-identical to TQEC_MODE_FUSED_F32)        */
 
 /* Source of the plan-specialised kernel for a tile configuration
  * (inspection and offline compilation checks); `flags` as above, with the
@@ -318,7 +331,7 @@ int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, 
                           size_t cap, size_t *len);
 
 /* Pin / read the TQEC_MODE_SPEC_F32 configuration: tile, threads,
- * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB | TQEC_SPEC_REG flags, and `waves` = minimum
+ * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB | TQEC_SPEC_REG | TQEC_SPEC_DYNSCALE flags, and `waves` = minimum
  * waves per SIMD requested from the compiler (bits 0-7, 0 = none) | unroll
  * of the per-lane output loop << 8 (1, 2, 4 or 8; 0 = 1).  No reference
  * counterpart. */

@@ -10,6 +10,7 @@
 #include <array>
 #include <cstdio>
 #include <cctype>
+#include <cmath>
 #include <sstream>
 #include <vector>
 
@@ -699,12 +700,154 @@ bool reg_fetch(const OperandDesc &op, const RLay &LC, const RLay &LX, int p, uin
     return true;
 }
 
-// One step of the register-resident kernel.  FP32: per-tensor power-of-two
-// scaling (the generic FP32 kernel's); FP64 (`fp64`): no scaling, the
-// constant exponents folded out of table data restored per entry (the
-// generic FP64 kernel's).
+// Plan-time scaling of the register-resident FP32 kernel (static mode).
+//
+// Every entry of every table is non-negative, so contracting the plan in
+// FP64 with each sliced table replaced by the sum of its variants (the
+// evidence of all syndromes at once) bounds every intermediate entry of every
+// syndrome from above (ub).  The kernel stores intermediate X as
+// X_true * 2^-E_X, E_X being the sum of the folded exponents of X's tables
+// plus any static shifts below it.  A step shifts its output (one multiply
+// per output register) only where its pre-shift bound
+// log2 max ub - (E_a + E_b) would leave [kShiftLo, kShiftHi], so every value
+// and every partial sum stays below 2^127 whatever the syndrome, and the
+// plan is refused (the dynamic mode is used instead) when a bound cannot be
+// kept below kOverflow.
+//
+// Underflow: an operation whose result drops below FP32's normal range errs
+// by at most 2^-126 of its unit 2^E; carried to the root by the environment
+// bound (env: the adjoint of the ub contraction, again over all syndromes),
+// all of them together err by at most errsum in absolute terms (true
+// units), also counting the FP32 rounding of the tables.  A syndrome whose
+// largest root entry is below 2^theta, theta = log2 errsum + kErrBits, is
+// re-checked in FP64, so accepted marginals err by at most 2^-kErrBits of
+// the row maximum from underflow: one comparison at the root replaces the
+// per-step maxima, exponents and rescales of the dynamic mode.
+struct RegScale {
+    std::vector<int> shift;   // per step: output scaled by 2^-shift
+    std::vector<int> E;       // per step: exponent of the stored values
+    std::vector<double> pre;  // per step: pre-shift bound, log2
+    int e_root = 0;
+    double theta = 0;         // log2 of the smallest root maximum accepted without re-check
+};
+
+constexpr double kShiftHi = 60, kShiftLo = -30, kOverflow = 120, kErrBits = 21;
+
+// True values (in memory order) of a table operand, summed over its
+// variants for a sliced one.
+std::vector<double> table_bound(const CompiledPlan &cp, const OperandDesc &op) {
+    const size_t n = size_t(1) << op.rank;
+    std::vector<double> v(n, 0.0);
+    const double sc = std::ldexp(1.0, op.cexp);
+    if (op.kind == kConst) {
+        for (size_t i = 0; i < n; ++i) v[i] = cp.const_pool[(size_t)op.off + i] * sc;
+    } else {
+        for (uint32_t var = 0; var < (1u << op.nev); ++var)
+            for (size_t i = 0; i < n; ++i) v[i] += cp.sliced_pool[(size_t)op.off + ((size_t)var << op.rank) + i] * sc;
+    }
+    return v;
+}
+
+bool reg_static_scales(const CompiledPlan &cp, const RegPlan &rp, RegScale &rs, std::string &err) {
+    const int ns = (int)cp.steps.size();
+    std::vector<std::vector<double>> ub(ns), env(ns);
+    std::vector<std::array<std::vector<double>, 2>> opv(ns);
+    std::vector<std::array<int, 2>> opE(ns);
+    rs.shift.assign(ns, 0);
+    rs.E.assign(ns, 0);
+    rs.pre.assign(ns, 0);
+    auto lg = [](const std::vector<double> &v) {
+        double m = 0;
+        for (double x : v) m = std::max(m, x);
+        return m > 0 ? std::log2(m) : -HUGE_VAL;
+    };
+    // forward: upper bounds and exponents
+    for (int s = 0; s < ns; ++s) {
+        const StepDesc &st = cp.steps[s];
+        for (int side = 0; side < 2; ++side) {
+            const OperandDesc &op = side ? st.b : st.a;
+            const int p = rp.prod[side][s];
+            opv[s][side] = p >= 0 ? ub[p] : table_bound(cp, op);
+            opE[s][side] = p >= 0 ? rs.E[p] : op.cexp;
+            const double single = lg(opv[s][side]) - opE[s][side] + op.nsingle;
+            if (single > kOverflow) return err = "operand bound too large for static scaling", false;
+        }
+        const OperandDesc &A = st.a, &B = st.b;
+        const std::vector<double> &va = opv[s][0], &vb = opv[s][1];
+        std::vector<double> &c = ub[s];
+        c.assign(size_t(1) << st.rc, 0.0);
+        for (uint32_t ci = 0; ci < (1u << st.rc); ++ci) {
+            const uint32_t ia = gather_host(ci, A), ib = gather_host(ci, B);
+            double acc = 0;
+            for (uint32_t k = 0; k < (1u << st.nk); ++k) {
+                double sa = 0, sb = 0;
+                for (uint32_t q = 0; q < (1u << A.nsingle); ++q) sa += va[ia | (q << A.sshift) | (k << A.kshift)];
+                for (uint32_t q = 0; q < (1u << B.nsingle); ++q) sb += vb[ib | (q << B.sshift) | (k << B.kshift)];
+                acc += sa * sb;
+            }
+            c[ci] = acc;
+        }
+        const int eab = opE[s][0] + opE[s][1];
+        const double pre = lg(c) - eab;
+        if (!(pre < HUGE_VAL)) return err = "non-finite bound", false;
+        rs.pre[s] = pre;
+        int sh = 0;
+        if (pre > kShiftHi || (pre < kShiftLo && pre > -HUGE_VAL)) sh = (int)std::floor(pre);
+        if (pre > kOverflow) return err = "step bound too large for static scaling", false;
+        rs.shift[s] = sh;
+        rs.E[s] = eab + sh;
+    }
+    // adjoint: environment bounds, and the underflow error they carry
+    env[rp.root].assign(size_t(1) << cp.steps[rp.root].rc, 1.0);
+    double errsum = 0;
+    for (int s = ns - 1; s >= 0; --s) {
+        const StepDesc &st = cp.steps[s];
+        const OperandDesc &A = st.a, &B = st.b;
+        const std::vector<double> &va = opv[s][0], &vb = opv[s][1], &ec = env[s];
+        std::vector<double> ea(va.size(), 0.0), eb(vb.size(), 0.0);
+        double envsum = 0;
+        for (uint32_t ci = 0; ci < (1u << st.rc); ++ci) {
+            envsum += ec[ci];
+            const uint32_t ia = gather_host(ci, A), ib = gather_host(ci, B);
+            for (uint32_t k = 0; k < (1u << st.nk); ++k) {
+                double sa = 0, sb = 0;
+                for (uint32_t q = 0; q < (1u << A.nsingle); ++q) sa += va[ia | (q << A.sshift) | (k << A.kshift)];
+                for (uint32_t q = 0; q < (1u << B.nsingle); ++q) sb += vb[ib | (q << B.sshift) | (k << B.kshift)];
+                for (uint32_t q = 0; q < (1u << A.nsingle); ++q) ea[ia | (q << A.sshift) | (k << A.kshift)] += ec[ci] * sb;
+                for (uint32_t q = 0; q < (1u << B.nsingle); ++q) eb[ib | (q << B.sshift) | (k << B.kshift)] += ec[ci] * sa;
+            }
+        }
+        // this step's operations: per output entry 2^nk products and the
+        // single-side sums, each within 2^-126 of its unit when it underflows
+        const double ops = std::ldexp(1.0, st.nk) * (1.0 + std::ldexp(1.0, A.nsingle) + std::ldexp(1.0, B.nsingle));
+        const int unit = opE[s][0] + opE[s][1] + std::max(rs.shift[s], 0);
+        errsum += ops * envsum * std::ldexp(1.0, unit - 126);
+        for (int side = 0; side < 2; ++side) {
+            const std::vector<double> &e = side ? eb : ea;
+            const int p = rp.prod[side][s];
+            if (p >= 0) {
+                env[p] = e;
+            } else {
+                // FP32 rounding of the table's entries below the normal range
+                double es = 0;
+                for (double x : e) es += x;
+                errsum += es * std::ldexp(1.0, opE[s][side] - 126);
+            }
+        }
+    }
+    if (!std::isfinite(errsum)) return err = "non-finite error bound", false;
+    rs.e_root = rs.E[rp.root];
+    rs.theta = errsum > 0 ? std::log2(errsum) + kErrBits : -HUGE_VAL;
+    return true;
+}
+
+// One step of the register-resident kernel.  FP32 with `rs`: the static
+// exponents and shifts of reg_static_scales; FP32 without: per-tensor
+// power-of-two scaling from each operand's maximum (the generic FP32
+// kernel's); FP64 (`fp64`): no scaling, the constant exponents folded out of
+// table data restored per entry (the generic FP64 kernel's).
 bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan &rp, int s, size_t const_n,
-                   bool fp64, std::string &err) {
+                   bool fp64, const RegScale *rs, std::string &err) {
     const StepDesc &st = cp.steps[s];
     const RLay &LC = rp.lay[s];
     const std::vector<int> RB = reg_bits(LC, st.rc);
@@ -712,14 +855,18 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
     const uint32_t nkk = 1u << st.nk;
     const char *T = fp64 ? "double" : "float";
     const char *zero = fp64 ? "0.0" : "0.f";
-    if (!fp64) o << "    float m" << s << ";\n    int e" << s << ";\n";
+    const bool dyn = !fp64 && !rs;
+    if (dyn) o << "    float m" << s << ";\n    int e" << s << ";\n";
     o << "    " << T;
     for (uint32_t r = 0; r < nreg; ++r) o << (r ? ", " : " ") << "x" << s << "_" << r;
     o << ";\n";
     o << "    { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << ", " << nreg
-      << " registers per lane\n";
+      << " registers per lane";
+    if (rs) o << "; bound 2^" << (int)std::ceil(rs->pre[s]) << " before a shift of " << rs->shift[s] << ", exponent "
+              << rs->E[s];
+    o << "\n";
     const char *ab[2] = {"a", "b"};
-    for (int side = 0; side < 2 && !fp64; ++side) {
+    for (int side = 0; side < 2 && dyn; ++side) {
         const OperandDesc &op = side ? st.b : st.a;
         const int p = rp.prod[side][s];
         if (op.kind == kInter)
@@ -728,8 +875,8 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
         else
             o << "      const int n" << ab[side] << " = 0, e" << ab[side] << " = " << op.cexp << ";\n";
     }
-    if (!fp64) o << "      const int shift = -(na + nb);\n";
-    if (!fp64) {
+    if (dyn) o << "      const int shift = -(na + nb);\n";
+    if (dyn) {
         std::string cond;
         if (st.a.kind == kInter) cond += "na < -100";
         if (st.b.kind == kInter) cond += std::string(cond.empty() ? "" : " || ") + "nb < -100";
@@ -838,6 +985,18 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
         o << "    }\n";
         return true;
     }
+    if (rs) {
+        const int sh = rs->shift[s];
+        for (uint32_t r = 0; r < nreg; ++r) {
+            o << "      x" << s << "_" << r << " = ";
+            if (sh != 0)
+                o << "ldexpf(acc" << r << ", " << -sh << ");\n";
+            else
+                o << "acc" << r << ";\n";
+        }
+        o << "    }\n";
+        return true;
+    }
     // Entries are >= 0 or NaN, and NaN-ignoring fmaxf over them equals the
     // generic kernel's running `x > m ? x : m` from 0 (max is exact and order
     // free); a tree of three-input maxima halves the instructions.
@@ -886,6 +1045,18 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
     std::ostringstream o;
     o << "// plan-specialised " << (fp64 ? "FP64" : "FP32")
       << " TNMMAP kernel, one syndrome per quad of lanes, tensors in registers (generated)\n";
+    RegScale rsc;
+    bool stat = false;
+    if (!fp64 && !cfg.dyn) {
+        std::string why;
+        stat = reg_static_scales(cp, rp, rsc, why);
+        if (stat)
+            o << "// static exponents: root exponent " << rsc.e_root << ", FP64 re-check below 2^"
+              << (int)std::ceil(rsc.theta) << " (true units)\n";
+        else
+            o << "// static scaling not applicable (" << why << "): per-step scaling\n";
+    }
+    const bool dyn = !fp64 && !stat;
     if (fp64)
         o << "#define QP(x, c) __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), (c), 0xf, 0xf, false), "
              "__builtin_amdgcn_mov_dpp(__double2loint(x), (c), 0xf, 0xf, false))\n";
@@ -933,9 +1104,9 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
                 for (int j = 0; j < op->nev; ++j) used[op->ev_bit[j]] = 1;
     for (int j = 0; j < cp.n_syn; ++j)
         if (used[j]) o << "    const unsigned s" << j << " = valid ? (unsigned)(row[" << j << "] & 1u) : 0u;\n";
-    if (!fp64) o << "    int lowf = 0;\n";
+    if (dyn) o << "    int lowf = 0;\n";
     for (int s = 0; s < (int)cp.steps.size(); ++s)
-        if (!emit_reg_step(o, cp, rp, s, const_n, fp64, err)) return "";
+        if (!emit_reg_step(o, cp, rp, s, const_n, fp64, stat ? &rsc : nullptr, err)) return "";
     // every lane gathers the root's entries; the quad's lane 0 finishes the row
     const RLay &LR = rp.lay[rp.root];
     const std::vector<int> RR = reg_bits(LR, cp.steps[rp.root].rc);
@@ -954,7 +1125,8 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
         o << "    rv[" << j << "] = " << (ctrl == kIdentityCtrl ? nm : "QP(" + nm + ", " + std::to_string(ctrl) + ")") << ";\n";
     }
     o << "    if (q == 0 && valid) {\n"
-         "      const int e = " << (fp64 ? std::string("0") : "e" + std::to_string(rp.root)) << ";\n"
+         "      const int e = "
+      << (fp64 ? std::string("0") : stat ? std::to_string(rsc.e_root) : "e" + std::to_string(rp.root)) << ";\n"
          "      " << T << " best = 0, second = -1;\n"
          "      int bi = 0;\n"
          "      for (int j = 0; j < NOUT; ++j) {\n"
@@ -971,9 +1143,17 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
          "      if (cls) cls[b] = bi;\n"
          "      const bool low = NOUT > 1 && second >= 0 && (double)(best - second) <= tie_tol * (double)best;\n"
          "      if (ties) ties[b] = low ? 1 : 0;\n";
+    std::string low = "lowf != 0";
+    if (stat) {
+        // the root maximum in stored units against 2^(theta - e_root)
+        const double lim = std::ceil(rsc.theta - rsc.e_root);
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), "%.17g", lim < -1074 ? 0.0 : std::ldexp(1.0, (int)std::min(lim, 1000.0)));
+        low = std::string("(double)best < ") + buf;
+    }
     if (!fp64)
         o << "      if (rl) {\n"
-             "        const bool rc = !(best > 0) || lowf != 0 ||\n"
+             "        const bool rc = !(best > 0) || " << low << " ||\n"
              "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
              "        if (rc) {\n"
              "          const int slot = atomicAdd(&seg_n, 1);\n"

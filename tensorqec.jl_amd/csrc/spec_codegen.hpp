@@ -9,8 +9,10 @@
 // offsets as immediates, tables staged in LDS), compiled at run time with
 // hiprtc for gfx950.  The arithmetic — order of every sum, fma, scale and
 // comparison — is the generic FP32 kernel's, so both give bit-identical
-// results.  The FP64 variant (no scaling) likewise matches the generic FP64
-// fused kernel and serves the FP64 re-check and TQEC_MODE_SPEC_F64.
+// results, except for the register-resident FP32 kernel by default: it
+// replaces the per-step scaling by plan-time static exponents (reg_source).
+// The FP64 variant (no scaling) matches the generic FP64 fused kernel and
+// serves the FP64 re-check and TQEC_MODE_SPEC_F64.
 #pragma once
 #include <stddef.h>
 
@@ -31,13 +33,15 @@ struct SpecCfg {
     int unroll = 1;      // output entries per lane in flight (unroll of the output loop)
     bool mfma = false;   // FP32 tiles of one syndrome: GEMM-shaped steps on BF16 MFMA (FP32 accumulation)
     bool reg = false;    // FP32, one syndrome per quad of lanes, every tensor in registers (glog = log2(threads / 4))
+    bool dyn = false;    // reg FP32: per-step scaling from operand maxima (bit-identical to the generic kernel)
+                         // instead of plan-time static exponents
 };
 
 // Register-resident variant (cfg.reg): each syndrome is owned by a quad of
 // lanes; a tensor's entries are spread over the quad by two of its index bits
 // and over registers by the rest, operands cross lanes with DPP quad
-// permutations and per-tensor maxima with two DPP steps, so the step program
-// touches no LDS and needs no barrier.  Returns false (with the reason) when
+// permutations (and, with cfg.dyn, per-tensor maxima with two DPP steps), so
+// the step program touches no LDS and needs no barrier.  Returns false (with the reason) when
 // the plan does not suit it: more than 16 classes, or more live tensor entries
 // per lane than the register file holds.
 bool spec_reg_supported(const CompiledPlan &cp, std::string *why = nullptr);

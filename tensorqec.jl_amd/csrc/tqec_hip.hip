@@ -1368,7 +1368,8 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
 
 std::tuple<int, int, int, int, int, int, int> spec_key(int dev, const SpecCfg &cfg) {
     return std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
-                           (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.reg ? 4 : 0) | (cfg.unroll << 3));
+                           (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.reg ? 4 : 0) | (cfg.unroll << 3) |
+                               (cfg.dyn ? 1 << 8 : 0));
 }
 
 // The compiled kernel of a configuration on a device, or nullptr when it has
@@ -1422,17 +1423,17 @@ size_t spec_slab_need(const tqec_plan *p, const SpecKernel *k, int64_t n) {
 }
 
 // Launch a specialised kernel over n syndromes (or the listed ones).
-int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n,
-                const int32_t *in_list, const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie,
-                int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream,
-                int64_t nseg = 0, int32_t segcap = 0) {
+int spec_launch(tqec_plan *p, void *slab, size_t slab_bytes, SpecKernel *k, DeviceData *dd, const uint8_t *d_syn,
+                int64_t n, const int32_t *in_list, const int32_t *in_count, double *d_mar, int32_t *d_cls,
+                uint8_t *d_tie, int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol,
+                hipStream_t stream, int64_t nseg = 0, int32_t segcap = 0) {
     const int64_t ntiles = nseg > 0 ? nseg : (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
     void *a_slab = nullptr;
     if (k->cfg.slab) {
         // the caller sized the slab before enqueueing anything that uses it
-        if (spec_slab_need(p, k, n) > w.spec_slab_bytes) return set_error(TQEC_EINVAL, "internal: slab too small");
-        a_slab = w.spec_slab;
+        if (spec_slab_need(p, k, n) > slab_bytes) return set_error(TQEC_EINVAL, "internal: slab too small");
+        a_slab = slab;
     }
     const unsigned char *a_syn = d_syn;
     long long a_n = (long long)n;
@@ -1449,6 +1450,15 @@ int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const
                     &a_slab, &a_nseg, &a_segcap};
     TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
     return TQEC_OK;
+}
+
+// The same with the slab of a (plan, stream) workspace.
+int spec_launch(tqec_plan *p, Workspace &w, SpecKernel *k, DeviceData *dd, const uint8_t *d_syn, int64_t n,
+                const int32_t *in_list, const int32_t *in_count, double *d_mar, int32_t *d_cls, uint8_t *d_tie,
+                int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, hipStream_t stream,
+                int64_t nseg = 0, int32_t segcap = 0) {
+    return spec_launch(p, w.spec_slab, w.spec_slab_bytes, k, dd, d_syn, n, in_list, in_count, d_mar, d_cls, d_tie, rl,
+                       rcount, rcap, tie_tol, recheck_tol, stream, nseg, segcap);
 }
 
 // FP64 re-check kernel: one syndrome per 256-thread workgroup (the list is
@@ -2708,6 +2718,7 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     c.slab = (flags & 4) != 0;
     c.mfma = (flags & 8) != 0;
     c.reg = (flags & 16) != 0;
+    c.dyn = (flags & 32) != 0;
     c.unroll = std::max(1, (flags >> 8) & 0xff);
     if (c.reg && (c.mfma || c.slab || tile * 4 != threads))
         return set_error(TQEC_EINVAL, "register-resident kernels run without slab or MFMA, tile = threads / 4");
@@ -2753,6 +2764,7 @@ int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t flag
     c.slab = (flags & 4) != 0;
     c.mfma = (flags & 8) != 0;
     c.reg = (flags & 16) != 0;
+    c.dyn = (flags & 32) != 0;
     c.waves = waves;
     c.unroll = unroll ? unroll : 1;
     if (c.reg && (c.mfma || c.slab || tile * 4 != threads))
@@ -2805,7 +2817,9 @@ int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t
     if (p->spec_cfg.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the specialised kernel");
     if (tile) *tile = 1 << p->spec_cfg.glog;
     if (threads) *threads = p->spec_cfg.threads;
-    if (flags) *flags = (p->spec_cfg.tab_lds ? 1 : 0) | (p->spec_cfg.slab ? 4 : 0) | (p->spec_cfg.reg ? 16 : 0);
+    if (flags)
+        *flags = (p->spec_cfg.tab_lds ? 1 : 0) | (p->spec_cfg.slab ? 4 : 0) | (p->spec_cfg.reg ? 16 : 0) |
+                 (p->spec_cfg.dyn ? 32 : 0);
     if (waves) *waves = p->spec_cfg.waves | (p->spec_cfg.unroll << 8);
     return TQEC_OK;
 }

@@ -150,3 +150,29 @@ def test_dem_error_pattern_rule():
     # the parity unchanged; the reference's own test (tndecoder.jl:112-120)
     # only checks syndrome consistency, which the rule can break.
     assert mask.sum() == 6
+
+
+def test_factorized_dem_refuses_single_member_checks():
+    """tndecoder.jl:221-238 chains every check that does not have exactly two
+    members through c[2], so a detector (or logical) flipped by a single
+    mechanism fails there under `factorize`; the network builder refuses the
+    same DEM with a ValueError naming the check, and builds it without
+    factorization (one parity tensor per check), whose contraction still
+    matches the brute-force marginal."""
+    from tensorqec.network import tnmmap_dem_network
+    text = ("error(0.1) D0 D1\nerror(0.2) D1 D2\nerror(0.05) D2 L0\nerror(0.3) D0 D2\n"
+            "error(0.15) D3\nerror(0.25) D1 L0\n")
+    dem = parse_dem_string(text)
+    with pytest.raises(ValueError, match="detector 4 has 1 member"):
+        tnmmap_dem_network(dem, factorize=True)
+    g = tnmmap_dem_network(dem, factorize=False)
+    assert len(g.ixs) == 4 + 1 + 6 + 4          # checks, logical, priors, evidence
+    ct = D.compile(D.TNMMAP(factorize=False), dem)
+    ns = 4
+    rows = np.array([[(s >> j) & 1 for j in range(ns)] for s in range(1 << ns)], dtype=np.uint8)
+    bf = O.brute_force_dem(*O.parse_dem_text(text))
+    got, _, _ = OraclePlan(ct.plan).decode(rows)
+    for r, row in zip(got, rows):
+        want = bf.get(tuple(int(x) for x in row), np.zeros(2))
+        assert np.allclose(r, want, rtol=1e-12, atol=1e-300)
+    assert abs(got.sum() - 1.0) < 1e-12

@@ -33,7 +33,7 @@ def say(*a):
 
 
 @pytest.mark.parametrize("code,p,mode", [(SurfaceCode(5, 5), 0.05, "spec_f32"), (SurfaceCode(7, 7), 0.08, "spec_f32"),
-                                         (ToricCode(3, 3), 0.06, "fused_f64")])
+                                         (ToricCode(3, 3), 0.06, "plain_f64")])
 def test_css_ler_loop_matches_host(code, p, mode):
     t = CSSTannerGraph.from_code(code)
     em = iid_error(p / 3, p / 3, p / 3, t.nq)

@@ -1,10 +1,14 @@
 """Plan-specialised kernels (TQEC_MODE_SPEC_F32 / SPEC_F64) against the
 generic fused kernels of the same precision on the same syndromes:
 marginals, classes and tie flags must be bit-identical — the generated kernel performs the same operations in
-the same order, with the descriptor values substituted.  Several tile
-configurations and the list (gated fall-back) entry point are covered.  Deep
-tabulation, plans too wide for LDS and the BF16-MFMA mode are also checked
-against the CPU oracle's FP64 contraction of the same plan (tolerances in
+the same order, with the descriptor values substituted.  This holds for the
+LDS-tile kernels, the FP64 kernels and the register-resident FP32 kernel
+with per-step scaling (TQEC_SPEC_DYNSCALE).  The register-resident FP32
+kernel's default, plan-time static exponents, changes the arithmetic: it is
+checked against the CPU oracle's FP64 contraction of the same plan on the
+same syndromes instead.  Several tile configurations and the list (gated
+fall-back) entry point are covered.  Deep tabulation, plans too wide for LDS
+and the BF16-MFMA mode are also checked against the oracle (tolerances in
 tests/_parity.py; BF16 steps 3e-2 of the row maximum)."""
 import sys
 
@@ -65,14 +69,13 @@ def test_spec_bit_identical_to_generic(name, code, p, n):
         got64 = run(torch, ct, dS, n, "spec_f64")
         for a, b in zip(got64, ref64):
             assert np.array_equal(a, b), (name, "f64")
-    configs = [None, (1, 64, 1), (4, 128, 1), (16, 256, 1), (4, 64, 2), (2, 64, 4), (8, 256, 4)]
+    configs = [(1, 64, 1), (4, 128, 1), (16, 256, 1), (4, 64, 2), (2, 64, 4), (8, 256, 4)]
     for cfg in configs:
-        if cfg is not None:
-            try:
-                ct.hip.set_spec(cfg[0], cfg[1], False, 0, False, cfg[2])
-            except L.TqecError as e:
-                assert e.code == L.TQEC_EUNSUPPORTED
-                continue
+        try:
+            ct.hip.set_spec(cfg[0], cfg[1], False, 0, False, cfg[2])
+        except L.TqecError as e:
+            assert e.code == L.TQEC_EUNSUPPORTED
+            continue
         say(f"  tile {cfg}")
         got = run(torch, ct, dS, n, "spec_f32")
         for a, b in zip(got, ref):
@@ -86,9 +89,12 @@ def test_spec_bit_identical_to_generic(name, code, p, n):
                                            ("toric3", ToricCode(3, 3), 0.07, 65)])
 def test_spec_reg_bit_identical_to_generic(name, code, p, n):
     """Register-resident kernel (one syndrome per quad of lanes, DPP
-    exchanges): same operations in the same order as the generic FP32
-    kernel, so bit-identical, for every thread count, table placement and
-    register cap; also through the list entry point."""
+    exchanges).  With per-step scaling: the same operations in the same
+    order as the generic FP32 kernel, so bit-identical, with the same FP64
+    re-checks, for every thread count, table placement and register cap.
+    With static exponents (the default): within the FP32 tolerance of the
+    CPU oracle's FP64 contraction, classes equal on every non-tie syndrome.
+    Also through the list entry point."""
     torch = pytest.importorskip("torch")
     say(f"case spec reg {name} n={n}")
     t = CSSTannerGraph.from_code(code)
@@ -103,21 +109,31 @@ def test_spec_reg_bit_identical_to_generic(name, code, p, n):
         assert e.code == L.TQEC_EUNSUPPORTED
         pytest.skip(f"{name}: plan does not fit the fused executors")
     ref_rechecks = ct.hip.last_recheck_count()
+    ref_generic = ref
+    m64, _, _ = OraclePlan(ct.plan).decode(S)
     done = 0
     for thr, tab, wv in ((64, False, 0), (128, True, 0), (256, True, 0), (256, False, 2)):
-        try:
-            ct.hip.set_spec(thr // 4, thr, tab, wv, reg=True)
-        except L.TqecError as e:
-            assert e.code == L.TQEC_EUNSUPPORTED
-            continue
-        assert ct.hip.spec_config()["reg"]
-        say(f"  threads {thr} tables_in_lds {tab} waves {wv}")
-        got = run(torch, ct, dS, n, "spec_f32")
-        for a, b in zip(got, ref):
-            assert np.array_equal(a, b), (name, thr, tab, wv)
-        # the same syndromes were re-checked in FP64 (per-tile re-check segments)
-        assert ct.hip.last_recheck_count() == ref_rechecks, (name, thr, ref_rechecks)
-        done += 1
+        for dyn in (True, False):
+            try:
+                ct.hip.set_spec(thr // 4, thr, tab, wv, reg=True, dyn=dyn)
+            except L.TqecError as e:
+                assert e.code == L.TQEC_EUNSUPPORTED
+                continue
+            cfg = ct.hip.spec_config()
+            assert cfg["reg"] and cfg["dyn"] == dyn
+            say(f"  threads {thr} tables_in_lds {tab} waves {wv} per-step scaling {dyn}")
+            got = run(torch, ct, dS, n, "spec_f32")
+            if dyn:
+                for a, b in zip(got, ref_generic):
+                    assert np.array_equal(a, b), (name, thr, tab, wv)
+                # the same syndromes were re-checked in FP64 (per-tile re-check segments)
+                assert ct.hip.last_recheck_count() == ref_rechecks, (name, thr, ref_rechecks)
+            else:
+                err, ties = check_f32(got[0], got[1], m64, f"{name} static {thr}")
+                say(f"    static exponents: max err {err:.2e} of the row maximum, {ties} ties, "
+                    f"{ct.hip.last_recheck_count()} FP64 re-checks (per-step scaling: {ref_rechecks})")
+                ref = got
+            done += 1
     if done == 0:
         pytest.skip(f"{name}: too many live entries per lane for registers")
     say("  list entry point")
@@ -147,8 +163,9 @@ def test_spec_reg_bit_identical_to_generic(name, code, p, n):
 def test_spec_reg_plan_variants_bit_identical(name, code, tab, factorize):
     """Register-resident kernels on plans of other shapes (rectangular code,
     no chain factorization, tabulation depth 0-8, 16 classes for the toric
-    code): FP32 bit-identical to the generic FP32 kernel, FP64 to the
-    generic FP64 kernel."""
+    code): FP32 with per-step scaling bit-identical to the generic FP32
+    kernel, FP64 to the generic FP64 kernel; FP32 with static exponents
+    within the FP32 tolerance of the CPU oracle."""
     torch = pytest.importorskip("torch")
     say(f"case spec reg variant {name} tab={tab} factorize={factorize}")
     t = CSSTannerGraph.from_code(code)
@@ -157,22 +174,28 @@ def test_spec_reg_plan_variants_bit_identical(name, code, tab, factorize):
     n = 2500
     S = css_syndrome_rows(sample_depolarizing(em, seed=1234 + t.nq + tab, count=n), t)
     dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
-    ct.hip.set_spec(16, 64, False, 0, reg=True)
+    ct.hip.set_spec(16, 64, False, 0, reg=True, dyn=True)
     assert ct.hip.spec_config()["reg"]
     for mode, ref_mode in (("spec_f32", "fused_f32"), ("spec_f64", "fused_f64")):
         ref = run(torch, ct, dS, n, ref_mode)
         got = run(torch, ct, dS, n, mode)
         for a, b in zip(got, ref):
             assert np.array_equal(a, b), (name, mode)
+    ct.hip.set_spec(16, 64, False, 0, reg=True)
+    got = run(torch, ct, dS, n, "spec_f32")
+    m64, _, _ = OraclePlan(ct.plan).decode(S)
+    err, ties = check_f32(got[0], got[1], m64, f"{name} static")
+    say(f"  static exponents: max err {err:.2e}, {ties} ties")
 
 
 @pytest.mark.parametrize("d,tab,n", [(5, 5, 3001), (5, 6, 3001), (7, 6, 8192), (7, 7, 8192), (7, 8, 8192)])
 def test_deep_tabulation_matches_oracle(d, tab, n):
     """Subtrees on up to 8 syndrome bits tabulated at plan creation (up to
-    256 variants each): the register-resident kernel stays bit-identical to
-    the generic FP32 kernel, and both agree with the CPU oracle's FP64
-    contraction (classes on every non-tie syndrome, marginals to 2e-5 of the
-    row maximum)."""
+    256 variants each): the register-resident kernel with per-step scaling
+    stays bit-identical to the generic FP32 kernel, and both it and the
+    static-exponent default agree with the CPU oracle's FP64 contraction
+    (classes on every non-tie syndrome, marginals to 2e-5 of the row
+    maximum)."""
     torch = pytest.importorskip("torch")
     say(f"case tabulate_bits={tab} d={d} n={n}")
     t = CSSTannerGraph.from_code(SurfaceCode(d, d))
@@ -183,13 +206,16 @@ def test_deep_tabulation_matches_oracle(d, tab, n):
     S = css_syndrome_rows(sample_depolarizing(em, seed=606 + d + tab, count=n), t)
     dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
     ref = run(torch, ct, dS, n, "fused_f32")
-    ct.hip.set_spec(64, 256, False, 2, reg=True)
+    ct.hip.set_spec(64, 256, False, 2, reg=True, dyn=True)
     got = run(torch, ct, dS, n, "spec_f32")
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
     m64, _, _ = OraclePlan(ct.plan).decode(S)
     err, _ = check_f32(got[0], got[1], m64, f"tab {tab}")
-    say(f"  max err {err:.2e}")
+    ct.hip.set_spec(64, 256, False, 2, reg=True)
+    got = run(torch, ct, dS, n, "spec_f32")
+    err_s, _ = check_f32(got[0], got[1], m64, f"tab {tab} static")
+    say(f"  max err {err:.2e} (per-step scaling), {err_s:.2e} (static exponents)")
 
 
 @pytest.mark.parametrize("reg", [True, False])
@@ -257,11 +283,16 @@ def test_spec_list_decode_and_autotune():
     dS = torch.from_numpy(np.ascontiguousarray(S)).to(dev)
     say("case spec autotune")
     tile, thr = ct.hip.autotune("spec_f32", dS.data_ptr(), n)
-    say(f"  chosen tile {tile} threads {thr}")
-    ref = run(torch, ct, dS, n, "fused_f32")
+    cfg = ct.hip.spec_config()
+    say(f"  chosen {cfg}")
     got = run(torch, ct, dS, n, "spec_f32")
-    for a, b in zip(got, ref):
-        assert np.array_equal(a, b)
+    if cfg["reg"] and not cfg["dyn"]:
+        m64, _, _ = OraclePlan(ct.plan).decode(S)
+        check_f32(got[0], got[1], m64, "autotuned static")
+    else:
+        for a, b in zip(got, run(torch, ct, dS, n, "fused_f32")):
+            assert np.array_equal(a, b)
+    ref = got
     say("case spec list decode")
     rows = np.arange(3, n, 7, dtype=np.int32)
     dl = torch.from_numpy(rows).to(dev)

@@ -13,6 +13,12 @@ from tensorqec.codes import CSSTannerGraph, SurfaceCode, ToricCode
 from tensorqec.errors import iid_error
 from tensorqec import decoders as D
 
+
+def _compiler_env():
+    """Environment for hipcc child processes: without a preloaded sanitizer
+    runtime (the sanitized-library run preloads one into this process)."""
+    return {k: v for k, v in os.environ.items() if k != "LD_PRELOAD"}
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -176,7 +182,7 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
     r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
                         str(f), "-o", str(tmp_path / "spec.o"), "-Xclang", "-target-feature", "-Xclang",
                         "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
-                       capture_output=True, text=True, timeout=300)
+                       capture_output=True, text=True, timeout=300, env=_compiler_env())
     assert r.returncode == 0, r.stderr[-2000:]
     m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
     assert m and int(m.group(1)) == 0
@@ -188,7 +194,9 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
 def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, tab):
     """The register-resident variant (one syndrome per quad of lanes, every
     tensor in registers, DPP quad exchanges) compiles for gfx950 without
-    scratch on the d=5 plan; configurations it cannot take are refused."""
+    scratch on the d=5 plan, with static exponents (no per-step maxima or
+    rescales) and with per-step scaling; configurations it cannot take are
+    refused."""
     import shutil
     import subprocess
     hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
@@ -197,20 +205,24 @@ def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, 
     t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
     p = 0.05 / 3
     ct = D.compile(D.TNMMAP(mode="fused_f32"), t, iid_error(p, p, p, t.nq))
-    src = ct.hip.spec_source(threads // 4, threads, tab, reg=True)
-    assert "tqec_spec_f32" in src and f"constexpr int G = {threads // 4}," in src
-    assert "__builtin_amdgcn_mov_dpp" in src
-    steps = src[src.index("int lowf"):src.index("float rv[")]
-    assert "__syncthreads" not in steps and "__shared__" not in steps   # the step program never synchronises
-    f = tmp_path / "reg.hip"
-    f.write_text("#include <hip/hip_runtime.h>\n" + src)
-    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
-                        str(f), "-o", str(tmp_path / "reg.o"), "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
-                       capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr[-2000:]
-    m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
-    assert m and int(m.group(1)) == 0
+    for dyn in (False, True):
+        src = ct.hip.spec_source(threads // 4, threads, tab, reg=True, dyn=dyn)
+        assert "tqec_spec_f32" in src and f"constexpr int G = {threads // 4}," in src
+        assert "__builtin_amdgcn_mov_dpp" in src
+        steps = src[src.index("{ // step 0"):src.index("float rv[")]
+        assert "__syncthreads" not in steps and "__shared__" not in steps   # the step program never synchronises
+        # static exponents: no per-tensor maxima, exponents or rescales between steps
+        assert ("lmax" in steps and "ilogbf" in steps) == dyn
+        assert ("static exponents" in src) != dyn
+        f = tmp_path / "reg.hip"
+        f.write_text("#include <hip/hip_runtime.h>\n" + src)
+        r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only",
+                            "-c", str(f), "-o", str(tmp_path / "reg.o"), "-Xclang", "-target-feature", "-Xclang",
+                            "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
+                           capture_output=True, text=True, timeout=300, env=_compiler_env())
+        assert r.returncode == 0, r.stderr[-2000:]
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
+        assert m and int(m.group(1)) == 0
     with pytest.raises(L.TqecError):
         ct.hip.spec_source(threads // 2, threads, tab, reg=True)   # tile must be threads / 4
     # the FP64 form (SPEC_F64): doubles in register pairs, no scaling, no re-check list
@@ -221,7 +233,7 @@ def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, 
     r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
                         str(f64), "-o", str(tmp_path / "reg64.o"), "-Xclang", "-target-feature", "-Xclang",
                         "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
-                       capture_output=True, text=True, timeout=300)
+                       capture_output=True, text=True, timeout=300, env=_compiler_env())
     assert r.returncode == 0, r.stderr[-2000:]
     m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
     assert m and int(m.group(1)) == 0
@@ -254,7 +266,7 @@ def test_mixed_kernel_source_compiles_for_gfx950(tmp_path):
     r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
                         str(f), "-o", str(tmp_path / "mixed.o"), "-Xclang", "-target-feature", "-Xclang",
                         "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
-                       capture_output=True, text=True, timeout=600)
+                       capture_output=True, text=True, timeout=600, env=_compiler_env())
     assert r.returncode == 0, r.stderr[-2000:]
     m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
     assert m and int(m.group(1)) == 0
