@@ -173,6 +173,27 @@ int tqec_decode_batch_device(tqec_plan *plan, const uint8_t *d_syndromes, int64_
                              void *stream);
 
 /*
+ * tqec_decode_batch_device without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Modes TQEC_MODE_SPEC_F32 and TQEC_MODE_SPEC_F64 only, at most
+ * TQEC_NOALLOC_MAX_ROWS syndromes per call (TQEC_EINVAL beyond).  The FP64
+ * re-check list and counts live in one static device array per GPU
+ * (~4.3 MB, held until the process ends) instead of the per-stream
+ * workspace; every call on a GPU shares it, so calls that overlap on one GPU
+ * (different streams, or threads) race and must be serialised by the caller.
+ * The plan's tables and kernels must already be on the device: make one
+ * tqec_decode_batch[_device] call in the same mode there first (TQEC_EINVAL
+ * "plan not prepared" otherwise).  Plans whose kernels need a global slab
+ * return TQEC_EUNSUPPORTED.  Results equal tqec_decode_batch_device's (same
+ * kernels); tqec_last_recheck_count does not see these calls.
+ */
+#define TQEC_NOALLOC_MAX_ROWS (1 << 20)
+int tqec_decode_batch_device_noalloc(tqec_plan *plan, const uint8_t *d_syndromes, int64_t n, int32_t mode,
+                                     double *d_marginals, int32_t *d_classes, uint8_t *d_tie_flags,
+                                     void *stream);
+
+/*
  * Most probable configuration (TNMAP) for a max-product plan: replaces
  * TensorInference's `most_probable_config` after `update_evidence!`
  * (src/decoding/tndecoder.jl:53-57).  The contraction keeps, per output

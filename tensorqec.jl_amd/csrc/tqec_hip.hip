@@ -1969,6 +1969,39 @@ void mixed_free(tqec_mixed *m) {
 
 }  // namespace
 
+namespace {
+
+// Scratch of tqec_decode_batch_device_noalloc, one copy per device, held by
+// the loaded code object until the process ends: the FP32 pass's re-check
+// list (one int32 per row) followed by the re-check counts (register-resident
+// kernel: one per tile of at least 16 rows; LDS-tile kernel: one list length).
+constexpr int64_t kNoAllocRows = TQEC_NOALLOC_MAX_ROWS;
+constexpr int64_t kNoAllocCounts = kNoAllocRows / 16 + 64;
+static __device__ __align__(256) int32_t scratch[kNoAllocRows + kNoAllocCounts];
+
+// Device address of `scratch` per device.  In host code the array's name only
+// stands for the symbol; the address kernels may use comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int scratch_address(int dev, int32_t **out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return set_error(TQEC_EUNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(TQEC_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (int32_t *)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return TQEC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int tqec_plan_create(const tqec_plan_desc *desc, tqec_plan **out) {
@@ -2074,6 +2107,71 @@ int tqec_decode_batch_device(tqec_plan *p, const uint8_t *d_syn, int64_t n, int3
         return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
     std::lock_guard<std::mutex> lk(p->mu);
     return decode_device(p, d_syn, n, mode, d_mar, d_cls, d_tie, (hipStream_t)stream, nullptr);
+}
+
+int tqec_decode_batch_device_noalloc(tqec_plan *p, const uint8_t *d_syn, int64_t n, int32_t mode, double *d_mar,
+                                     int32_t *d_cls, uint8_t *d_tie, void *stream) {
+    if (!p) return set_error(TQEC_ENULL, "null plan");
+    if (n < 0) return set_error(TQEC_EINVAL, "negative batch size");
+    if (n > 0 && !d_syn) return set_error(TQEC_ENULL, "null syndrome pointer");
+    if (n > kNoAllocRows)
+        return set_error(TQEC_EINVAL, "batch of " + std::to_string(n) + " rows exceeds the no-allocation limit of " +
+                                          std::to_string(kNoAllocRows) + " rows");
+    if (p->cp.semiring != TQEC_SEMIRING_SUM)
+        return set_error(TQEC_EINVAL, "a max-product plan is decoded by tqec_map_decode_batch");
+    if (mode != TQEC_MODE_SPEC_F32 && mode != TQEC_MODE_SPEC_F64)
+        return set_error(TQEC_EUNSUPPORTED, "the no-allocation entry point runs TQEC_MODE_SPEC_F32 and "
+                                            "TQEC_MODE_SPEC_F64 only");
+    std::lock_guard<std::mutex> lk(p->mu);
+    const hipStream_t st = (hipStream_t)stream;
+    int dev;
+    TQ_HIP(hipGetDevice(&dev));
+    int32_t *scr;
+    int rc;
+    if ((rc = scratch_address(dev, &scr))) return rc;
+    // Tables and kernels must already be on this device: building them
+    // allocates, so it is left to a tqec_decode_batch[_device] call in the
+    // same mode (or tqec_plan_autotune) made beforehand.
+    const char *unprepared = "plan not prepared on this device: make one tqec_decode_batch_device call in this mode "
+                             "first";
+    auto dit = p->dev.find(dev);
+    const bool f64 = mode == TQEC_MODE_SPEC_F64;
+    if (dit == p->dev.end() || !(f64 ? p->spec64_cfg_set : p->spec_cfg_set)) return set_error(TQEC_EINVAL, unprepared);
+    DeviceData *dd = &dit->second;
+    const SpecCfg &cfg = f64 ? p->spec64_cfg : p->spec_cfg;
+    if (cfg.glog < 0) return set_error(TQEC_EUNSUPPORTED, "plan does not fit the specialised kernel");
+    SpecKernel *k = spec_find(p, dev, cfg);
+    if (!k) return set_error(TQEC_EINVAL, unprepared);
+    if (cfg.slab) return set_error(TQEC_EUNSUPPORTED, "plan runs from a global slab, which this entry point lacks");
+    if (n == 0) return TQEC_OK;
+    if (f64)
+        return spec_launch(p, nullptr, 0, k, dd, d_syn, n, nullptr, nullptr, d_mar, d_cls, d_tie, nullptr, nullptr, 0,
+                           p->tie_tol, p->recheck_tol, st);
+    const SpecCfg rcfg = spec_recheck_cfg(p);
+    SpecKernel *k64 = spec_find(p, dev, rcfg);
+    if (!k64) return set_error(TQEC_EINVAL, unprepared);
+    if (rcfg.slab) return set_error(TQEC_EUNSUPPORTED, "plan's FP64 re-check runs from a global slab");
+    int32_t *list = scr, *counts = scr + kNoAllocRows;
+    if (cfg.reg) {
+        // per-tile segments written whole by each tile (no clearing pass);
+        // a tile flags only its own valid rows, so list indices stay below n
+        const int32_t G = 1 << cfg.glog;
+        const int64_t nseg = (n + G - 1) / G;
+        if (G < 16 || nseg > kNoAllocCounts) return set_error(TQEC_EINVAL, "internal: tile too small for the scratch");
+        if ((rc = spec_launch(p, nullptr, 0, k, dd, d_syn, n, nullptr, nullptr, d_mar, d_cls, d_tie, list, counts, G,
+                              p->tie_tol, p->recheck_tol, st)))
+            return rc;
+        return spec_launch(p, nullptr, 0, k64, dd, d_syn, n, list, counts, d_mar, d_cls, d_tie, nullptr, nullptr, 0,
+                           p->tie_tol, p->recheck_tol, st, nseg, G);
+    }
+    // LDS tiles: one list with its length cleared by a fill kernel
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, counts, 1);
+    TQ_HIP(hipGetLastError());
+    if ((rc = spec_launch(p, nullptr, 0, k, dd, d_syn, n, nullptr, nullptr, d_mar, d_cls, d_tie, list, counts, n,
+                          p->tie_tol, p->recheck_tol, st)))
+        return rc;
+    return spec_launch(p, nullptr, 0, k64, dd, d_syn, n, list, counts, d_mar, d_cls, d_tie, nullptr, nullptr, 0,
+                       p->tie_tol, p->recheck_tol, st);
 }
 
 int tqec_decode_batch(tqec_plan *p, const uint8_t *syn, int64_t n, int32_t mode, double *mar, int32_t *cls,

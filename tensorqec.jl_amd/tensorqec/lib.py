@@ -33,6 +33,7 @@ MODE_SPEC_BF16 = 6
 MODES = {"plain_f64": MODE_PLAIN_F64, "fused_f64": MODE_FUSED_F64, "fused_f32": MODE_FUSED_F32,
          "wide_f32": MODE_WIDE_F32, "spec_f32": MODE_SPEC_F32, "spec_f64": MODE_SPEC_F64,
          "spec_bf16": MODE_SPEC_BF16}
+NOALLOC_MAX_ROWS = 1 << 20   # TQEC_NOALLOC_MAX_ROWS
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -111,6 +112,9 @@ EXPORTS = [
     ("tqec_decode_batch", ctypes.c_int,
      [ctypes.c_void_p, _u8p, ctypes.c_int64, ctypes.c_int32, _f64p, _i32p, _u8p, ctypes.c_void_p]),
     ("tqec_decode_batch_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p]),
+    ("tqec_decode_batch_device_noalloc", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p]),
     ("tqec_plan_set_tolerances", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
@@ -328,6 +332,18 @@ class HipPlan:
                                               ctypes.c_void_p(d_cls) if d_cls else None,
                                               ctypes.c_void_p(d_tie) if d_tie else None,
                                               ctypes.c_void_p(stream) if stream else None))
+
+    def decode_device_noalloc(self, d_syn: int, n: int, mode, d_mar: int, d_cls: int, d_tie: int,
+                              stream: int = 0) -> None:
+        """As decode_device, with no device allocation (specialised modes, at
+        most NOALLOC_MAX_ROWS rows, the plan prepared on the device by an
+        earlier decode_device call in the same mode); enqueues only."""
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        check(load().tqec_decode_batch_device_noalloc(self._h, ctypes.c_void_p(d_syn), int(n), m,
+                                                      ctypes.c_void_p(d_mar) if d_mar else None,
+                                                      ctypes.c_void_p(d_cls) if d_cls else None,
+                                                      ctypes.c_void_p(d_tie) if d_tie else None,
+                                                      ctypes.c_void_p(stream) if stream else None))
 
     def autotune(self, mode, d_syn: int, n: int, stream: int = 0):
         """Pick the fastest fused tile configuration on a device-resident
