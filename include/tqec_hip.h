@@ -248,6 +248,14 @@ int tqec_mlp_train_step(tqec_mlp *mlp, const uint8_t *d_syndromes, const float *
  * whose confidence is below `threshold` (*d_reject_count is reset first). */
 int tqec_mlp_gate(tqec_mlp *mlp, const uint8_t *d_syndromes, int64_t n, float threshold, int32_t *d_classes,
                   float *d_confidence, int32_t *d_reject, int32_t *d_reject_count, void *stream);
+/* Kernel of tqec_mlp_forward / tqec_mlp_gate: 1 (default where it applies:
+ * hidden 512, n_out <= 32, n_in <= 512) the fused kernel — all four layers
+ * for 128 syndromes per 8-wave workgroup with the activations in LDS; 0 the
+ * layer-by-layer kernels that training uses.  Logits agree to FP32
+ * rounding (the fused kernel adds the bias before the products).
+ * tqec_mlp_inference reports the kernel in use (0 or 1). */
+int tqec_mlp_set_inference(tqec_mlp *mlp, int32_t kernel);
+int tqec_mlp_inference(const tqec_mlp *mlp, int32_t *kernel);
 void tqec_mlp_destroy(tqec_mlp *mlp);
 /* Training targets from TN marginals: q = marginal / sum over classes (a row
  * of zeros gives the uniform distribution). */

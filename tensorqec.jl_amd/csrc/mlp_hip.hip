@@ -238,6 +238,154 @@ __global__ void __launch_bounds__(256) gemm_fwd_kernel(const bf16 *__restrict__ 
         }
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference: the whole network for a tile of 128 syndromes in one
+// workgroup, hidden width 512.  The tile's activations stay in LDS
+// (128 x 512 BF16 = 128 KB, rows of 64 16-byte chunks, chunk c of row m
+// stored at c ^ (m & 15) so that the 16 lanes of a ds_read_b128 pass hit
+// distinct banks) and each layer overwrites them in place after a barrier.
+// Every product runs on v_mfma_f32_32x32x16_bf16 with the operands swapped
+// (D = W^T * act^T): wave w owns output columns [w * 512 / NW, ...) for all
+// 128 rows, so no two waves read the same weights and each wave streams its
+// weight fragments straight from global memory (L2-resident, 1.1 MB for the
+// d=7 network) in the fragment order laid out once by pack_fused_kernel,
+// two k-steps ahead; only the activations are shared, with no barrier inside
+// a layer.  A lane of a D tile holds 4 consecutive output columns of one
+// row, written back as one 8-byte LDS store per group.  Biases seed the
+// accumulators.  The output layer (N <= 32, padded) runs the same way with
+// one wave per 32 rows.  FP32 accumulation, BF16 activations, as the
+// layer-by-layer kernels (same roundings; the bias enters before the
+// products instead of after them).
+// ---------------------------------------------------------------------------
+
+constexpr int kFusedRows = 128, kFusedHid = 512;
+
+__device__ __forceinline__ int act_off(int m, int chunk) { return m * (kFusedHid * 2) + ((chunk ^ (m & 15)) << 4); }
+
+// Fragment-ordered weights.  Hidden layer l (K x 512): for each 16-row k-step
+// s and 32-column tile t, 64 lanes x 8 BF16: lane L holds W[16 s + 8 (L >> 5)
+// + e][32 t + (L & 31)], e = 0..7.  Output layer: one 32-column tile, columns
+// >= nout zero.
+__global__ void pack_fused_kernel(const bf16 *__restrict__ W, int K, int N, int nout_pad, bf16 *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int tiles = nout_pad / 32;
+    if (i >= (int64_t)K * nout_pad) return;
+    const int e = (int)(i & 7), L = (int)((i >> 3) & 63);
+    const int64_t q = i >> 9;   // (s, t)
+    const int t = (int)(q % tiles), s = (int)(q / tiles);
+    const int k = 16 * s + 8 * (L >> 5) + e, n = 32 * t + (L & 31);
+    out[i] = n < N ? W[(int64_t)k * N + n] : (bf16)0;
+}
+
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) mlp_fused_fwd_kernel(const uint8_t *__restrict__ syn, int64_t B, int n_in,
+                                                                 int kin, const bf16 *__restrict__ wf1,
+                                                                 const bf16 *__restrict__ wf2,
+                                                                 const bf16 *__restrict__ wf3,
+                                                                 const float *__restrict__ b1,
+                                                                 const float *__restrict__ b2,
+                                                                 const float *__restrict__ b3,
+                                                                 const bf16 *__restrict__ wf4,
+                                                                 const float *__restrict__ b4, int nout,
+                                                                 float *__restrict__ z) {
+    extern __shared__ __align__(16) unsigned char act[];
+    constexpr int NT = 16 / NW;                  // 32-column tiles per wave
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m32 = lane & 31, h = lane >> 5;
+    const int64_t row0 = (int64_t)blockIdx.x * kFusedRows;
+    // syndrome bits -> BF16 0/1 features in columns [0, kin)
+    const int cpr = kin >> 3;
+    for (int q = tid; q < kFusedRows * cpr; q += 64 * NW) {
+        const int m = q / cpr, c = q - m * cpr;
+        const int64_t r = row0 + m;
+        uint16_t v[8];
+        for (int e = 0; e < 8; ++e) {
+            const int f = 8 * c + e;
+            v[e] = (r < B && f < n_in && (syn[r * n_in + f] & 1)) ? (uint16_t)0x3f80 : (uint16_t)0;
+        }
+        uint4 u;
+        memcpy(&u, v, 16);
+        *reinterpret_cast<uint4 *>(act + act_off(m, c)) = u;
+    }
+    __syncthreads();
+    for (int l = 0; l < 3; ++l) {
+        const int K = l == 0 ? kin : kFusedHid;
+        const int steps = K >> 4;   // even: K is a multiple of 32
+        const bf16 *W = l == 0 ? wf1 : (l == 1 ? wf2 : wf3);
+        const float *bias = l == 0 ? b1 : (l == 1 ? b2 : b3);
+        f32x16 acc[NT][4];
+        for (int t = 0; t < NT; ++t) {
+            // lane holds columns 32 T + 8 g + 4 h + r for v = 4 g + r
+            const int n0 = 32 * (wave * NT + t) + 4 * h;
+            for (int g = 0; g < 4; ++g) {
+                const float4 bb = *reinterpret_cast<const float4 *>(bias + n0 + 8 * g);
+                for (int mt = 0; mt < 4; ++mt) {
+                    acc[t][mt][4 * g + 0] = bb.x;
+                    acc[t][mt][4 * g + 1] = bb.y;
+                    acc[t][mt][4 * g + 2] = bb.z;
+                    acc[t][mt][4 * g + 3] = bb.w;
+                }
+            }
+        }
+        // weight fragments of k-steps s and s + 1 in flight (two register
+        // sets, the loop unrolled by two so that every index is static)
+        bf16x8 wa0[NT], wa1[NT];
+        auto wload = [&](int s, bf16x8 *dst) {
+            for (int t = 0; t < NT; ++t)
+                dst[t] = *reinterpret_cast<const bf16x8 *>(W + ((((int64_t)s * 16 + wave * NT + t) << 6) + lane) * 8);
+        };
+        auto kstep = [&](int s, bf16x8 *wa) {
+            bf16x8 xb[4], cur[NT];
+            for (int mt = 0; mt < 4; ++mt)
+                xb[mt] = *reinterpret_cast<const bf16x8 *>(act + act_off(32 * mt + m32, 2 * s + h));
+            for (int t = 0; t < NT; ++t) cur[t] = wa[t];
+            if (s + 2 < steps) wload(s + 2, wa);
+            for (int t = 0; t < NT; ++t)
+                for (int mt = 0; mt < 4; ++mt)
+                    acc[t][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[t], xb[mt], acc[t][mt], 0, 0, 0);
+        };
+        wload(0, wa0);
+        wload(1, wa1);
+        for (int s = 0; s < steps; s += 2) {
+            kstep(s, wa0);
+            kstep(s + 1, wa1);
+        }
+        __syncthreads();   // every wave is done reading this layer's input
+        for (int t = 0; t < NT; ++t)
+            for (int mt = 0; mt < 4; ++mt) {
+                const int m = 32 * mt + m32;
+                for (int g = 0; g < 4; ++g) {
+                    const int n0 = 32 * (wave * NT + t) + 8 * g + 4 * h;
+                    uint16_t v[4];
+                    for (int r = 0; r < 4; ++r) v[r] = f2bf(fmaxf(acc[t][mt][4 * g + r], 0.f));
+                    uint2 u;
+                    memcpy(&u, v, 8);
+                    *reinterpret_cast<uint2 *>(act + act_off(m, n0 >> 3) + ((n0 & 7) << 1)) = u;
+                }
+            }
+        __syncthreads();
+    }
+    // output layer: wave w -> rows [32 w, 32 w + 32) of the tile, all (padded) 32 columns
+    for (int mt = wave; mt < 4; mt += NW) {
+        f32x16 acc;
+        for (int v = 0; v < 16; ++v) {
+            const int n = 8 * (v >> 2) + 4 * h + (v & 3);
+            acc[v] = n < nout ? b4[n] : 0.f;
+        }
+        for (int s = 0; s < kFusedHid / 16; ++s) {
+            const bf16x8 w = *reinterpret_cast<const bf16x8 *>(wf4 + (((int64_t)s << 6) + lane) * 8);
+            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(act + act_off(32 * mt + m32, 2 * s + h));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, x, acc, 0, 0, 0);
+        }
+        const int64_t r = row0 + 32 * mt + m32;
+        if (r < B)
+            for (int v = 0; v < 16; ++v) {
+                const int n = 8 * (v >> 2) + 4 * h + (v & 3);
+                if (n < nout) z[r * nout + n] = acc[v];
+            }
+    }
+}
+
 // Wt[n][k] = W[k][n] for the three hidden-layer weight matrices.
 __global__ void transpose_bf16_kernel(const bf16 *__restrict__ W, int K, int N, bf16 *__restrict__ Wt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -448,6 +596,13 @@ struct tqec_mlp {
     bf16 *wb = nullptr;
     bf16 *wt[3] = {nullptr, nullptr, nullptr};   // transposed BF16 weights of the hidden layers (inference GEMM)
     bool wt_stale = true;
+    // fused inference (hidden width 512): fragment-ordered BF16 weights of the
+    // four layers, rebuilt after every weight change
+    bf16 *wf[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool wf_stale = true;
+    bool fused = true;             // forward / gate on the fused kernel where it applies
+    int64_t zcap = 0;              // rows the logits buffer z holds
+    bool fused_attr = false;
     int64_t step = 0;
     // activations / scratch for up to `cap` rows
     int64_t cap = 0;
@@ -476,10 +631,13 @@ int ensure_cap(tqec_mlp *p, int64_t B) {
         if (*b) (void)hipFree(*b);
     if (p->z) (void)hipFree(p->z);
     if (p->dz) (void)hipFree(p->dz);
+    p->z = nullptr;
+    p->zcap = 0;
     p->cap = 0;
     TQM_HIP(hipMalloc((void **)&p->X, (size_t)B * p->kin * sizeof(bf16)));
     for (int i = 1; i < 6; ++i) TQM_HIP(hipMalloc((void **)bufs[i], (size_t)B * p->hid * sizeof(bf16)));
     TQM_HIP(hipMalloc((void **)&p->z, (size_t)B * p->nout * sizeof(float)));
+    p->zcap = B;
     TQM_HIP(hipMalloc((void **)&p->dz, (size_t)B * p->nout * sizeof(float)));
     const int64_t nb = (B + 255) / 256;
     if (p->loss_cap < nb) {
@@ -497,8 +655,46 @@ void gemm(const GemmArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((gemm_bf16_kernel<AT, BT, EPI>), grid, dim3(256), 0, st, a);
 }
 
-int forward(tqec_mlp *p, const uint8_t *d_syn, int64_t B, hipStream_t st) {
+bool fused_ok(const tqec_mlp *p) { return p->hid == kFusedHid && p->nout <= 32 && p->kin <= kFusedHid; }
+
+// Logits only (no activations kept): the fused kernel, one workgroup per 128 rows.
+int forward_fused(tqec_mlp *p, const uint8_t *d_syn, int64_t B, hipStream_t st) {
+    if (p->zcap < B) {
+        // logits only; the activation buffers are allocated for training
+        if (p->z) (void)hipFree(p->z);
+        p->z = nullptr;
+        p->zcap = 0;
+        TQM_HIP(hipMalloc((void **)&p->z, (size_t)B * p->nout * sizeof(float)));
+        p->zcap = B;
+    }
+    const size_t lds = (size_t)kFusedRows * kFusedHid * 2;
+    if (!p->fused_attr) {
+        TQM_HIP(hipFuncSetAttribute((const void *)mlp_fused_fwd_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        p->fused_attr = true;
+    }
+    if (p->wf_stale || !p->wf[0]) {
+        const int kdim[4] = {p->kin, p->hid, p->hid, p->hid};
+        for (int l = 0; l < 4; ++l) {
+            const int npad = l < 3 ? p->hid : 32;
+            if (!p->wf[l]) TQM_HIP(hipMalloc((void **)&p->wf[l], (size_t)kdim[l] * npad * sizeof(bf16)));
+            const int64_t n = (int64_t)kdim[l] * npad;
+            hipLaunchKernelGGL(pack_fused_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                               p->wb + p->off[2 * l], kdim[l], l < 3 ? p->hid : p->nout, npad, p->wf[l]);
+        }
+        p->wf_stale = false;
+    }
+    const unsigned grid = (unsigned)((B + kFusedRows - 1) / kFusedRows);
+    const float *b[4] = {p->w + p->off[1], p->w + p->off[3], p->w + p->off[5], p->w + p->off[7]};
+    hipLaunchKernelGGL(mlp_fused_fwd_kernel<8>, dim3(grid), dim3(512), lds, st, d_syn, B, p->n_in, p->kin, p->wf[0],
+                       p->wf[1], p->wf[2], b[0], b[1], b[2], p->wf[3], b[3], p->nout, p->z);
+    TQM_HIP(hipGetLastError());
+    return TQEC_OK;
+}
+
+int forward(tqec_mlp *p, const uint8_t *d_syn, int64_t B, hipStream_t st, bool keep = true) {
     int rc;
+    if (!keep && p->fused && fused_ok(p)) return forward_fused(p, d_syn, B, st);
     if ((rc = ensure_cap(p, B))) return rc;
     const int64_t nx = B * p->kin;
     hipLaunchKernelGGL(embed_kernel, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, d_syn, B, p->n_in, p->kin,
@@ -667,6 +863,7 @@ int tqec_mlp_set_params(tqec_mlp *p, const float *host) {
     TQM_HIP(hipDeviceSynchronize());
     p->step = 0;
     p->wt_stale = true;
+    p->wf_stale = true;
     return TQEC_OK;
 }
 
@@ -682,7 +879,7 @@ int tqec_mlp_forward(tqec_mlp *p, const uint8_t *d_syn, int64_t B, float *d_logi
     if ((rc = check_batch(p, B, false))) return rc;
     if (!d_syn || !d_logits) return fail(TQEC_ENULL, "null device pointer");
     hipStream_t st = (hipStream_t)stream;
-    if ((rc = forward(p, d_syn, B, st))) return rc;
+    if ((rc = forward(p, d_syn, B, st, false))) return rc;
     TQM_HIP(hipMemcpyAsync(d_logits, p->z, (size_t)B * p->nout * 4, hipMemcpyDeviceToDevice, st));
     return TQEC_OK;
 }
@@ -719,6 +916,7 @@ int tqec_mlp_train_step(tqec_mlp *p, const uint8_t *d_syn, const float *d_target
     hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((p->nparams + 255) / 256)), dim3(256), 0, st, p->w, p->g, p->m,
                        p->v, p->wb, p->nparams, lr, b1, b2, 1e-8f, c1, c2);
     p->wt_stale = true;
+    p->wf_stale = true;
     // padded rows of W1 get zero gradient (zero inputs) and so stay zero
     TQM_HIP(hipGetLastError());
     return TQEC_OK;
@@ -730,7 +928,7 @@ int tqec_mlp_gate(tqec_mlp *p, const uint8_t *d_syn, int64_t B, float threshold,
     if ((rc = check_batch(p, B, false))) return rc;
     if (!d_syn || (d_reject && !d_reject_count)) return fail(TQEC_ENULL, "null device pointer");
     hipStream_t st = (hipStream_t)stream;
-    if ((rc = forward(p, d_syn, B, st))) return rc;
+    if ((rc = forward(p, d_syn, B, st, false))) return rc;
     if (d_reject_count) hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, d_reject_count);
     hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, p->z, B, p->nout, threshold,
                        d_classes, d_confidence, d_reject, d_reject_count);
@@ -749,10 +947,26 @@ int tqec_posteriors_from_marginals(const double *d_marginals, int64_t n, int32_t
     return TQEC_OK;
 }
 
+int tqec_mlp_set_inference(tqec_mlp *p, int32_t kernel) {
+    if (!p) return fail(TQEC_ENULL, "null network");
+    if (kernel != 0 && kernel != 1) return fail(TQEC_EINVAL, "inference kernel must be 0 (layer by layer) or 1 (fused)");
+    if (kernel == 1 && !fused_ok(p))
+        return fail(TQEC_EUNSUPPORTED, "the fused kernel needs hidden width 512, n_out <= 32 and n_in <= 512");
+    p->fused = kernel == 1;
+    return TQEC_OK;
+}
+
+int tqec_mlp_inference(const tqec_mlp *p, int32_t *kernel) {
+    if (!p || !kernel) return fail(TQEC_ENULL, "null argument");
+    *kernel = (p->fused && fused_ok(p)) ? 1 : 0;
+    return TQEC_OK;
+}
+
 void tqec_mlp_destroy(tqec_mlp *p) {
     if (!p) return;
     void *bufs[] = {p->w,  p->g,  p->m,  p->v,  p->wb, p->X,  p->H1,         p->H2,    p->H3,
-                    p->dA, p->dB, p->z,  p->dz, p->loss_parts, p->wt[0], p->wt[1], p->wt[2]};
+                    p->dA, p->dB, p->z,  p->dz, p->loss_parts, p->wt[0], p->wt[1], p->wt[2],
+                    p->wf[0], p->wf[1], p->wf[2], p->wf[3]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;

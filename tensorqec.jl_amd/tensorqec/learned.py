@@ -105,6 +105,16 @@ class HipMLP:
         L.check(L.load().tqec_mlp_gate(self._h, _vp(d_syn), int(n), float(threshold), _vp(d_cls), _vp(d_conf),
                                        _vp(d_reject), _vp(d_count), _vp(stream)))
 
+    def set_inference(self, kernel: int) -> None:
+        """Kernel of forward / gate: 1 fused (default where it applies), 0 the
+        layer-by-layer training kernels."""
+        L.check(L.load().tqec_mlp_set_inference(self._h, int(kernel)))
+
+    def inference(self) -> int:
+        k = ctypes.c_int32()
+        L.check(L.load().tqec_mlp_inference(self._h, ctypes.byref(k)))
+        return k.value
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -165,6 +165,8 @@ EXPORTS = [
     ("tqec_mlp_gate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p]),
+    ("tqec_mlp_set_inference", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("tqec_mlp_inference", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     ("tqec_mlp_destroy", None, [ctypes.c_void_p]),
     ("tqec_posteriors_from_marginals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                                        ctypes.c_void_p, ctypes.c_void_p]),

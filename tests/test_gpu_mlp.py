@@ -112,3 +112,43 @@ def test_train_and_gate_d5():
     print(f"d5 gate: fallback {fall:.4f}, disagreement {(a != b).mean():.5f}, LER tn {ler_tn:.5f} gated {ler_g:.5f}")
     assert fall < 0.5
     assert abs(ler_g - ler_tn) < 3 * sigma + 1e-4
+
+
+@pytest.mark.parametrize("n_in,n_out,B", [(48, 4, 1000), (120, 2, 4096), (24, 2, 77), (80, 4, 65536)])
+def test_fused_forward_matches_layered_and_oracle(n_in, n_out, B):
+    """The fused inference kernel (all four layers per 128-row tile, the
+    activations in LDS) against the layer-by-layer kernels and the float64
+    oracle: logits within 1e-2 of the largest |logit| (BF16 activations on
+    both sides; the fused kernel adds the bias before the products, which can
+    move an activation across a BF16 rounding boundary), gate classes equal
+    wherever the top two logits are apart.  Ragged tiles (B not a multiple
+    of 128) and every supported input width class are covered."""
+    torch, mlp, S, Q = setup(n_in=n_in, n_out=n_out, B=B, seed=11)
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(S).to(dev)
+    assert mlp.inference() == 1
+    zf = torch.full((B, n_out), float("nan"), dtype=torch.float32, device=dev)
+    mlp.forward(dS.data_ptr(), B, zf.data_ptr())
+    mlp.set_inference(0)
+    assert mlp.inference() == 0
+    zl = torch.full((B, n_out), float("nan"), dtype=torch.float32, device=dev)
+    mlp.forward(dS.data_ptr(), B, zl.data_ptr())
+    mlp.set_inference(1)
+    torch.cuda.synchronize()
+    zf, zl = zf.cpu().numpy(), zl.cpu().numpy()
+    assert np.isfinite(zf).all()
+    scale = np.max(np.abs(zl))
+    assert np.max(np.abs(zf - zl)) < 1e-2 * scale
+    m = min(B, 2048)
+    params = mlp.unflatten(mlp.get_params())
+    _, z, _ = MO.forward_backward(params, features(S[:m], mlp.kin), Q[:m])
+    assert np.max(np.abs(zf[:m] - z)) < 1e-2 * np.max(np.abs(z))
+    srt = np.sort(zl, axis=1)
+    apart = (srt[:, -1] - srt[:, -2]) > 2e-2 * scale
+    assert np.array_equal(np.argmax(zf, axis=1)[apart], np.argmax(zl, axis=1)[apart])
+    # the gate runs the same kernel
+    cls = torch.empty(B, dtype=torch.int32, device=dev)
+    conf = torch.empty(B, dtype=torch.float32, device=dev)
+    mlp.gate(dS.data_ptr(), B, 0.9, cls.data_ptr(), conf.data_ptr(), 0, 0)
+    torch.cuda.synchronize()
+    assert np.array_equal(cls.cpu().numpy(), np.argmax(zf, axis=1))
