@@ -151,6 +151,10 @@ std::string subst_c(const std::string &expr, const std::string &by) {
 // stores after them.  Every output keeps the generic kernel's order of
 // operations (k ascending, single-side sums first), so results are
 // bit-identical.
+// Contraction terms per output unrolled in full; longer contractions run a
+// loop over chunks of this many terms (same order of the multiply-adds).
+constexpr uint32_t kMaxUnrollK = 16;
+
 void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int block, bool fp64) {
     const OperandDesc &A = st.a, &B = st.b;
     const int rc = st.rc;
@@ -206,11 +210,31 @@ void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int bloc
         b << "      const unsigned ar" << x << " = " << subst_c(ga, "c | " + std::to_string(acombo[x]) + "u") << ";\n";
     for (size_t y = 0; y < bcombo.size(); ++y)
         b << "      const unsigned br" << y << " = " << subst_c(gb, "c | " + std::to_string(bcombo[y]) + "u") << ";\n";
-    // operand values, loaded once per (row, k)
-    for (uint32_t k = 0; k < nkk; ++k) {
+    // contraction terms in chunks of kch (all of them at once when short);
+    // operand values loaded once per (row, k) of the chunk
+    const uint32_t kch = std::min(nkk, kMaxUnrollK);
+    const bool kloop = kch < nkk;
+    const std::string ia = kloop ? "(ar%1 | (kb << " + std::to_string(A.kshift) + "))" : "ar%1";
+    const std::string ib = kloop ? "(br%1 | (kb << " + std::to_string(B.kshift) + "))" : "br%1";
+    auto idx_of = [](const std::string &pat, size_t v) {
+        std::string s = pat;
+        const size_t at = s.find("%1");
+        return s.replace(at, 2, std::to_string(v));
+    };
+    std::vector<size_t> out_x, out_y;
+    for (uint32_t t = 0; t < (1u << ns); ++t) {
+        uint32_t m = 0;
+        for (int j = 0; j < ns; ++j)
+            if ((t >> j) & 1u) m |= 1u << S[j];
+        out_x.push_back(std::find(acombo.begin(), acombo.end(), m & am) - acombo.begin());
+        out_y.push_back(std::find(bcombo.begin(), bcombo.end(), m & bm) - bcombo.begin());
+        b << "      T acc" << t << " = 0;\n";
+    }
+    if (kloop) b << "      #pragma unroll 1\n      for (unsigned kb = 0; kb < " << nkk << "u; kb += " << kch << "u) {\n";
+    for (uint32_t k = 0; k < kch; ++k) {
         for (size_t x = 0; x < acombo.size(); ++x) {
             std::ostringstream pre;
-            const std::string nm = "ar" + std::to_string(x);
+            const std::string nm = idx_of(ia, x);
             const std::string tmp = "sa" + std::to_string(k) + "_" + std::to_string(x);
             const std::string v = load_expr(A, "pa", nm.c_str(), k << A.kshift, pre, tmp);
             b << pre.str();
@@ -218,26 +242,24 @@ void emit_outputs(std::ostringstream &o, const StepDesc &st, int cstep, int bloc
         }
         for (size_t y = 0; y < bcombo.size(); ++y) {
             std::ostringstream pre;
-            const std::string nm = "br" + std::to_string(y);
+            const std::string nm = idx_of(ib, y);
             const std::string tmp = "sb" + std::to_string(k) + "_" + std::to_string(y);
             const std::string v = load_expr(B, "pb", nm.c_str(), k << B.kshift, pre, tmp);
             b << pre.str();
             if (v != tmp) b << "      const T " << tmp << " = " << v << ";\n";
         }
     }
+    for (uint32_t t = 0; t < (1u << ns); ++t)
+        for (uint32_t k = 0; k < kch; ++k)
+            b << "      acc" << t << " = " << (fp64 ? "fma(" : "fmaf(") << "sa" << k << "_" << out_x[t] << ", sb" << k << "_"
+              << out_y[t] << ", acc" << t << ");\n";
+    if (kloop) b << "      }\n";
     // outputs of the block
     for (uint32_t t = 0; t < (1u << ns); ++t) {
         uint32_t m = 0;
         for (int j = 0; j < ns; ++j)
             if ((t >> j) & 1u) m |= 1u << S[j];
-        const uint32_t ma = m & am, mb = m & bm;
-        const size_t x = std::find(acombo.begin(), acombo.end(), ma) - acombo.begin();
-        const size_t y = std::find(bcombo.begin(), bcombo.end(), mb) - bcombo.begin();
         const std::string acc = "acc" + std::to_string(t);
-        b << "      T " << acc << " = 0;\n";
-        for (uint32_t k = 0; k < nkk; ++k)
-            b << "      " << acc << " = " << (fp64 ? "fma(" : "fmaf(") << "sa" << k << "_" << x << ", sb" << k << "_" << y
-              << ", " << acc << ");\n";
         if (fp64) {
             if (cexp != 0) b << "      " << acc << " = ldexp(" << acc << ", " << cexp << ");\n";
         } else {
@@ -1030,6 +1052,24 @@ bool emit_reg_step(std::ostringstream &o, const CompiledPlan &cp, const RegPlan 
     return true;
 }
 
+void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &cfg, const std::string &name,
+                  bool noinline = false);
+
+// FP64 configuration of the register-resident kernel's inline re-check: one
+// syndrome at a time over the whole workgroup, intermediates in LDS.
+SpecCfg inline_recheck_cfg(const SpecCfg &cfg) {
+    SpecCfg c;
+    c.glog = 0;
+    c.threads = cfg.threads;
+    c.tab_lds = false;
+    c.fp64 = true;
+    return c;
+}
+
+// LDS a workgroup of the register-resident kernel may spend on its inline
+// FP64 re-check (beyond it, the separate re-check kernel runs instead).
+constexpr size_t kInlineRecheckLds = 48 * 1024;
+
 std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &err) {
     RegPlan rp;
     if (!reg_plan(cp, rp)) {
@@ -1062,6 +1102,10 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
              "__builtin_amdgcn_mov_dpp(__double2loint(x), (c), 0xf, 0xf, false))\n";
     else
         o << "#define QP(x, c) __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), (c), 0xf, 0xf, false))\n";
+    const bool inl = spec_reg_inline_recheck(cp, cfg);
+    const SpecCfg c64 = inline_recheck_cfg(cfg);
+    // a call of its own, so that its registers do not crowd the FP32 program's
+    if (inl) emit_tile_fn(o, cp, c64, "recheck64", true);
     o << "extern \"C\" __global__ void __launch_bounds__(" << cfg.threads;
     if (cfg.waves > 0) o << ", " << cfg.waves;
     o << ") " << spec_kernel_name(cfg) << "(\n"
@@ -1070,10 +1114,16 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
          "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
          "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, " << T
       << " *__restrict__ gslab,\n"
-         "    int nseg, int segcap) {\n";
+         "    int nseg, int segcap, const double *__restrict__ gtab64) {\n";
     o << "  constexpr int G = " << G << ", NBITS = " << std::max(1, cp.n_syn) << ", NOUT = " << nout << ";\n";
-    o << "  (void)gslab; (void)nseg; (void)segcap;\n";
+    o << "  (void)gslab; (void)nseg; (void)segcap; (void)gtab64;\n";
     o << "  __shared__ int seg_n;\n";
+    if (inl) {
+        // rows this tile flags for the FP64 re-check, done by the whole
+        // workgroup at the end of the tile
+        o << "  __shared__ int flagged[G];\n";
+        o << "  __shared__ __attribute__((aligned(16))) unsigned char smem64[" << layout(cp, c64).total << "];\n";
+    }
     if (cfg.tab_lds) {
         o << "  __shared__ " << T << " tab[" << std::max<size_t>(1, tab_n) << "];\n";
         o << "  for (int i = threadIdx.x; i < " << tab_n << "; i += " << cfg.threads << ") tab[i] = gtab[i];\n";
@@ -1151,7 +1201,15 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
         std::snprintf(buf, sizeof(buf), "%.17g", lim < -1074 ? 0.0 : std::ldexp(1.0, (int)std::min(lim, 1000.0)));
         low = std::string("(double)best < ") + buf;
     }
-    if (!fp64)
+    if (fp64)
+        o << "      (void)recheck_tol;\n";
+    else if (inl)
+        o << "      {\n"
+             "        const bool rc = !(best > 0) || " << low << " ||\n"
+             "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
+             "        if (rc) flagged[atomicAdd(&seg_n, 1)] = (int)b;\n"
+             "      }\n";
+    else
         o << "      if (rl) {\n"
              "        const bool rc = !(best > 0) || " << low << " ||\n"
              "                        (NOUT > 1 && second >= 0 && (double)(best - second) <= recheck_tol * (double)best);\n"
@@ -1160,18 +1218,30 @@ std::string reg_source(const CompiledPlan &cp, const SpecCfg &cfg, std::string &
              "          if (slot < rcap) rl[t * rcap + slot] = (int)b;\n"
              "        }\n"
              "      }\n";
-    else
-        o << "      (void)recheck_tol;\n";
     o << "    }\n"
          "   }\n"
-         "   __syncthreads();\n"
-         "   if (threadIdx.x == 0 && rcount) rcount[t] = min(seg_n, (int)rcap);\n"
-         "  }\n"
+         "   __syncthreads();\n";
+    if (inl)
+        // at most G flagged rows (one per valid row of the tile), re-contracted
+        // in FP64 one after the other by the whole workgroup
+        o << "   const int nf = seg_n;\n"
+             "   for (int i = 0; i < nf; ++i)\n"
+             "     recheck64(syn, nf, flagged, i, gtab64, mar, cls, ties, nullptr, nullptr, 0, tie_tol, recheck_tol, "
+             "smem64, nullptr, 0u);\n"
+             "   if (threadIdx.x == 0 && rcount) rcount[t] = nf;\n"
+             "   __syncthreads();\n";
+    else
+        o << "   if (threadIdx.x == 0 && rcount) rcount[t] = min(seg_n, (int)rcap);\n";
+    o << "  }\n"
          "}\n";
     return o.str();
 }
 
 }  // namespace
+
+bool spec_reg_inline_recheck(const CompiledPlan &cp, const SpecCfg &cfg) {
+    return cfg.reg && !cfg.fp64 && cp.ws_entries > 0 && layout(cp, inline_recheck_cfg(cfg)).total <= kInlineRecheckLds;
+}
 
 size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg) {
     if (cfg.reg) return cfg.tab_lds ? round_up(std::max<size_t>(1, cp.const_pool.size() + cp.sliced_pool.size()) * 4, 16) : 0;
@@ -1215,12 +1285,13 @@ namespace {
 // whole step program, as a device function: the body of the generic fused
 // kernel's tile with every descriptor value substituted.  Starts and ends
 // with a workgroup barrier.
-void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &cfg, const std::string &name) {
+void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &cfg, const std::string &name,
+                  bool noinline) {
     const Layout L = layout(cp, cfg);
     const size_t const_n = cp.const_pool.size();
     const int nsteps = (int)cp.steps.size();
     const char *tn = cfg.fp64 ? "double" : "float";
-    o << "__device__ __forceinline__ void " << name << "(\n"
+    o << "__device__ " << (noinline ? "__attribute__((noinline))" : "__forceinline__") << " void " << name << "(\n"
       << "    const unsigned char *__restrict__ syn, long long count, const int *__restrict__ list, long long b0,\n"
       << "    const " << tn << " *__restrict__ tab, double *__restrict__ mar, int *__restrict__ cls,\n"
       << "    unsigned char *__restrict__ ties, int *rl, int *rcount, long long rcap, double tie_tol,\n"
@@ -1348,7 +1419,8 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
       << "    const int *__restrict__ list_count, const " << tn << " *__restrict__ gtab, double *__restrict__ mar,\n"
       << "    int *__restrict__ cls, unsigned char *__restrict__ ties, int *__restrict__ rl,\n"
       << "    int *__restrict__ rcount, long long rcap, double tie_tol, double recheck_tol, " << tn
-      << " *__restrict__ gslab, int nseg, int segcap) {\n";
+      << " *__restrict__ gslab, int nseg, int segcap, const double *__restrict__ gtab64) {\n";
+    o << "  (void)gtab64;\n";
     o << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << L.total << "];\n";
     if (cfg.tab_lds) {
         o << "  " << tn << " *tab = reinterpret_cast<" << tn << " *>(smem + " << L.tab << ");\n";

@@ -46,6 +46,12 @@ struct SpecCfg {
 // per lane than the register file holds.
 bool spec_reg_supported(const CompiledPlan &cp, std::string *why = nullptr);
 
+// The register-resident FP32 kernel re-contracts the syndromes it flags in
+// FP64 itself, at the end of each tile (one syndrome at a time over the
+// workgroup, in LDS), when that fits a modest LDS budget; otherwise the
+// separate FP64 re-check kernel follows it.
+bool spec_reg_inline_recheck(const CompiledPlan &cp, const SpecCfg &cfg);
+
 // Peak live tensor registers per lane of the register-resident variant.
 int spec_reg_peak(const CompiledPlan &cp);
 
@@ -70,11 +76,13 @@ size_t spec_slab_entries(const CompiledPlan &cp, const SpecCfg &cfg);
 // `tqec_spec_f64` (extern "C", see spec_kernel_name).  Parameters:
 //   (const uint8_t *syn, int64_t n, const int32_t *list, const int32_t *list_count,
 //    const T *tab, double *mar, int32_t *cls, uint8_t *tie,
-//    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, T *slab)
+//    int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol, T *slab,
+//    int nseg, int segcap, const double *tab64)
 // `slab` holds spec_slab_entries() entries per workgroup (slab configurations only).
 // `tab` is the constant pool followed by the sliced pool, in the kernel's
-// precision (float, or double when cfg.fp64); the FP64 variant ignores the
-// re-check arguments.
+// precision (float, or double when cfg.fp64), `tab64` the same in double
+// (the inline FP64 re-check's); the FP64 variant ignores the re-check
+// arguments.
 std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg);
 
 // One plan of the mixed-code kernel: its FP32 tile configuration, the FP64

@@ -1379,6 +1379,10 @@ SpecKernel *spec_find(tqec_plan *p, int dev, const SpecCfg &cfg) {
     return it == p->spec.end() ? nullptr : &it->second;
 }
 
+// Largest generated source handed to hiprtc (beyond it a compile takes
+// minutes; such plans stay on the generic executors).
+constexpr size_t kMaxSpecSource = size_t(2) << 20;
+
 int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     const auto key = spec_key(dev, cfg);
     auto it = p->spec.find(key);
@@ -1391,6 +1395,9 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
     int rc;
     const std::string src = spec_source(p->cp, cfg);
     if (src.empty()) return set_error(TQEC_EUNSUPPORTED, "plan does not suit this specialised kernel");
+    if (src.size() > kMaxSpecSource)
+        return set_error(TQEC_EUNSUPPORTED, "specialised kernel source of " + std::to_string(src.size() >> 10) +
+                                                " KB is beyond the compile budget; use the generic executors");
     if ((rc = spec_code(src, code))) return rc;
     SpecKernel k;
     k.cfg = cfg;
@@ -1428,7 +1435,13 @@ int spec_launch(tqec_plan *p, void *slab, size_t slab_bytes, SpecKernel *k, Devi
                 uint8_t *d_tie, int32_t *rl, int32_t *rcount, int64_t rcap, double tie_tol, double recheck_tol,
                 hipStream_t stream, int64_t nseg = 0, int32_t segcap = 0) {
     const int64_t ntiles = nseg > 0 ? nseg : (n + (1 << k->cfg.glog) - 1) >> k->cfg.glog;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, k->blocks));
+    // Register-resident kernels with tables in global memory have no per-
+    // workgroup set-up, so they get one workgroup per tile and the dispatcher
+    // balances the load (a tile that re-checks a row in FP64 takes longer);
+    // the others loop over tiles in as many workgroups as stay resident (a
+    // slab holds one workspace per resident workgroup).
+    const bool per_tile = k->cfg.reg && !k->cfg.tab_lds && !k->cfg.slab;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, per_tile ? ntiles : std::min<int64_t>(ntiles, k->blocks));
     void *a_slab = nullptr;
     if (k->cfg.slab) {
         // the caller sized the slab before enqueueing anything that uses it
@@ -1446,8 +1459,9 @@ int spec_launch(tqec_plan *p, void *slab, size_t slab_bytes, SpecKernel *k, Devi
     long long a_cap = (long long)rcap;
     double a_tt = tie_tol, a_rt = recheck_tol;
     int a_nseg = (int)nseg, a_segcap = segcap;
-    void *args[] = {&a_syn, &a_n, &a_list, &a_count, &a_tab, &a_mar, &a_cls, &a_tie, &a_rl, &a_rc, &a_cap, &a_tt, &a_rt,
-                    &a_slab, &a_nseg, &a_segcap};
+    const double *a_tab64 = dd->tab64;
+    void *args[] = {&a_syn, &a_n,  &a_list, &a_count, &a_tab, &a_mar,  &a_cls,    &a_tie,  &a_rl,
+                    &a_rc,  &a_cap, &a_tt,  &a_rt,    &a_slab, &a_nseg, &a_segcap, &a_tab64};
     TQ_HIP(hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)k->cfg.threads, 1, 1, 0, stream, args, nullptr));
     return TQEC_OK;
 }
@@ -1534,12 +1548,14 @@ int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
     }
     if (cfg.glog < 0)
         return set_error(TQEC_EUNSUPPORTED, "plan needs more LDS per syndrome than one workgroup holds (FP32)");
-    SpecKernel *k, *k64;
+    SpecKernel *k, *k64 = nullptr;
     int rc;
     if ((rc = spec_get(p, w.device, cfg, &k))) return rc;
-    if ((rc = spec_recheck_kernel(p, w.device, &k64))) return rc;
+    // the register-resident kernel may re-check its flagged rows itself
+    const bool inline_rc = spec_reg_inline_recheck(p->cp, cfg);
+    if (!inline_rc && (rc = spec_recheck_kernel(p, w.device, &k64))) return rc;
     if ((rc = grow((char **)&w.spec_slab, &w.spec_slab_bytes,
-                   std::max(spec_slab_need(p, k, n), spec_slab_need(p, k64, n)))))
+                   std::max(spec_slab_need(p, k, n), k64 ? spec_slab_need(p, k64, n) : size_t(0)))))
         return rc;
     const double rtol = bf16 ? std::max(p->recheck_tol, kBf16RecheckTol) : p->recheck_tol;
     if (cfg.reg) {
@@ -1567,11 +1583,13 @@ int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
                               p->tie_tol, rtol, stream)))
             return rc;
         if ((rc = record(p, w, 1, stream))) return rc;
-        if ((rc = record(p, w, 2, stream))) return rc;
-        if (nseg > 0 && (rc = spec_launch(p, w, k64, dd, d_syn, n, w.list, w.segc, d_mar, d_cls, d_tie, nullptr,
-                                          nullptr, 0, p->tie_tol, p->recheck_tol, stream, nseg, G)))
-            return rc;
-        if ((rc = record(p, w, 3, stream))) return rc;
+        if (!inline_rc) {
+            if ((rc = record(p, w, 2, stream))) return rc;
+            if (nseg > 0 && (rc = spec_launch(p, w, k64, dd, d_syn, n, w.list, w.segc, d_mar, d_cls, d_tie, nullptr,
+                                              nullptr, 0, p->tie_tol, p->recheck_tol, stream, nseg, G)))
+                return rc;
+            if ((rc = record(p, w, 3, stream))) return rc;
+        }
         w.count_nseg = nseg;
         w.count_on_device = true;   // read on demand by tqec_last_recheck_count
         return TQEC_OK;
@@ -2147,11 +2165,20 @@ int tqec_decode_batch_device_noalloc(tqec_plan *p, const uint8_t *d_syn, int64_t
     if (f64)
         return spec_launch(p, nullptr, 0, k, dd, d_syn, n, nullptr, nullptr, d_mar, d_cls, d_tie, nullptr, nullptr, 0,
                            p->tie_tol, p->recheck_tol, st);
+    int32_t *list = scr, *counts = scr + kNoAllocRows;
+    if (cfg.reg && spec_reg_inline_recheck(p->cp, cfg)) {
+        // flagged rows are re-contracted in FP64 inside the kernel; only the
+        // per-tile counts go to the scratch
+        const int32_t G = 1 << cfg.glog;
+        const int64_t nseg = (n + G - 1) / G;
+        if (G < 16 || nseg > kNoAllocCounts) return set_error(TQEC_EINVAL, "internal: tile too small for the scratch");
+        return spec_launch(p, nullptr, 0, k, dd, d_syn, n, nullptr, nullptr, d_mar, d_cls, d_tie, list, counts, G,
+                           p->tie_tol, p->recheck_tol, st);
+    }
     const SpecCfg rcfg = spec_recheck_cfg(p);
     SpecKernel *k64 = spec_find(p, dev, rcfg);
     if (!k64) return set_error(TQEC_EINVAL, unprepared);
     if (rcfg.slab) return set_error(TQEC_EUNSUPPORTED, "plan's FP64 re-check runs from a global slab");
-    int32_t *list = scr, *counts = scr + kNoAllocRows;
     if (cfg.reg) {
         // per-tile segments written whole by each tile (no clearing pass);
         // a tile flags only its own valid rows, so list indices stay below n

@@ -209,8 +209,11 @@ def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, 
         src = ct.hip.spec_source(threads // 4, threads, tab, reg=True, dyn=dyn)
         assert "tqec_spec_f32" in src and f"constexpr int G = {threads // 4}," in src
         assert "__builtin_amdgcn_mov_dpp" in src
-        steps = src[src.index("{ // step 0"):src.index("float rv[")]
+        body = src[src.index('extern "C" __global__'):]
+        steps = body[body.index("{ // step 0"):body.index("float rv[")]
         assert "__syncthreads" not in steps and "__shared__" not in steps   # the step program never synchronises
+        # flagged rows are re-contracted in FP64 inside the kernel, by a call of its own
+        assert "noinline)) void recheck64(" in src and "recheck64(syn, nf, flagged" in body
         # static exponents: no per-tensor maxima, exponents or rescales between steps
         assert ("lmax" in steps and "ilogbf" in steps) == dyn
         assert ("static exponents" in src) != dyn
