@@ -43,6 +43,37 @@ class GreedyMethod:
 
 
 @dataclass(frozen=True)
+class Dissection:
+    """Nested dissection of the error units (network.dissected_parity_network):
+    the parity checks rewritten as trees of rank-3 parity tensors along a
+    recursive bisection of the mechanisms or qubits, contracted in that
+    order, then refined by ``sweeps`` of tree rotations; the best of
+    ``seeds`` dissections.  Narrow where greedy and elimination orders run
+    wide (circuit-level detector error models, codes on a torus).  Works on
+    its own network, so only with evidence given as slices."""
+    seeds: int = 2
+    sweeps: int = 200
+
+
+def dissection_plan(build, opt: "Dissection"):
+    """The cheapest (multiply-adds) of ``opt.seeds`` dissected networks
+    ``build(seed) -> (graph, tree)``, annealed: ``(graph, plan, cost)``."""
+    from .planner import anneal_tree
+    best = None
+    for seed in range(max(1, opt.seeds)):
+        g, tree = build(seed)
+        if opt.sweeps > 0:
+            ref = anneal_tree(tree, g.iy, seed=seed, sweeps=opt.sweeps)
+            if tree_cost(ref)[0] < tree_cost(tree)[0]:
+                tree = ref
+        c = tree_cost(tree)
+        if best is None or (c[0], c[2]) < (best[2][0], best[2][2]):
+            best = (g, tree, c)
+    g, tree, c = best
+    return g, flatten(g.without_evidence_leaves(), tree, True), c
+
+
+@dataclass(frozen=True)
 class NoOptimizer:
     """Contract in leaf order, ``((t1 t2) t3) ...`` (tndecoder.jl:64)."""
 
@@ -189,7 +220,13 @@ def compile(decoder, tanner, em=None):
         from dataclasses import replace
         n = tanner.nq
         decoder = replace(decoder, optimizer=replace(opt, primary=tuple(v for q in range(n) for v in (q, q + n))))
-    plan = build_plan(graph, decoder)
+    if isinstance(opt, Dissection):
+        if decoder.evidence != "slices":
+            raise ValueError("the dissection optimizer builds its own network: evidence must be 'slices'")
+        from .network import dissected_css_network
+        graph, plan, _ = dissection_plan(lambda s: dissected_css_network(tanner, em, lxlz=(lx, lz), seed=s), opt)
+    else:
+        plan = build_plan(graph, decoder)
     tab = decoder.tabulate_bits
     if tab is None:
         tab = 0 if decoder.mode == "plain_f64" else 4

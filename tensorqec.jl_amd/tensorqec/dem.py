@@ -107,16 +107,40 @@ class CompiledDEMTNMMAP:
     l2q_mask: np.ndarray = None    # logicals x mechanisms
 
 
+# Reference networks with more tensors than this are not searched by the
+# greedy / elimination candidates of TreeSA (the dissection is used alone).
+_TREESA_TENSOR_LIMIT = 1200
+
+
 def compile_dem(decoder, dem: DetectorErrorModel) -> CompiledDEMTNMMAP:
-    """``compile(TNMMAP(), dem)`` (tndecoder.jl:186-219)."""
+    """``compile(TNMMAP(), dem)`` (tndecoder.jl:186-219).
+
+    With the default optimizer two networks compete: the reference's
+    (chain-factorized checks, tree from the seeded TreeSA search with a
+    time-sweep candidate) and the nested-dissection rewrite
+    (``decoders.Dissection``); the plan with fewer multiply-adds per
+    syndrome is kept.  Both give the reference's marginals (the same joint
+    distribution; only auxiliary variables differ)."""
     from dataclasses import replace
-    from .decoders import TreeSA, build_plan
+    from .decoders import Dissection, TreeSA, build_plan, dissection_plan
+    from .network import dissected_dem_network
     g = tnmmap_dem_network(dem, factorize=decoder.factorize)
     opt = decoder.optimizer
-    if isinstance(opt, TreeSA) and opt.primary is None:
-        # mechanisms in detector (time) order seed a sweep candidate
-        opt = replace(opt, primary=tuple(range(len(dem.error_rates))))
-    plan = build_plan(g, replace(decoder, optimizer=opt))
+    plan = None
+    if isinstance(opt, Dissection) or (isinstance(opt, TreeSA) and decoder.evidence == "slices"):
+        gd, pd, cd = dissection_plan(lambda s: dissected_dem_network(dem, seed=s),
+                                     opt if isinstance(opt, Dissection) else Dissection())
+        if isinstance(opt, Dissection) or len(g.ixs) > _TREESA_TENSOR_LIMIT:
+            g, plan = gd, pd
+    if plan is None:
+        if isinstance(opt, TreeSA) and opt.primary is None:
+            # mechanisms in detector (time) order seed a sweep candidate
+            opt = replace(opt, primary=tuple(range(len(dem.error_rates))))
+        plan = build_plan(g, replace(decoder, optimizer=opt))
+        st = plan.stats()
+        if isinstance(decoder.optimizer, TreeSA) and decoder.evidence == "slices" and \
+                (cd[0], cd[2]) < (st["tc"], st["sc"]):
+            g, plan = gd, pd
     tab = decoder.tabulate_bits
     if tab is None:
         tab = 0 if decoder.mode == "plain_f64" else 4

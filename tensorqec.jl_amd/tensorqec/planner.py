@@ -411,6 +411,145 @@ def sweep_order(ixs: Sequence[Sequence[int]], iy: Sequence[int], primary: Sequen
     return order
 
 
+def dissect(n: int, edges: Sequence[Sequence[int]], is_output: Sequence[bool], seed: int = 0,
+            balance: float = 0.2, refine: int = 2) -> List[Tuple[int, int]]:
+    """Binary dissection tree over items ``0..n-1`` (internal node ``n + i`` is
+    ``children[i]``, post-order) for the hypergraph ``edges`` (item lists).
+    A part is bisected along the Fiedler vector of its items' graph (items
+    sharing an edge joined, an edge of k items weighing 1/(k-1) per pair),
+    then improved by ``refine`` passes of single-item moves; the cut
+    minimises the larger side's *open* edges — edges with items on both
+    sides of that side's boundary, or with ``is_output`` set — with at least
+    ``balance`` of the part on each side."""
+    import numpy as np
+    rng = random.Random(seed)
+    item_edges: List[List[int]] = [[] for _ in range(n)]
+    for e, it in enumerate(edges):
+        for i in set(it):
+            item_edges[i].append(e)
+    total = [len(set(it)) for it in edges]
+    children: List[Tuple[int, int]] = []
+
+    def fiedler(part: List[int]) -> List[int]:
+        pos = {t: i for i, t in enumerate(part)}
+        m = len(part)
+        rows, cols, vals = [], [], []
+        seen = set()
+        for t in part:
+            for e in item_edges[t]:
+                if e in seen:
+                    continue
+                seen.add(e)
+                hs = sorted(set(pos[u] for u in edges[e] if u in pos))
+                k = len(hs)
+                if k < 2:
+                    continue
+                w = 1.0 / (k - 1)
+                for a in range(k):
+                    for b in range(a + 1, k):
+                        rows += [hs[a], hs[b]]
+                        cols += [hs[b], hs[a]]
+                        vals += [w, w]
+        from scipy.sparse import coo_matrix
+        from scipy.sparse.csgraph import connected_components
+        W = coo_matrix((vals, (rows, cols)), shape=(m, m)).tocsr()
+        jit = np.array([rng.random() for _ in range(m)])
+        ncomp, lab = connected_components(W, directed=False)
+        if ncomp > 1:
+            sizes = np.bincount(lab)
+            return [part[i] for i in sorted(range(m), key=lambda i: (-sizes[lab[i]], lab[i], jit[i]))]
+        deg = np.asarray(W.sum(axis=1)).ravel()
+        if m <= 2000:
+            _, vec = np.linalg.eigh(np.diag(deg) - W.toarray())
+            f = vec[:, 1]
+        else:
+            from scipy.sparse import diags
+            from scipy.sparse.linalg import eigsh
+            _, vec = eigsh(diags(deg) - W, k=2, sigma=-1e-6, which="LM", v0=jit)
+            f = vec[:, 1]
+        return [part[i] for i in np.lexsort((jit * 1e-9, f))]
+
+    def widths(side_of: Dict[int, int], part: List[int]):
+        """Open-edge counts of side 0 and side 1 of the part, and per edge
+        the items of the part on each side."""
+        cnt: Dict[int, List[int]] = {}
+        for t in part:
+            s = side_of[t]
+            for e in item_edges[t]:
+                c = cnt.setdefault(e, [0, 0])
+                c[s] += 1
+        w = [0, 0]
+        for e, c in cnt.items():
+            for s in (0, 1):
+                if c[s] > 0 and (c[s] < total[e] or is_output[e]):
+                    w[s] += 1
+        return w, cnt
+
+    def split(part: List[int]) -> Tuple[List[int], List[int]]:
+        order = fiedler(part)
+        m = len(order)
+        lo, hi = max(1, int(math.floor(balance * m))), min(m - 1, int(math.ceil((1 - balance) * m)))
+        side_of = {t: 1 for t in order}
+        w, cnt = widths(side_of, order)
+        best, best_i = None, lo
+        for i, t in enumerate(order[:hi], start=1):
+            for e in item_edges[t]:
+                c = cnt[e]
+                for s in (0, 1):
+                    w[s] -= c[s] > 0 and (c[s] < total[e] or is_output[e])
+                c[1] -= 1
+                c[0] += 1
+                for s in (0, 1):
+                    w[s] += c[s] > 0 and (c[s] < total[e] or is_output[e])
+            side_of[t] = 0
+            if i >= lo:
+                key = (max(w), w[0] + w[1], abs(2 * i - m))
+                if best is None or key < best:
+                    best, best_i = key, i
+        side_of = {t: (0 if j < best_i else 1) for j, t in enumerate(order)}
+        # single-item moves that lower (max width, total width), keeping balance
+        for _ in range(refine):
+            moved = False
+            for t in order:
+                s = side_of[t]
+                nside = sum(1 for u in order if side_of[u] == s)
+                if nside - 1 < max(1, int(balance * m)):
+                    continue
+                w0, _ = widths(side_of, order)
+                side_of[t] = 1 - s
+                w1, _ = widths(side_of, order)
+                if (max(w1), w1[0] + w1[1]) < (max(w0), w0[0] + w0[1]):
+                    moved = True
+                else:
+                    side_of[t] = s
+            if not moved:
+                break
+        left = [t for t in order if side_of[t] == 0]
+        right = [t for t in order if side_of[t] == 1]
+        return left, right
+
+    def build(part: List[int]) -> int:
+        if len(part) == 1:
+            return part[0]
+        if len(part) == 2:
+            a, b = part
+        else:
+            left, right = split(part)
+            a, b = build(left), build(right)
+        children.append((a, b))
+        return n + len(children) - 1
+
+    import sys
+    limit = sys.getrecursionlimit()
+    sys.setrecursionlimit(max(limit, 4 * n + 100))
+    try:
+        if n > 0:
+            build(list(range(n)))
+    finally:
+        sys.setrecursionlimit(limit)
+    return children
+
+
 def search_tree(ixs: Sequence[Sequence[int]], iy: Sequence[int], seed: int = 0, ntrials: int = 8,
                 sa_sweeps: int = 0, sc_target: Optional[int] = None,
                 objective: str = "tc", primary: Optional[Sequence[int]] = None,
