@@ -109,9 +109,12 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
     with p drawn uniformly from [0.01, 0.1]; every call decodes a window of
     batches (code drawn uniformly, size log-uniform in [1, 4096]) in one
     persistent launch of the mixed-code executor (kernel generated from every
-    plan's autotuned specialised code), replayed from a HIP graph.
-    Toric L=5/6 are left out: their exact contraction needs per-syndrome
-    tensors of rank >= 24 (64 MB per syndrome in FP32)."""
+    plan's autotuned specialised code), replayed from a HIP graph.  The
+    first timed window is decoded again, batch by batch, by the exact FP64
+    executor, and its marginals and classes are compared.  Toric L=5 is
+    measured on its own (`toric5_leg`): at 1.3 G multiply-adds per syndrome
+    one of its batches would take longer than the rest of a window; L=6
+    needs ~2^39 per syndrome."""
     import numpy as np
     from tensorqec import codes as C
     from tensorqec import decoders as D
@@ -185,6 +188,10 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
     dt = time.perf_counter() - t1
     # latency: one call at a time, per-batch device-clock latencies
     lat, call_ms, rechecks, errors, decided = [], [], 0, 0, 0
+    f64 = (torch.empty(4096 * 16, dtype=torch.float64, device=dev), torch.empty(4096, dtype=torch.int32, device=dev),
+           torch.empty(4096, dtype=torch.uint8, device=dev))
+    chk = {"reference": "exact FP64 single-plan executor (spec_f64; plain_f64 where a plan has no FP64 specialised kernel) on the first timed window's batches",
+           "syndromes": 0, "max_rel_marginal_err_vs_fp64": 0.0, "non_tie": 0, "class_agree_non_tie": 0}
     for ci in range(warmup, warmup + calls):
         h0 = time.perf_counter()
         mx.decode_device(tabs[ci][0], stream=sp)
@@ -194,13 +201,35 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
         rechecks += nre
         k = ci % 2
         cls_np = out_cls[k].cpu().numpy()
-        row = 0
+        mar_np = out_mar[k].cpu().numpy() if ci == warmup else None
+        row, mrow = 0, 0
         for b in range(ci * window, (ci + 1) * window):
             j, n, o = int(code_of[b]), int(size_of[b]), int(off_of[b])
             tr = info[j]["truth"][o:o + n].cpu().numpy()
             errors += int((cls_np[row:row + n] != tr).sum())
             decided += n
+            if mar_np is not None:
+                no = info[j]["nout"]
+                try:
+                    plans[j].hip.decode_device(info[j]["S"][o:].data_ptr(), n, "spec_f64", f64[0].data_ptr(),
+                                               f64[1].data_ptr(), f64[2].data_ptr())
+                except L.TqecError:
+                    plans[j].hip.decode_device(info[j]["S"][o:].data_ptr(), n, "plain_f64", f64[0].data_ptr(),
+                                               f64[1].data_ptr(), f64[2].data_ptr())
+                torch.cuda.synchronize()
+                m64 = f64[0][:n * no].cpu().numpy().reshape(n, no)
+                c64, t64 = f64[1][:n].cpu().numpy(), f64[2][:n].cpu().numpy()
+                m32 = mar_np[mrow:mrow + n * no].reshape(n, no)
+                valid = m64.max(axis=1) > 0
+                if valid.any():
+                    e = np.abs(m32[valid] - m64[valid]) / m64[valid].max(axis=1, keepdims=True)
+                    chk["max_rel_marginal_err_vs_fp64"] = max(chk["max_rel_marginal_err_vs_fp64"], float(e.max()))
+                ok = t64 == 0
+                chk["syndromes"] += n
+                chk["non_tie"] += int(ok.sum())
+                chk["class_agree_non_tie"] += int((cls_np[row:row + n][ok] == c64[ok]).sum())
             row += n
+            mrow += n * info[j]["nout"]
     # the same batches through the generic (descriptor-interpreting) mixed kernel
     mx.set_kernel(False)
     for ci in range(warmup):
@@ -236,6 +265,9 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192):
         "batch_latency_us_p50": pct(lat, 50), "batch_latency_us_p99": pct(lat, 99),
         "call_latency_ms_p50": pct(call_ms, 50), "call_latency_ms_p99": pct(call_ms, 99),
         "fp64_rechecks": rechecks, "logical_error_rate_all_codes": errors / max(decided, 1),
+        "fp64_check": {"reference": chk["reference"], "syndromes": chk["syndromes"],
+                       "max_rel_marginal_err_vs_fp64": chk["max_rel_marginal_err_vs_fp64"],
+                       "class_agreement_non_tie": chk["class_agree_non_tie"] / max(chk["non_tie"], 1)},
         "generic_mixed_kernel_syndromes_per_s": rows_timed / dt_gen,
         "separate_launches_syndromes_per_s": rows_timed / dt_sep,
         "separate_launches": "one specialised single-plan launch (plus its FP64 re-check) per batch",
@@ -254,13 +286,11 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
     out.  Reported against FP64 TN decisions on the same syndromes: the plain
     executor, or (``ref_mode="spec_f64"``, for codes where the plain one is
     too slow) the exact FP64 specialised kernel on the same plan."""
-    import numpy as np
     from tensorqec import decoders as D
     from tensorqec import lib as L
     from tensorqec.codes import CSSTannerGraph, SurfaceCode
     from tensorqec.errors import iid_error
     from tensorqec.gf2 import logical_operator
-    from tensorqec.learned import GatedDecoder, HipMLP, train_on_tn
 
     t = CSSTannerGraph.from_code(SurfaceCode(d, d))
     em = iid_error(p / 3, p / 3, p / 3, t.nq)
@@ -271,13 +301,49 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
         plain = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64"), t, em)
     lx, lz = logical_operator(t)
     smp = L.HipSampler.css(t, em, lx, lz)
-    mlp = HipMLP(t.ns, 4, seed=seed % 1000)
+    out = learned_core(torch, dev, seed, ct, smp, t.ns, 4, mode, n_train, train_seconds, batch, eval_batches,
+                       ref_mode, plain)
+    return dict({"workload": f"learned front-end, rotated surface code d={d}, code-capacity p={p}: MLP "
+                             f"{t.ns}-512-512-512-4 (BF16 weights, FP32 accumulation), trained on TN-labelled "
+                             f"syndromes, confidence-gated, TN ({mode}) for the rest"}, **out)
+
+
+def learned_dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n_train=2_000_000, train_seconds=60.0):
+    """The learned front-end on a circuit-level detector error model (memory-Z,
+    rotated surface code, depolarizing p after every gate, reset and
+    measurement flips p): one input per detector, 2 classes (logical flip or
+    not), trained on syndromes labelled by the TN decoder of the DEM network,
+    gated, judged against the exact FP64 specialised kernel on the same
+    plan."""
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    dem = detector_error_model_from_stim(rotated_memory_z(d, rounds, p))
+    ct = D.compile(D.TNMMAP(mode="spec_f32"), dem)
+    smp = L.HipSampler.flips(dem.error_rates, dem.detector_matrix(), dem.logical_matrix())
+    S = torch.empty((16384, smp.nsyn), dtype=torch.uint8, device=dev)
+    tr = torch.empty(16384, dtype=torch.int32, device=dev)
+    smp.sample(seed + 4, 0, 16384, S.data_ptr(), tr.data_ptr())
+    ct.hip.autotune("spec_f32", S.data_ptr(), 16384)
+    out = learned_core(torch, dev, seed, ct, smp, smp.nsyn, 2, "spec_f32", n_train, train_seconds, 65536, 4,
+                       "spec_f64", None)
+    return dict({"workload": f"learned front-end, circuit-level memory-Z, rotated surface code d={d}, {rounds} "
+                             f"rounds, p={p}: MLP {smp.nsyn}-512-512-512-2 over the detectors, trained on "
+                             "TN-labelled syndromes, confidence-gated, TN (spec_f32) for the rest",
+                 "tn_macs_per_syndrome": ct.hip.stats()["dep_tc"], "tn_config": ct.hip.spec_config()}, **out)
+
+
+def learned_core(torch, dev, seed, ct, smp, n_in, n_classes, mode, n_train, train_seconds, batch, eval_batches,
+                 ref_mode, plain):
+    """Train, calibrate the gate, evaluate on held-out batches (see learned_leg)."""
+    from tensorqec.learned import GatedDecoder, HipMLP, train_on_tn
+    mlp = HipMLP(n_in, n_classes, seed=seed % 1000)
     rep = train_on_tn(mlp, ct, smp, n_samples=n_train, batch=8192, lr=2e-3, lr_final=1e-4, seed=seed + 5,
                       torch=torch, mode=mode, max_seconds=train_seconds)
     stream = torch.cuda.current_stream().cuda_stream
-    S = torch.empty((batch, t.ns), dtype=torch.uint8, device=dev)
+    S = torch.empty((batch, n_in), dtype=torch.uint8, device=dev)
     truth = torch.empty(batch, dtype=torch.int32, device=dev)
-    mar = torch.empty((batch, 4), dtype=torch.float64, device=dev)
+    mar = torch.empty((batch, n_classes), dtype=torch.float64, device=dev)
     cls_tn = torch.empty(batch, dtype=torch.int32, device=dev)
     cls_g = torch.empty(batch, dtype=torch.int32, device=dev)
     conf = torch.empty(batch, dtype=torch.float32, device=dev)
@@ -332,9 +398,6 @@ def learned_leg(torch, dev, seed, ct=None, mode="fused_f32", d=7, p=0.05, n_trai
     ci = wilson(res["err_fp64"], n)
     ler_g = res["err_g"] / n
     return {
-        "workload": f"learned front-end, rotated surface code d={d}, code-capacity p={p}: MLP {t.ns}-512-512-512-4 "
-                    "(BF16 weights, FP32 accumulation), trained on TN-labelled syndromes, confidence-gated, "
-                    f"TN ({mode}) for the rest",
         "train_samples": rep.samples, "train_steps": rep.steps, "train_s": rep.seconds,
         "train_samples_per_s": rep.samples / max(rep.seconds, 1e-9),
         "loss_first_last": [rep.losses[0][1], rep.losses[-1][1]] if rep.losses else None,
@@ -472,33 +535,44 @@ def tnmap_leg(torch, dev, seed, d=5, p=0.05, batch=16384, reps=3):
 
 def mlp_leg(torch, dev, seed, n_in=48, batch=65536, reps=20):
     """Inference of the learned front-end alone: the 48-512-512-512-4 MLP
-    (random weights) on a batch of syndromes, forward + confidence gate."""
+    (random weights) on a batch of syndromes, with the fused kernel (all
+    layers per 128-row tile, activations in LDS) and with the
+    layer-by-layer kernels training uses.  Algorithmic flops count the
+    padded input width the GEMMs run (kin) and the 4 real output columns."""
     from tensorqec.learned import HipMLP
     mlp = HipMLP(n_in, 4, seed=seed % 1000)
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     S = (torch.rand((batch, n_in), device=dev, generator=g) < 0.1).to(torch.uint8)
     z = torch.empty((batch, 4), dtype=torch.float32, device=dev)
-    for _ in range(3):
-        mlp.forward(S.data_ptr(), batch, z.data_ptr())
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        mlp.forward(S.data_ptr(), batch, z.data_ptr())
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
     flops = 2.0 * batch * (mlp.kin * 512 + 2 * 512 * 512 + 512 * 4)
-    return {"workload": f"MLP {n_in}-512-512-512-4 forward, BF16 weights, FP32 accumulation, {batch} syndromes",
-            "ms_per_batch": dt * 1e3, "syndromes_per_s": batch / dt, "tflops": flops / dt / 1e12,
-            "frac_of_bf16_peak": flops / dt / 2.5e15}
+    out = {"workload": f"MLP {n_in}-512-512-512-4 forward, BF16 weights, FP32 accumulation, {batch} syndromes",
+           "gflop_per_batch": flops / 1e9}
+    for name, kern in (("fused", 1), ("layer_by_layer", 0)):
+        mlp.set_inference(kern)
+        for _ in range(3):
+            mlp.forward(S.data_ptr(), batch, z.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mlp.forward(S.data_ptr(), batch, z.data_ptr())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"ms_per_batch": dt * 1e3, "syndromes_per_s": batch / dt, "tflops": flops / dt / 1e12,
+                     "frac_of_bf16_peak": flops / dt / 2.5e15}
+    mlp.set_inference(1)
+    return out
 
 
-def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
+def dem_leg(torch, dev, seed, d=5, rounds=2, p=0.001, n=16384, reps=3):
     """Circuit-level memory-Z experiment (rotated surface code, depolarizing
     p after every 1- and 2-qubit gate, reset and measurement flips p),
     compiled to a detector error model by the library's own generator and
-    decoded by TNMMAP over the DEM network; FP32 executors against plain FP64
-    on the same seeded syndromes."""
+    decoded by TNMMAP over the DEM network (the cheaper of the reference's
+    chain-factorized network and its nested-dissection rewrite); FP32 and
+    BF16-MFMA executors against plain FP64 on the same seeded syndromes.
+    d=5 with 2 rounds is the largest round count with an exact contraction
+    in reach (21 M multiply-adds per syndrome; 3 rounds need ~2^39)."""
     import numpy as np
     from tensorqec import decoders as D
     from tensorqec import lib as L
@@ -522,15 +596,18 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
            "plan": {k: st[k] for k in ("sc", "dep_tc", "dep_rw", "plain_ws_entries", "n_dep_steps")},
            "compile_s": compile_s, "modes": {}}
     t1 = time.time()
+    if st["sc"] <= 16:
+        # plans that fit the LDS tiles are autotuned; wider ones (slabs) keep
+        # the default configuration
+        for m in ("spec_f32", "spec_bf16"):
+            try:
+                ct.hip.autotune(m, S.data_ptr(), min(n, 4096), stream)
+            except L.TqecError:
+                pass
     try:
-        ct.hip.autotune("spec_f32", S.data_ptr(), min(n, 4096), stream)
         out["spec_config"] = ct.hip.spec_config()
     except L.TqecError as e:
         out["spec_config"] = str(e)
-    try:
-        ct.hip.autotune("spec_bf16", S.data_ptr(), min(n, 4096), stream)
-    except L.TqecError:
-        pass
     out["spec_tune_s"] = time.time() - t1
     ref = None
     for mode in ("plain_f64", "wide_f32", "spec_f32", "spec_bf16"):
@@ -564,6 +641,80 @@ def dem_leg(torch, dev, seed, d=3, rounds=3, p=0.001, n=16384, reps=5):
     return out
 
 
+def toric5_leg(torch, dev, seed, L=5, p=0.03, n=2048, reps=1):
+    """Toric code L=5 (50 qubits, 2 logical qubits, 16 classes) under
+    code-capacity depolarizing noise px = py = pz = p, decoded exactly: the
+    nested-dissection network (width 2^21, 1.33 G multiply-adds per
+    syndrome) in FP64 (plain executor) and FP32 (wide and specialised
+    executors, FP64 re-check of low margins), on the same device-sampled
+    syndromes; logical error rate against the sampled class."""
+    import numpy as np
+    from tensorqec import decoders as D
+    from tensorqec import lib as L_
+    from tensorqec.codes import CSSTannerGraph, ToricCode
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+    t0 = time.time()
+    t = CSSTannerGraph.from_code(ToricCode(L, L))
+    em = iid_error(p, p, p, t.nq)
+    ct = D.compile(D.TNMMAP(mode="plain_f64", optimizer=D.Dissection(), tabulate_bits=0), t, em)
+    st = ct.hip.stats()
+    compile_s = time.time() - t0
+    lx, lz = logical_operator(t)
+    smp = L_.HipSampler.css(t, em, lx, lz)
+    S = torch.empty((n, t.ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    smp.sample(seed + 17, 0, n, S.data_ptr(), truth.data_ptr(), stream=stream)
+    tr = truth.cpu().numpy()
+    nout = 1 << ct.hip.n_out
+    out = {"workload": f"toric code L={L}, code-capacity depolarizing px=py=pz={p}, {n} syndromes, "
+                       f"nested-dissection network",
+           "plan": {k: st[k] for k in ("sc", "dep_tc", "dep_rw", "plain_ws_entries", "n_dep_steps")},
+           "compile_s": compile_s, "modes": {}}
+    ref = None
+    for mode in ("plain_f64", "wide_f32", "spec_f32"):
+        mar = torch.empty((n, nout), dtype=torch.float64, device=dev)
+        cls = torch.empty(n, dtype=torch.int32, device=dev)
+        tie = torch.empty(n, dtype=torch.uint8, device=dev)
+        try:
+            ct.hip.decode_device(S.data_ptr(), n, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), stream)
+        except L_.TqecError as e:
+            out["modes"][mode] = {"unsupported": str(e)}
+            continue
+        torch.cuda.synchronize()
+        h0 = time.perf_counter()
+        for _ in range(reps):
+            ct.hip.decode_device(S.data_ptr(), n, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - h0) / reps
+        m, c, ti = mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy()
+        r = {"syndromes_per_s": n / dt, "tflops": 2.0 * st["dep_tc"] * n / dt / 1e12, "ler": float((c != tr).mean())}
+        if ref is None:
+            ref = (m, c, ti)
+        else:
+            valid = ref[0].max(axis=1) > 0
+            r["max_rel_marginal_err_vs_fp64"] = float(np.max(np.abs(m[valid] - ref[0][valid]) /
+                                                             ref[0][valid].max(axis=1, keepdims=True)))
+            ok = ref[2] == 0
+            r["class_agreement_non_tie"] = float((c[ok] == ref[1][ok]).mean())
+        out["modes"][mode] = r
+    return out
+
+
+def copy_calibration(torch, dev, mib):
+    """Kernels of known traffic: a fill writes and a sum reads `mib` MiB
+    once; their WRITE_SIZE / FETCH_SIZE counters calibrate the HBM byte
+    counters of the decode kernels in the same profiled run."""
+    if mib <= 0:
+        return
+    src = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
+    src.fill_(1.0)
+    float(src.sum().item())
+    torch.cuda.synchronize()
+    del src
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -584,12 +735,13 @@ def main():
                          "REG 1 = register-resident specialised kernel (TILE = THREADS / 4)")
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the secondary workloads (uneven stream, learned front-end, circuit-level DEM)")
+                    help="skip the secondary workloads (uneven stream, learned front-ends, circuit-level DEM, "
+                         "toric L=5, MLP, device loop, TNMAP)")
     ap.add_argument("--only", default=None,
-                    choices=["stream", "learned", "learned_d9", "dem", "mlp", "pipeline", "tnmap"],
+                    choices=["stream", "learned", "learned_d9", "learned_dem", "dem", "mlp", "pipeline", "tnmap", "toric5"],
                     help="run one secondary workload alone and print its JSON")
-    ap.add_argument("--dem-d", type=int, default=3, help="distance of the circuit-level DEM workload")
-    ap.add_argument("--dem-rounds", type=int, default=3, help="syndrome rounds of the circuit-level DEM workload")
+    ap.add_argument("--dem-d", type=int, default=5, help="distance of the circuit-level DEM workload")
+    ap.add_argument("--dem-rounds", type=int, default=2, help="syndrome rounds of the circuit-level DEM workload")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed steps (HBM counter calibration)")
     args = ap.parse_args()
@@ -603,9 +755,11 @@ def main():
     import torch
 
     if args.only:
+        copy_calibration(torch, torch.device("cuda:0"), args.copy_calibration)
         kw = {"dem": {"d": args.dem_d, "rounds": args.dem_rounds}}.get(args.only, {})
-        fn = {"stream": stream_leg, "learned": learned_leg, "learned_d9": learned_d9_leg, "dem": dem_leg,
-              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg}[args.only]
+        fn = {"stream": stream_leg, "learned": learned_leg, "learned_d9": learned_d9_leg, "learned_dem": learned_dem_leg,
+              "dem": dem_leg,
+              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg, "toric5": toric5_leg}[args.only]
         print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 
@@ -733,16 +887,7 @@ def main():
     cls = torch.empty(B, dtype=torch.int32, device=dev)
     tie = torch.empty(B, dtype=torch.uint8, device=dev)
     print(f"[bench] {nb} batches of {B} syndromes resident", file=sys.stderr, flush=True)
-    if args.copy_calibration > 0:
-        # kernels of known traffic: a fill writes and a sum reads the given
-        # size once; their WRITE_SIZE / FETCH_SIZE counters calibrate the HBM
-        # byte counters of the decode kernels
-        n_el = args.copy_calibration * (1 << 20) // 4
-        src = torch.empty(n_el, dtype=torch.float32, device=dev)
-        src.fill_(1.0)
-        float(src.sum().item())
-        torch.cuda.synchronize()
-        del src
+    copy_calibration(torch, dev, args.copy_calibration)
 
     def run(ct, mode, label, warmup=None, instrument=True):
         """Warm-up, then the timed region (K steps, no instrumentation:
@@ -885,15 +1030,27 @@ def main():
         print(f"[bench] learned d=9 leg {time.time() - t0:.1f}s: gated {res['learned_d9']['gated_syndromes_per_s']:.0f} "
               f"vs TN {res['learned_d9']['tn_only_syndromes_per_s']:.0f} syn/s", file=sys.stderr, flush=True)
         t0 = time.time()
+        res["learned_dem"] = learned_dem_leg(torch, dev, args.seed)
+        print(f"[bench] learned DEM leg {time.time() - t0:.1f}s: fallback {res['learned_dem']['fallback_fraction']:.4f}, "
+              f"gated {res['learned_dem']['gated_syndromes_per_s']:.0f} vs TN "
+              f"{res['learned_dem']['tn_only_syndromes_per_s']:.0f} syn/s", file=sys.stderr, flush=True)
+        t0 = time.time()
         res["pipeline"] = pipeline_leg(torch, dev, args.seed, ct=chosen, mode=mode_c, d=d, p=p)
         print(f"[bench] pipeline leg {time.time() - t0:.1f}s: {res['pipeline']['syndromes_per_s']:.0f} syn/s",
               file=sys.stderr, flush=True)
+        t0 = time.time()
+        res["mlp"] = mlp_leg(torch, dev, args.seed)
+        print(f"[bench] mlp leg {time.time() - t0:.1f}s: fused {res['mlp']['fused']['tflops']:.0f} TFLOP/s, "
+              f"layer by layer {res['mlp']['layer_by_layer']['tflops']:.0f} TFLOP/s", file=sys.stderr, flush=True)
         t0 = time.time()
         res["tnmap"] = tnmap_leg(torch, dev, args.seed)
         print(f"[bench] tnmap leg {time.time() - t0:.1f}s: {res['tnmap']}", file=sys.stderr, flush=True)
         t0 = time.time()
         res["dem"] = dem_leg(torch, dev, args.seed)
         print(f"[bench] dem leg {time.time() - t0:.1f}s: {res['dem']['modes']}", file=sys.stderr, flush=True)
+        t0 = time.time()
+        res["toric5"] = toric5_leg(torch, dev, args.seed)
+        print(f"[bench] toric5 leg {time.time() - t0:.1f}s: {res['toric5']['modes']}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
