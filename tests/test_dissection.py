@@ -111,3 +111,17 @@ def test_compile_keeps_the_cheaper_network():
     plan_r = D.build_plan(gr, D.TNMMAP(optimizer=D.TreeSA(primary=tuple(range(len(dem.error_rates))))))
     auto = D.compile(D.TNMMAP(), dem)
     assert auto.plan.stats()["tc"] == min(ct_d.plan.stats()["tc"], plan_r.stats()["tc"])
+
+
+def test_toric5_fixture_inputs():
+    """tests/golden/toric5.npz (the oracle's exact marginals of toric L=5 that
+    the GPU tests check against): its syndromes are the seeded sampler's, and
+    its marginals are probabilities of those syndromes (positive, summing to
+    at most one over the 16 classes)."""
+    g = np.load(os.path.join(GOLD, "toric5.npz"))
+    t = CSSTannerGraph.from_code(ToricCode(5, 5))
+    p = float(g["p"])
+    S = css_syndrome_rows(sample_depolarizing(iid_error(p, p, p, t.nq), seed=5, count=16), t)
+    assert np.array_equal(S, g["syn"])
+    assert g["mar"].shape == (16, 16)
+    assert np.all(g["mar"] > 0) and np.all(g["mar"].sum(axis=1) <= 1.0)

@@ -10,6 +10,10 @@ src/decoding/tndecoder.jl:160-165, evaluated for every syndrome).
     register-resident kernel, against the CPU oracle's FP64 executor on
     the same plan and the same syndromes.
   * d=9 on 2,048 syndromes: every fused executor the plan fits.
+  * toric code L=5 (16 classes), nested-dissection network of width 2^21:
+    the plain FP64, wide FP32 and specialised FP32 executors on the 16
+    syndromes of tests/golden/toric5.npz (the C oracle's marginals, written
+    by tests/golden/make_toric5.py; about a CPU-minute per syndrome).
 Tolerances: tests/_parity.py (FP64 1e-12 relative with exact zeros, tie
 flags equal; FP32 2e-5 of the row maximum; classes bit-exact on every
 non-tie syndrome).
@@ -23,7 +27,7 @@ from _parity import check_f32, check_f64
 from oracle.cexec import OraclePlan
 from tensorqec import decoders as D
 from tensorqec import lib as L
-from tensorqec.codes import CSSTannerGraph, SurfaceCode
+from tensorqec.codes import CSSTannerGraph, SurfaceCode, ToricCode
 from tensorqec.errors import css_syndrome_rows, iid_error, sample_depolarizing
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +97,27 @@ def test_d9_vs_oracle():
         else:
             err = check_f64(m, c, tie, ref, f"d9 {mode}")
         say(f"  max err {err:.2e}")
+
+
+@pytest.mark.parametrize("mode", ["plain_f64", "wide_f32", "spec_f32"])
+def test_toric5_vs_oracle_fixture(mode):
+    import os
+    torch = pytest.importorskip("torch")
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "toric5.npz"))
+    S, ref, p = g["syn"], g["mar"], float(g["p"])
+    say(f"case toric L=5 {mode}: {len(S)} syndromes")
+    t = CSSTannerGraph.from_code(ToricCode(5, 5))
+    em = iid_error(p, p, p, t.nq)
+    ct = D.compile(D.TNMMAP(mode=mode, optimizer=D.Dissection(), tabulate_bits=0), t, em)
+    assert ct.hip.stats()["sc"] <= 22
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
+    mar, cls, tie = run(torch, ct, dS, len(S), mode)
+    if mode == "plain_f64":
+        check_f64(mar, cls, tie, ref, mode)
+    else:
+        # the plan's last step contracts 18 labels: every marginal is one
+        # FP32 accumulation chain of 2^18 products, whose rounding reaches
+        # ~2.5e-5 of the row maximum, so this plan is held to 1e-4; classes
+        # stay exact on every non-tie syndrome (FP64 re-check below a 1e-3
+        # margin)
+        check_f32(mar, cls, ref, mode, tol=1e-4)
