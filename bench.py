@@ -665,16 +665,10 @@ def toric5_leg(torch, dev, seed, L=5, p=0.03, n=2048, reps=1):
     smp.sample(seed + 17, 0, n, S.data_ptr(), truth.data_ptr(), stream=stream)
     tr = truth.cpu().numpy()
     nout = 1 << ct.hip.n_out
-    t1 = time.time()
-    try:
-        ct.hip.autotune("spec_f32", S.data_ptr(), min(n, 256), stream)
-    except L_.TqecError:
-        pass
-    tune_s = time.time() - t1
     out = {"workload": f"toric code L={L}, code-capacity depolarizing px=py=pz={p}, {n} syndromes, "
                        f"nested-dissection network",
            "plan": {k: st[k] for k in ("sc", "dep_tc", "dep_rw", "plain_ws_entries", "n_dep_steps")},
-           "compile_s": compile_s, "spec_tune_s": tune_s, "spec_config": ct.hip.spec_config(), "modes": {}}
+           "compile_s": compile_s, "spec_config": ct.hip.spec_config(), "modes": {}}
     ref = None
     for mode in ("plain_f64", "wide_f32", "spec_f32"):
         mar = torch.empty((n, nout), dtype=torch.float64, device=dev)

@@ -350,6 +350,11 @@ void tqec_mixed_destroy(tqec_mixed *m);
                                       from each operand's maximum instead, the
                                       generic FP32 kernel's arithmetic
                                       (bit-identical to TQEC_MODE_FUSED_F32)        */
+#define TQEC_SPEC_MFMA32 64        /* SPEC_F32 tiles of one syndrome (LDS or slab):
+                                      GEMM-shaped steps (M, N >= 32, K >= 2 per
+                                      syndrome) on FP32 MFMA with FP32 operands;
+                                      within the FP32 tolerance, not bit-identical
+                                      (summation order)                              */
 
 /* Source of the plan-specialised kernel for a tile configuration
  * (inspection and offline compilation checks); `flags` as above, with the
@@ -360,7 +365,8 @@ int tqec_plan_spec_source(const tqec_plan *plan, int32_t tile, int32_t threads, 
                           size_t cap, size_t *len);
 
 /* Pin / read the TQEC_MODE_SPEC_F32 configuration: tile, threads,
- * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB | TQEC_SPEC_REG | TQEC_SPEC_DYNSCALE flags, and `waves` = minimum
+ * TQEC_SPEC_TABLES_IN_LDS | TQEC_SPEC_SLAB | TQEC_SPEC_REG | TQEC_SPEC_DYNSCALE |
+ * TQEC_SPEC_MFMA32 flags, and `waves` = minimum
  * waves per SIMD requested from the compiler (bits 0-7, 0 = none) | unroll
  * of the per-lane output loop << 8 (1, 2, 4 or 8; 0 = 1).  No reference
  * counterpart. */

@@ -358,6 +358,110 @@ void emit_outputs_mfma(std::ostringstream &o, const StepDesc &st, int threads) {
     o << "        }\n      }\n    }\n";
 }
 
+// GEMM-shaped step on FP32 MFMA (v_mfma_f32_32x32x2f32), full FP32 operands:
+// per syndrome C[i, j] = sum_k A[i, k] B[k, j] as above, each wave owning a
+// block of ra x rb tiles of 32 x 32 (up to two more A-only and B-only output
+// bits index the tiles of a block), so every operand value a lane loads
+// feeds rb (or ra) MFMAs.  The k loop takes 2 ku contracted values per
+// iteration and loads the next iteration's operands before issuing its own
+// MFMAs.  Layouts (wave64):
+// lane l holds A[l % 32][l / 32], B[l / 32][l % 32] and
+// D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] for v = 0..15.  Every index stays
+// inside the operand and output ranges by construction (i, j < 32 over five
+// output bits each, k < 2^nk), so a layout mistake shows as wrong values,
+// never as an out-of-range access.
+void emit_outputs_mfma32(std::ostringstream &o, const StepDesc &st, int threads) {
+    const OperandDesc &A = st.a, &B = st.b;
+    const uint32_t am = source_mask(A), bm = source_mask(B);
+    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
+    const std::vector<int> aon = bits_of(am & ~bm & outm), bon = bits_of(bm & ~am & outm);
+    const std::vector<int> ai(aon.begin(), aon.begin() + 5), bj(bon.begin(), bon.begin() + 5);
+    const int xa = std::min<int>(1, (int)aon.size() - 5), xb = std::min<int>(1, (int)bon.size() - 5);
+    const std::vector<int> a2(aon.begin() + 5, aon.begin() + 5 + xa), b2(bon.begin() + 5, bon.begin() + 5 + xb);
+    std::vector<int> rest;
+    for (int b = 0; b < st.rc; ++b) {
+        const bool used = std::find(ai.begin(), ai.end(), b) != ai.end() ||
+                          std::find(bj.begin(), bj.end(), b) != bj.end() ||
+                          std::find(a2.begin(), a2.end(), b) != a2.end() ||
+                          std::find(b2.begin(), b2.end(), b) != b2.end();
+        if (!used) rest.push_back(b);
+    }
+    const int ra = 1 << xa, rb = 1 << xb;
+    const unsigned ntiles = 1u << rest.size();
+    const int nwaves = threads / 64;
+    const unsigned nkk = 1u << st.nk;
+    const unsigned ku = std::min(4u, nkk / 2);   // k pairs per loop iteration
+    const std::string ga = gather_expr(A), gb = gather_expr(B);
+    auto ld = [&](const OperandDesc &op, const char *base, const std::string &idx, const std::string &k) {
+        const std::string e = idx + " | ((" + k + ") << " + std::to_string(op.kshift) + ")";
+        if (op.kind == kInter) return std::string(base) + "[(" + e + ") << GLOG]";
+        return std::string(base) + "[" + e + "]";
+    };
+    o << "    {\n";
+    o << "      const int l = tid & 63, w = tid >> 6;\n";
+    o << "      #pragma unroll 1\n";
+    o << "      for (unsigned t = (unsigned)w; t < " << ntiles << "u; t += " << nwaves << "u) {\n";
+    o << "        const unsigned tb = " << deposit("t", rest) << ";\n";
+    for (int x = 0; x < ra; ++x)
+        o << "        const unsigned ar" << x << " = "
+          << subst_c(ga, "(tb | " + deposit("(unsigned)(l & 31)", ai) + " | " + deposit(std::to_string(x) + "u", a2) +
+                             ")")
+          << ";\n";
+    for (int y = 0; y < rb; ++y)
+        o << "        const unsigned br" << y << " = "
+          << subst_c(gb, "(tb | " + deposit("(unsigned)(l & 31)", bj) + " | " + deposit(std::to_string(y) + "u", b2) +
+                             ")")
+          << ";\n";
+    for (int x = 0; x < ra; ++x)
+        for (int y = 0; y < rb; ++y) o << "        f32x16 acc" << x << "_" << y << " = {};\n";
+    // operands of the next k-block are loaded before this block's MFMAs
+    // issue, so their latency hides behind the matrix work (one wave per
+    // SIMD leaves no other wave to hide it)
+    auto loads = [&](const std::string &kbase, const char *pre, bool decl) {
+        for (unsigned u = 0; u < ku; ++u) {
+            const std::string k = kbase + " + " + std::to_string(2 * u) + "u";
+            for (int x = 0; x < ra; ++x)
+                o << "          " << (decl ? "float " : "") << pre << "a" << u << "_" << x << " = "
+                  << ld(A, "pa", "ar" + std::to_string(x), k) << ";\n";
+            for (int y = 0; y < rb; ++y)
+                o << "          " << (decl ? "float " : "") << pre << "b" << u << "_" << y << " = "
+                  << ld(B, "pb", "br" + std::to_string(y), k) << ";\n";
+        }
+    };
+    o << "        {\n";
+    loads("(unsigned)(l >> 5)", "n", true);
+    o << "        #pragma unroll 1\n";
+    o << "        for (unsigned kb = (unsigned)(l >> 5); kb < " << nkk << "u; kb += " << 2 * ku << "u) {\n";
+    for (unsigned u = 0; u < ku; ++u) {
+        for (int x = 0; x < ra; ++x) o << "          const float a" << u << "_" << x << " = na" << u << "_" << x << ";\n";
+        for (int y = 0; y < rb; ++y) o << "          const float b" << u << "_" << y << " = nb" << u << "_" << y << ";\n";
+    }
+    // the same for both halves of the wave (kb differs by l >> 5 < 2): uniform
+    o << "          if (kb + " << 2 * ku << "u < " << nkk << "u) {\n";
+    loads("kb + " + std::to_string(2 * ku) + "u", "n", false);
+    o << "          }\n";
+    for (unsigned u = 0; u < ku; ++u)
+        for (int x = 0; x < ra; ++x)
+            for (int y = 0; y < rb; ++y)
+                o << "          acc" << x << "_" << y << " = __builtin_amdgcn_mfma_f32_32x32x2f32(a" << u << "_" << x
+                  << ", b" << u << "_" << y << ", acc" << x << "_" << y << ", 0, 0, 0);\n";
+    o << "        }\n";
+    o << "        }\n";
+    for (int x = 0; x < ra; ++x)
+        for (int y = 0; y < rb; ++y) {
+            o << "        #pragma unroll\n";
+            o << "        for (int v = 0; v < 16; ++v) {\n";
+            o << "          const unsigned i = 8u * (unsigned)(v >> 2) + 4u * (unsigned)(l >> 5) + (unsigned)(v & 3);\n";
+            o << "          const unsigned c = tb | " << deposit("i", ai) << " | " << deposit(std::to_string(x) + "u", a2)
+              << " | " << deposit("(unsigned)(l & 31)", bj) << " | " << deposit(std::to_string(y) + "u", b2) << ";\n";
+            o << "          const float x = ldexpf(acc" << x << "_" << y << "[v], shift);\n";
+            o << "          lmax = x > lmax ? x : lmax;\n";
+            o << "          pc[c << GLOG] = x;\n";
+            o << "        }\n";
+        }
+    o << "      }\n    }\n";
+}
+
 void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64, int unroll,
                const SpecCfg *mfma_cfg) {
     const OperandDesc &A = st.a, &B = st.b;
@@ -398,6 +502,8 @@ void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n,
     o << "    T lmax = 0;\n";
     if (mfma_cfg && spec_step_uses_mfma(st, *mfma_cfg))
         emit_outputs_mfma(o, st, mfma_cfg->threads);
+    else if (mfma_cfg && spec_step_uses_mfma32(st, *mfma_cfg))
+        emit_outputs_mfma32(o, st, mfma_cfg->threads);
     else
         emit_outputs(o, st, cstep, unroll, false);
     o << "    for (int w = G; w < 64; w <<= 1) { const float other = __shfl_xor(lmax, w); lmax = other > lmax ? other : lmax; }\n";
@@ -1333,7 +1439,7 @@ void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &
     o << "  __syncthreads();\n";
     for (int s = 0; s < nsteps; ++s)
         emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64, std::max(1, cfg.unroll),
-                  (cfg.mfma && !cfg.fp64) ? &cfg : nullptr);
+                  ((cfg.mfma || cfg.mfma32) && !cfg.fp64) ? &cfg : nullptr);
     o << "  if (tid < G) {\n"
          "    const long long q = b0 + tid;\n"
          "    if (q < count) {\n"
@@ -1390,6 +1496,21 @@ const char *kMfmaPrelude =
 
 }  // namespace
 
+bool spec_step_uses_mfma32(const StepDesc &st, const SpecCfg &cfg) {
+    if (!cfg.mfma32 || cfg.mfma || cfg.fp64 || cfg.reg || cfg.glog != 0 || cfg.threads % 64 != 0) return false;
+    if (st.nk < 1) return false;
+    if ((st.a.kind == kInter && st.a.nsingle) || (st.b.kind == kInter && st.b.nsingle)) return false;
+    const uint32_t am = source_mask(st.a), bm = source_mask(st.b);
+    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
+    return __builtin_popcount(am & ~bm & outm) >= 5 && __builtin_popcount(bm & ~am & outm) >= 5;
+}
+
+bool spec_plan_uses_mfma32(const CompiledPlan &cp, const SpecCfg &cfg) {
+    for (const StepDesc &st : cp.steps)
+        if (spec_step_uses_mfma32(st, cfg)) return true;
+    return false;
+}
+
 bool spec_step_uses_mfma(const StepDesc &st, const SpecCfg &cfg) {
     if (!cfg.mfma || cfg.fp64 || cfg.glog != 0 || cfg.threads % 64 != 0) return false;
     if (st.nk < 4) return false;
@@ -1408,7 +1529,7 @@ std::string spec_source(const CompiledPlan &cp, const SpecCfg &cfg) {
     const int nsteps = (int)cp.steps.size();
     std::ostringstream o;
     o << "// plan-specialised " << (cfg.fp64 ? "FP64" : "FP32") << " TNMMAP kernel (generated)\n";
-    if (cfg.mfma) o << kMfmaPrelude;
+    if (cfg.mfma || cfg.mfma32) o << kMfmaPrelude;
     emit_tile_fn(o, cp, cfg, "tile");
     const char *tn = cfg.fp64 ? "double" : "float";
     const int G = 1 << cfg.glog;
@@ -1576,7 +1697,7 @@ std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threa
     std::ostringstream o;
     o << "// mixed-code kernel: plan-specialised tiles of " << plans.size() << " plans (generated)\n";
     bool any_mfma = false;
-    for (const MixedSpecPlan &q : plans) any_mfma = any_mfma || q.f32.mfma;
+    for (const MixedSpecPlan &q : plans) any_mfma = any_mfma || q.f32.mfma || q.f32.mfma32;
     if (any_mfma) o << kMfmaPrelude;
     // the layouts of the host's batch table, header and counters
     o << "struct MBatch { const unsigned char *syn; double *mar; int *cls; unsigned char *tie; long long n;\n"

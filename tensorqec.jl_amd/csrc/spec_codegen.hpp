@@ -32,6 +32,7 @@ struct SpecCfg {
     bool slab = false;   // intermediates in a per-workgroup slab of global memory instead of LDS
     int unroll = 1;      // output entries per lane in flight (unroll of the output loop)
     bool mfma = false;   // FP32 tiles of one syndrome: GEMM-shaped steps on BF16 MFMA (FP32 accumulation)
+    bool mfma32 = false; // FP32 tiles of one syndrome: GEMM-shaped steps on FP32 MFMA (operands kept in FP32)
     bool reg = false;    // FP32, one syndrome per quad of lanes, every tensor in registers (glog = log2(threads / 4))
     bool dyn = false;    // reg FP32: per-step scaling from operand maxima (bit-identical to the generic kernel)
                          // instead of plan-time static exponents
@@ -65,6 +66,14 @@ void spec_reg_prepare(CompiledPlan &cp);
 // with M, N >= 32 (output bits feeding only A / only B) and K >= 16
 // (contracted bits), with no labels summed on one side only.
 bool spec_step_uses_mfma(const StepDesc &st, const SpecCfg &cfg);
+
+// A step runs on FP32 MFMA (v_mfma_f32_32x32x2f32) when the configuration
+// asks for it (cfg.mfma32, not with cfg.mfma), the tile holds one syndrome,
+// and per syndrome the step is a GEMM with M, N >= 32 and K >= 2, with no
+// labels summed on one side only.  Unlike the BF16 steps the operands keep
+// their FP32 values; only the summation order differs from the scalar steps.
+bool spec_step_uses_mfma32(const StepDesc &st, const SpecCfg &cfg);
+bool spec_plan_uses_mfma32(const CompiledPlan &cp, const SpecCfg &cfg);
 
 // Static LDS bytes of the specialised kernel for this configuration.
 size_t spec_lds_bytes(const CompiledPlan &cp, const SpecCfg &cfg);

@@ -1310,6 +1310,7 @@ int spec_code(const std::string &src, std::vector<char> &code) {
 constexpr int kRegBudgetF64 = 440;
 
 bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
+    if (c.mfma32 && (c.mfma || c.reg || c.fp64 || c.glog != 0 || c.threads % 64 != 0)) return false;
     if (c.reg) {
         if (c.mfma || c.slab || c.threads % 64 != 0 || (1 << c.glog) * 4 != c.threads) return false;
         if (!spec_reg_supported(p->cp)) return false;
@@ -1361,6 +1362,11 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
         c.fp64 = fp64;
         c.slab = true;
         c.unroll = 4;
+        // GEMM-shaped steps of one-syndrome FP32 slabs on FP32 MFMA (toric L=5:
+        // 2.2x; circuit-level d=5, 2 rounds: 1.08x after tuning)
+        SpecCfg m = c;
+        m.mfma32 = true;
+        c.mfma32 = spec_plan_uses_mfma32(p->cp, m);   // false for FP64 and for tiles of several syndromes
         const size_t bytes = spec_slab_entries(p->cp, c) * (fp64 ? 8 : 4);
         if ((bytes <= (size_t(4) << 20) || gl == 0) && spec_lds_bytes(p->cp, c) <= p->lds_limit) return c;
     }
@@ -1372,7 +1378,7 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
 std::tuple<int, int, int, int, int, int, int> spec_key(int dev, const SpecCfg &cfg) {
     return std::make_tuple(dev, cfg.glog, cfg.threads, cfg.tab_lds ? 1 : 0, cfg.waves, cfg.fp64 ? 1 : 0,
                            (cfg.slab ? 1 : 0) | (cfg.mfma ? 2 : 0) | (cfg.reg ? 4 : 0) | (cfg.unroll << 3) |
-                               (cfg.dyn ? 1 << 8 : 0));
+                               (cfg.dyn ? 1 << 8 : 0) | (cfg.mfma32 ? 1 << 9 : 0));
 }
 
 // The compiled kernel of a configuration on a device, or nullptr when it has
@@ -1542,10 +1548,11 @@ int run_spec(tqec_plan *p, DeviceData *dd, Workspace &w, const uint8_t *d_syn, i
     if (!cfg_set) {
         cfg = choose_spec(p);
         if (bf16 && cfg.glog >= 0) {
-            // MFMA steps need tiles of one syndrome
+            // MFMA steps need tiles of one syndrome; BF16 steps replace the FP32 ones
             cfg.glog = 0;
             cfg.threads = 256;
             cfg.mfma = true;
+            cfg.mfma32 = false;
         }
         cfg_set = true;
     }
@@ -1742,6 +1749,12 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
                     }
         }
         time_all(stage2);
+        // FP32 MFMA on the GEMM-shaped steps of the winner (tiles of one syndrome)
+        if (!rc && !bf16 && !best.reg && best.glog == 0) {
+            SpecCfg c = best;
+            c.mfma32 = true;
+            if (spec_fits(p, c) && spec_plan_uses_mfma32(p->cp, c)) time_all({c});
+        }
     }
     p->timing = saved_timing;
     (void)hipEventDestroy(e0);
@@ -2847,9 +2860,11 @@ int tqec_plan_spec_source(const tqec_plan *p, int32_t tile, int32_t threads, int
     c.mfma = (flags & 8) != 0;
     c.reg = (flags & 16) != 0;
     c.dyn = (flags & 32) != 0;
+    c.mfma32 = (flags & 64) != 0;
     c.unroll = std::max(1, (flags >> 8) & 0xff);
-    if (c.reg && (c.mfma || c.slab || tile * 4 != threads))
+    if (c.reg && (c.mfma || c.mfma32 || c.slab || tile * 4 != threads))
         return set_error(TQEC_EINVAL, "register-resident kernels run without slab or MFMA, tile = threads / 4");
+    if (c.mfma && c.mfma32) return set_error(TQEC_EINVAL, "BF16 and FP32 MFMA steps are exclusive");
     std::string src;
     try {
         src = spec_source(p->cp, c);
@@ -2893,10 +2908,14 @@ int tqec_plan_set_spec(tqec_plan *p, int32_t tile, int32_t threads, int32_t flag
     c.mfma = (flags & 8) != 0;
     c.reg = (flags & 16) != 0;
     c.dyn = (flags & 32) != 0;
+    c.mfma32 = (flags & 64) != 0;
     c.waves = waves;
     c.unroll = unroll ? unroll : 1;
-    if (c.reg && (c.mfma || c.slab || tile * 4 != threads))
+    if (c.reg && (c.mfma || c.mfma32 || c.slab || tile * 4 != threads))
         return set_error(TQEC_EINVAL, "register-resident kernels are FP32 without slab or MFMA, tile = threads / 4");
+    if (c.mfma && c.mfma32) return set_error(TQEC_EINVAL, "BF16 and FP32 MFMA steps are exclusive");
+    if (c.mfma32 && (gl != 0 || threads % 64 != 0))
+        return set_error(TQEC_EINVAL, "FP32 MFMA steps need tiles of one syndrome");
     std::string why;
     if (c.reg && !spec_reg_supported(p->cp, &why))
         return set_error(TQEC_EUNSUPPORTED, "plan does not suit the register-resident kernel: " + why);
@@ -2947,7 +2966,7 @@ int tqec_plan_spec_config(tqec_plan *p, int32_t *tile, int32_t *threads, int32_t
     if (threads) *threads = p->spec_cfg.threads;
     if (flags)
         *flags = (p->spec_cfg.tab_lds ? 1 : 0) | (p->spec_cfg.slab ? 4 : 0) | (p->spec_cfg.reg ? 16 : 0) |
-                 (p->spec_cfg.dyn ? 32 : 0);
+                 (p->spec_cfg.dyn ? 32 : 0) | (p->spec_cfg.mfma32 ? 64 : 0);
     if (waves) *waves = p->spec_cfg.waves | (p->spec_cfg.unroll << 8);
     return TQEC_OK;
 }

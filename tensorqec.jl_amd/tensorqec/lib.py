@@ -362,11 +362,13 @@ class HipPlan:
         check(load().tqec_plan_set_tile(self._h, m, int(tile), int(threads)))
 
     def set_spec(self, tile: int, threads: int, tables_in_lds: bool, waves: int = 0, slab: bool = False,
-                 unroll: int = 1, reg: bool = False, dyn: bool = False) -> None:
+                 unroll: int = 1, reg: bool = False, dyn: bool = False, mfma32: bool = False) -> None:
         """Pin the SPEC_F32 configuration; `dyn` (register-resident only):
         per-step scaling, bit-identical to the generic FP32 kernel, instead
-        of plan-time static exponents."""
-        flags = int(bool(tables_in_lds)) | (4 if slab else 0) | (16 if reg else 0) | (32 if dyn else 0)
+        of plan-time static exponents; `mfma32` (tiles of one syndrome):
+        GEMM-shaped steps on FP32 MFMA."""
+        flags = (int(bool(tables_in_lds)) | (4 if slab else 0) | (16 if reg else 0) | (32 if dyn else 0) |
+                 (64 if mfma32 else 0))
         check(load().tqec_plan_set_spec(self._h, int(tile), int(threads), flags, int(waves) | (int(unroll) << 8)))
 
     def set_map_kernel(self, specialised: bool) -> None:
@@ -386,15 +388,16 @@ class HipPlan:
         check(load().tqec_plan_spec_config(self._h, *[ctypes.byref(x) for x in v]))
         return {"tile": v[0].value, "threads": v[1].value, "tables_in_lds": bool(v[2].value & 1),
                 "slab": bool(v[2].value & 4), "reg": bool(v[2].value & 16), "dyn": bool(v[2].value & 32),
+                "mfma32": bool(v[2].value & 64),
                 "waves": v[3].value & 0xff,
                 "unroll": max(1, v[3].value >> 8)}
 
     def spec_source(self, tile: int, threads: int, tables_in_lds: bool = True, fp64: bool = False,
                     slab: bool = False, unroll: int = 1, mfma: bool = False, reg: bool = False,
-                    dyn: bool = False) -> str:
+                    dyn: bool = False, mfma32: bool = False) -> str:
         """HIP source of the plan-specialised kernel for a tile choice."""
         flags = (int(bool(tables_in_lds)) | (2 if fp64 else 0) | (4 if slab else 0) | (8 if mfma else 0)
-                 | (16 if reg else 0) | (32 if dyn else 0) | (int(unroll) << 8))
+                 | (16 if reg else 0) | (32 if dyn else 0) | (64 if mfma32 else 0) | (int(unroll) << 8))
         n = ctypes.c_size_t()
         check(load().tqec_plan_spec_source(self._h, int(tile), int(threads), flags, None, 0, ctypes.byref(n)))
         buf = ctypes.create_string_buffer(n.value + 1)

@@ -99,7 +99,7 @@ def test_d9_vs_oracle():
         say(f"  max err {err:.2e}")
 
 
-@pytest.mark.parametrize("mode", ["plain_f64", "wide_f32", "spec_f32"])
+@pytest.mark.parametrize("mode", ["plain_f64", "wide_f32", "spec_f32", "spec_f32_mfma32"])
 def test_toric5_vs_oracle_fixture(mode):
     import os
     torch = pytest.importorskip("torch")
@@ -108,8 +108,13 @@ def test_toric5_vs_oracle_fixture(mode):
     say(f"case toric L=5 {mode}: {len(S)} syndromes")
     t = CSSTannerGraph.from_code(ToricCode(5, 5))
     em = iid_error(p, p, p, t.nq)
+    mfma32 = mode.endswith("_mfma32")
+    mode = mode.replace("_mfma32", "")
     ct = D.compile(D.TNMMAP(mode=mode, optimizer=D.Dissection(), tabulate_bits=0), t, em)
     assert ct.hip.stats()["sc"] <= 22
+    if mfma32:
+        # the 512 x 4096 x 512 step (81% of the multiply-adds) on FP32 MFMA
+        ct.hip.set_spec(1, 256, False, 0, slab=True, unroll=4, mfma32=True)
     dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
     mar, cls, tie = run(torch, ct, dS, len(S), mode)
     if mode == "plain_f64":

@@ -364,6 +364,33 @@ def test_spec_wide_plans_vs_oracle(case):
     assert np.max(np.abs(plain[0] - ref) / np.maximum(ref, 1e-300)) < 1e-12
 
 
+@pytest.mark.parametrize("d,rounds,slab,n", [(3, 3, False, 4096), (5, 2, True, 1024)])
+def test_spec_fp32_mfma_dem_vs_oracle(d, rounds, slab, n):
+    """Circuit-level DEMs with their GEMM-shaped steps (M, N >= 32, K >= 2 per
+    syndrome) on FP32 MFMA (TQEC_SPEC_MFMA32), in LDS tiles (d=3, 3 rounds)
+    and per-workgroup slabs (d=5, 2 rounds, nested-dissection network):
+    the FP32 contract, marginals within 2e-5 of the row maximum of the CPU
+    oracle's FP64 contraction and classes equal on every non-tie syndrome;
+    the generated kernel does use the MFMA steps."""
+    torch = pytest.importorskip("torch")
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    dev = torch.device("cuda:0")
+    dem = detector_error_model_from_stim(rotated_memory_z(d, rounds, 0.001))
+    ct = D.compile(D.TNMMAP(mode="spec_f32"), dem)
+    ct.hip.set_spec(1, 256, False, 0, slab=slab, unroll=2, mfma32=True)
+    assert ct.hip.spec_config()["mfma32"]
+    assert "__builtin_amdgcn_mfma_f32_32x32x2f32" in ct.hip.spec_source(1, 256, False, slab=slab, unroll=2,
+                                                                           mfma32=True)
+    smp = L.HipSampler.flips(dem.error_rates, dem.detector_matrix(), dem.logical_matrix())
+    dS = torch.empty((n, smp.nsyn), dtype=torch.uint8, device=dev)
+    smp.sample(31, 0, n, dS.data_ptr())
+    say(f"case spec fp32 mfma dem d{d} r{rounds} slab {slab}")
+    ref, _, _ = OraclePlan(ct.plan).decode(dS.cpu().numpy())
+    got = run(torch, ct, dS, n, "spec_f32")
+    err, ties = check_f32(got[0], got[1], ref, "fp32 mfma")
+    say(f"  max rel marginal err {err:.2e}, ties {ties}, rechecks {ct.hip.last_recheck_count()}")
+
+
 def test_spec_bf16_mfma_dem_vs_oracle():
     """Circuit-level d=3, 3-round DEM with its GEMM-shaped steps (M, N >= 32,
     K >= 16 per syndrome) on BF16 MFMA: marginals within 3e-2 of the row

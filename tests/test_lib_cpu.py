@@ -190,6 +190,41 @@ def test_specialised_kernel_source_compiles_for_gfx950(tmp_path, tile, threads, 
         ct.hip.spec_source(3, 64, True)          # not a power of two
 
 
+def test_fp32_mfma_kernel_source_compiles_for_gfx950(tmp_path):
+    """TQEC_SPEC_MFMA32: on a circuit-level DEM plan (d=3, 3 rounds) the
+    GEMM-shaped steps are emitted as v_mfma_f32_32x32x2f32 tiles with FP32
+    operands (no BF16 rounding), the other steps as scalar code; the kernel
+    compiles for gfx950 without scratch; the flag is refused with BF16 MFMA
+    steps, on tiles of several syndromes and on the register-resident
+    kernel."""
+    import shutil
+    import subprocess
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not available")
+    ct = D.compile(D.TNMMAP(mode="spec_f32"), detector_error_model_from_stim(rotated_memory_z(3, 3, 0.001)))
+    src = ct.hip.spec_source(1, 256, False, False, False, 4, mfma32=True)
+    plain = ct.hip.spec_source(1, 256, False, False, False, 4)
+    assert src.count("__builtin_amdgcn_mfma_f32_32x32x2f32") > 0 and "(short)to_bf16(" not in src
+    assert "mfma" not in plain
+    f = tmp_path / "spec.hip"
+    f.write_text("#include <hip/hip_runtime.h>\n" + src)
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-c",
+                        str(f), "-o", str(tmp_path / "spec.o"), "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=300, env=_compiler_env())
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)
+    assert m and int(m.group(1)) == 0
+    with pytest.raises(L.TqecError):
+        ct.hip.spec_source(1, 256, False, mfma=True, mfma32=True)
+    with pytest.raises(L.TqecError):
+        ct.hip.set_spec(4, 256, False, mfma32=True)
+    with pytest.raises(L.TqecError):
+        ct.hip.set_spec(64, 256, False, reg=True, mfma32=True)
+
+
 @pytest.mark.parametrize("threads,tab", [(64, False), (256, True)])
 def test_register_resident_kernel_source_compiles_for_gfx950(tmp_path, threads, tab):
     """The register-resident variant (one syndrome per quad of lanes, every
