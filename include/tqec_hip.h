@@ -351,8 +351,9 @@ void tqec_mixed_destroy(tqec_mixed *m);
                                       generic FP32 kernel's arithmetic
                                       (bit-identical to TQEC_MODE_FUSED_F32)        */
 #define TQEC_SPEC_MFMA32 64        /* SPEC_F32 tiles of one syndrome (LDS or slab):
-                                      GEMM-shaped steps (M, N >= 32, K >= 2 per
-                                      syndrome) on FP32 MFMA with FP32 operands;
+                                      GEMM-shaped steps (M, N >= 16, K >= 4, or
+                                      M, N >= 32, K >= 2 per syndrome) on FP32
+                                      MFMA with FP32 operands;
                                       within the FP32 tolerance, not bit-identical
                                       (summation order)                              */
 

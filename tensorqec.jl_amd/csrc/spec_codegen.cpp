@@ -358,26 +358,43 @@ void emit_outputs_mfma(std::ostringstream &o, const StepDesc &st, int threads) {
     o << "        }\n      }\n    }\n";
 }
 
-// GEMM-shaped step on FP32 MFMA (v_mfma_f32_32x32x2f32), full FP32 operands:
-// per syndrome C[i, j] = sum_k A[i, k] B[k, j] as above, each wave owning a
-// block of ra x rb tiles of 32 x 32 (up to two more A-only and B-only output
-// bits index the tiles of a block), so every operand value a lane loads
-// feeds rb (or ra) MFMAs.  The k loop takes 2 ku contracted values per
-// iteration and loads the next iteration's operands before issuing its own
-// MFMAs.  Layouts (wave64):
-// lane l holds A[l % 32][l / 32], B[l / 32][l % 32] and
-// D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] for v = 0..15.  Every index stays
-// inside the operand and output ranges by construction (i, j < 32 over five
-// output bits each, k < 2^nk), so a layout mistake shows as wrong values,
-// never as an out-of-range access.
+// Exclusive output bits of a step: those feeding only A and only B.
+void mfma_side_bits(const StepDesc &st, std::vector<int> &aon, std::vector<int> &bon) {
+    const uint32_t am = source_mask(st.a), bm = source_mask(st.b);
+    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
+    aon = bits_of(am & ~bm & outm);
+    bon = bits_of(bm & ~am & outm);
+}
+
+// GEMM-shaped step on FP32 MFMA, full FP32 operands: per syndrome
+// C[i, j] = sum_k A[i, k] B[k, j] as above, in tiles of 32 x 32
+// (v_mfma_f32_32x32x2f32, K = 2 per instruction) when both sides have five
+// exclusive output bits, else 16 x 16 (v_mfma_f32_16x16x4f32, K = 4).  Each
+// wave owns a block of ra x rb tiles (up to two more A-only output bits and
+// one more B-only bit: 4 x 2 tiles, the best of 2 x 2, 4 x 2 and 2 x 4 on
+// toric L=5; 4 x 4 spills), so every operand value a lane loads feeds rb (or
+// ra) MFMAs.  The k loop takes ku instructions' worth of contracted values
+// per iteration and loads the next iteration's operands before issuing its
+// own MFMAs (one wave per SIMD leaves no other wave to hide the loads).
+// Layouts (wave64), with T the tile edge and Q = 64 / T:
+// lane l holds A[l % T][l / T] and B[l / T][l % T]; T = 32: D[v] =
+// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], v = 0..15; T = 16: D[v] =
+// C[4 (l / 16) + v][l % 16], v = 0..3.  Every index stays inside the
+// operand and output ranges by construction (i, j < T over that many
+// output bits, k < 2^nk, and every lane runs the same number of k
+// iterations), so a layout mistake shows as wrong values, never as an
+// out-of-range access.
 void emit_outputs_mfma32(std::ostringstream &o, const StepDesc &st, int threads) {
     const OperandDesc &A = st.a, &B = st.b;
-    const uint32_t am = source_mask(A), bm = source_mask(B);
-    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
-    const std::vector<int> aon = bits_of(am & ~bm & outm), bon = bits_of(bm & ~am & outm);
-    const std::vector<int> ai(aon.begin(), aon.begin() + 5), bj(bon.begin(), bon.begin() + 5);
-    const int xa = std::min<int>(1, (int)aon.size() - 5), xb = std::min<int>(1, (int)bon.size() - 5);
-    const std::vector<int> a2(aon.begin() + 5, aon.begin() + 5 + xa), b2(bon.begin() + 5, bon.begin() + 5 + xb);
+    std::vector<int> aon, bon;
+    mfma_side_bits(st, aon, bon);
+    const bool big = aon.size() >= 5 && bon.size() >= 5;
+    const int tbits = big ? 5 : 4;                 // tile edge 2^tbits
+    const unsigned tedge = 1u << tbits, kq = 64u / tedge;   // k values per instruction
+    const std::vector<int> ai(aon.begin(), aon.begin() + tbits), bj(bon.begin(), bon.begin() + tbits);
+    const int xa = std::min<int>(2, (int)aon.size() - tbits), xb = std::min<int>(1, (int)bon.size() - tbits);
+    const std::vector<int> a2(aon.begin() + tbits, aon.begin() + tbits + xa);
+    const std::vector<int> b2(bon.begin() + tbits, bon.begin() + tbits + xb);
     std::vector<int> rest;
     for (int b = 0; b < st.rc; ++b) {
         const bool used = std::find(ai.begin(), ai.end(), b) != ai.end() ||
@@ -390,7 +407,11 @@ void emit_outputs_mfma32(std::ostringstream &o, const StepDesc &st, int threads)
     const unsigned ntiles = 1u << rest.size();
     const int nwaves = threads / 64;
     const unsigned nkk = 1u << st.nk;
-    const unsigned ku = std::min(4u, nkk / 2);   // k pairs per loop iteration
+    const unsigned ku = std::min(4u, nkk / kq);   // instructions' k groups per loop iteration
+    const unsigned kstep = kq * ku;
+    const std::string tm = std::to_string(tedge - 1) + "u", lsh = std::to_string(tbits);
+    const char *acc_t = big ? "f32x16" : "f32x4";
+    const char *mfma = big ? "__builtin_amdgcn_mfma_f32_32x32x2f32" : "__builtin_amdgcn_mfma_f32_16x16x4f32";
     const std::string ga = gather_expr(A), gb = gather_expr(B);
     auto ld = [&](const OperandDesc &op, const char *base, const std::string &idx, const std::string &k) {
         const std::string e = idx + " | ((" + k + ") << " + std::to_string(op.kshift) + ")";
@@ -404,22 +425,19 @@ void emit_outputs_mfma32(std::ostringstream &o, const StepDesc &st, int threads)
     o << "        const unsigned tb = " << deposit("t", rest) << ";\n";
     for (int x = 0; x < ra; ++x)
         o << "        const unsigned ar" << x << " = "
-          << subst_c(ga, "(tb | " + deposit("(unsigned)(l & 31)", ai) + " | " + deposit(std::to_string(x) + "u", a2) +
-                             ")")
+          << subst_c(ga, "(tb | " + deposit("(unsigned)(l & " + tm + ")", ai) + " | " +
+                             deposit(std::to_string(x) + "u", a2) + ")")
           << ";\n";
     for (int y = 0; y < rb; ++y)
         o << "        const unsigned br" << y << " = "
-          << subst_c(gb, "(tb | " + deposit("(unsigned)(l & 31)", bj) + " | " + deposit(std::to_string(y) + "u", b2) +
-                             ")")
+          << subst_c(gb, "(tb | " + deposit("(unsigned)(l & " + tm + ")", bj) + " | " +
+                             deposit(std::to_string(y) + "u", b2) + ")")
           << ";\n";
     for (int x = 0; x < ra; ++x)
-        for (int y = 0; y < rb; ++y) o << "        f32x16 acc" << x << "_" << y << " = {};\n";
-    // operands of the next k-block are loaded before this block's MFMAs
-    // issue, so their latency hides behind the matrix work (one wave per
-    // SIMD leaves no other wave to hide it)
+        for (int y = 0; y < rb; ++y) o << "        " << acc_t << " acc" << x << "_" << y << " = {};\n";
     auto loads = [&](const std::string &kbase, const char *pre, bool decl) {
         for (unsigned u = 0; u < ku; ++u) {
-            const std::string k = kbase + " + " + std::to_string(2 * u) + "u";
+            const std::string k = kbase + " + " + std::to_string(kq * u) + "u";
             for (int x = 0; x < ra; ++x)
                 o << "          " << (decl ? "float " : "") << pre << "a" << u << "_" << x << " = "
                   << ld(A, "pa", "ar" + std::to_string(x), k) << ";\n";
@@ -429,31 +447,35 @@ void emit_outputs_mfma32(std::ostringstream &o, const StepDesc &st, int threads)
         }
     };
     o << "        {\n";
-    loads("(unsigned)(l >> 5)", "n", true);
+    loads("(unsigned)(l >> " + lsh + ")", "n", true);
     o << "        #pragma unroll 1\n";
-    o << "        for (unsigned kb = (unsigned)(l >> 5); kb < " << nkk << "u; kb += " << 2 * ku << "u) {\n";
+    o << "        for (unsigned kb = (unsigned)(l >> " << lsh << "); kb < " << nkk << "u; kb += " << kstep << "u) {\n";
     for (unsigned u = 0; u < ku; ++u) {
         for (int x = 0; x < ra; ++x) o << "          const float a" << u << "_" << x << " = na" << u << "_" << x << ";\n";
         for (int y = 0; y < rb; ++y) o << "          const float b" << u << "_" << y << " = nb" << u << "_" << y << ";\n";
     }
-    // the same for both halves of the wave (kb differs by l >> 5 < 2): uniform
-    o << "          if (kb + " << 2 * ku << "u < " << nkk << "u) {\n";
-    loads("kb + " + std::to_string(2 * ku) + "u", "n", false);
+    // the same for every lane (kb differs across the wave by l >> tbits < kq <= kstep): uniform
+    o << "          if (kb + " << kstep << "u < " << nkk << "u) {\n";
+    loads("kb + " + std::to_string(kstep) + "u", "n", false);
     o << "          }\n";
     for (unsigned u = 0; u < ku; ++u)
         for (int x = 0; x < ra; ++x)
             for (int y = 0; y < rb; ++y)
-                o << "          acc" << x << "_" << y << " = __builtin_amdgcn_mfma_f32_32x32x2f32(a" << u << "_" << x
-                  << ", b" << u << "_" << y << ", acc" << x << "_" << y << ", 0, 0, 0);\n";
+                o << "          acc" << x << "_" << y << " = " << mfma << "(a" << u << "_" << x << ", b" << u << "_" << y
+                  << ", acc" << x << "_" << y << ", 0, 0, 0);\n";
     o << "        }\n";
     o << "        }\n";
     for (int x = 0; x < ra; ++x)
         for (int y = 0; y < rb; ++y) {
             o << "        #pragma unroll\n";
-            o << "        for (int v = 0; v < 16; ++v) {\n";
-            o << "          const unsigned i = 8u * (unsigned)(v >> 2) + 4u * (unsigned)(l >> 5) + (unsigned)(v & 3);\n";
+            o << "        for (int v = 0; v < " << (big ? 16 : 4) << "; ++v) {\n";
+            if (big)
+                o << "          const unsigned i = 8u * (unsigned)(v >> 2) + 4u * (unsigned)(l >> 5) + (unsigned)(v & 3);\n";
+            else
+                o << "          const unsigned i = 4u * (unsigned)(l >> 4) + (unsigned)v;\n";
             o << "          const unsigned c = tb | " << deposit("i", ai) << " | " << deposit(std::to_string(x) + "u", a2)
-              << " | " << deposit("(unsigned)(l & 31)", bj) << " | " << deposit(std::to_string(y) + "u", b2) << ";\n";
+              << " | " << deposit("(unsigned)(l & " + tm + ")", bj) << " | " << deposit(std::to_string(y) + "u", b2)
+              << ";\n";
             o << "          const float x = ldexpf(acc" << x << "_" << y << "[v], shift);\n";
             o << "          lmax = x > lmax ? x : lmax;\n";
             o << "          pc[c << GLOG] = x;\n";
@@ -463,9 +485,17 @@ void emit_outputs_mfma32(std::ostringstream &o, const StepDesc &st, int threads)
 }
 
 void emit_step(std::ostringstream &o, const StepDesc &st, int s, size_t const_n, int cstep, bool fp64, int unroll,
-               const SpecCfg *mfma_cfg) {
+               const SpecCfg *mfma_cfg, bool opaque_tid = false) {
     const OperandDesc &A = st.a, &B = st.b;
     o << "  { // step " << s << ": rank " << st.rc << ", contracted " << st.nk << "\n";
+    if (opaque_tid)
+        // the step's own copy of the thread index, opaque to the optimiser:
+        // nothing derived from it is hoisted out of the kernel's loop over
+        // tiles or shared with other steps, so per-thread index values live
+        // for one step only
+        o << "    int tid = (int)threadIdx.x;\n    asm volatile(\"\" : \"+v\"(tid));\n"
+             "    const int g = tid & (G - 1), c0 = tid >> GLOG, lane = tid & 63;\n"
+             "    (void)g; (void)c0; (void)lane;\n";
     if (fp64) {
         // generic FP64 step: no scaling, the folded constant exponents restored per entry
         emit_base(o, "pa", A, const_n);
@@ -1439,7 +1469,7 @@ void emit_tile_fn(std::ostringstream &o, const CompiledPlan &cp, const SpecCfg &
     o << "  __syncthreads();\n";
     for (int s = 0; s < nsteps; ++s)
         emit_step(o, cp.steps[s], s, const_n, cfg.threads >> cfg.glog, cfg.fp64, std::max(1, cfg.unroll),
-                  ((cfg.mfma || cfg.mfma32) && !cfg.fp64) ? &cfg : nullptr);
+                  ((cfg.mfma || cfg.mfma32) && !cfg.fp64) ? &cfg : nullptr, cfg.slab);
     o << "  if (tid < G) {\n"
          "    const long long q = b0 + tid;\n"
          "    if (q < count) {\n"
@@ -1489,6 +1519,7 @@ std::string dbl(double x) {
 const char *kMfmaPrelude =
     "typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;\n"
     "typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;\n"
+    "typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;\n"
     "__device__ __forceinline__ unsigned short to_bf16(float f) {\n"
     "  const unsigned u = __float_as_uint(f);\n"
     "  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);\n"
@@ -1498,11 +1529,11 @@ const char *kMfmaPrelude =
 
 bool spec_step_uses_mfma32(const StepDesc &st, const SpecCfg &cfg) {
     if (!cfg.mfma32 || cfg.mfma || cfg.fp64 || cfg.reg || cfg.glog != 0 || cfg.threads % 64 != 0) return false;
-    if (st.nk < 1) return false;
     if ((st.a.kind == kInter && st.a.nsingle) || (st.b.kind == kInter && st.b.nsingle)) return false;
-    const uint32_t am = source_mask(st.a), bm = source_mask(st.b);
-    const uint32_t outm = (st.rc >= 32) ? 0xffffffffu : ((1u << st.rc) - 1u);
-    return __builtin_popcount(am & ~bm & outm) >= 5 && __builtin_popcount(bm & ~am & outm) >= 5;
+    std::vector<int> aon, bon;
+    mfma_side_bits(st, aon, bon);
+    if (aon.size() >= 5 && bon.size() >= 5) return st.nk >= 1;   // 32 x 32 x 2
+    return aon.size() >= 4 && bon.size() >= 4 && st.nk >= 2;      // 16 x 16 x 4
 }
 
 bool spec_plan_uses_mfma32(const CompiledPlan &cp, const SpecCfg &cfg) {

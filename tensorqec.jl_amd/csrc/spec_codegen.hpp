@@ -67,10 +67,11 @@ void spec_reg_prepare(CompiledPlan &cp);
 // (contracted bits), with no labels summed on one side only.
 bool spec_step_uses_mfma(const StepDesc &st, const SpecCfg &cfg);
 
-// A step runs on FP32 MFMA (v_mfma_f32_32x32x2f32) when the configuration
-// asks for it (cfg.mfma32, not with cfg.mfma), the tile holds one syndrome,
-// and per syndrome the step is a GEMM with M, N >= 32 and K >= 2, with no
-// labels summed on one side only.  Unlike the BF16 steps the operands keep
+// A step runs on FP32 MFMA when the configuration asks for it (cfg.mfma32,
+// not with cfg.mfma), the tile holds one syndrome, and per syndrome the step
+// is a GEMM with no labels summed on one side only: M, N >= 32 and K >= 2
+// (v_mfma_f32_32x32x2f32 tiles), else M, N >= 16 and K >= 4
+// (v_mfma_f32_16x16x4f32).  Unlike the BF16 steps the operands keep
 // their FP32 values; only the summation order differs from the scalar steps.
 bool spec_step_uses_mfma32(const StepDesc &st, const SpecCfg &cfg);
 bool spec_plan_uses_mfma32(const CompiledPlan &cp, const SpecCfg &cfg);
