@@ -1821,6 +1821,8 @@ struct MixedPlanCfg {
 };
 
 constexpr int kMixedThreads = 256;
+// Slab memory of one mixed-code kernel's resident workgroups at most (bytes).
+constexpr size_t kMixedSlabTotal = size_t(32) << 30;
 constexpr size_t kMixedSlabBudget = size_t(4) << 20;   // FP32 slab per workgroup before shrinking the tile
 
 }  // namespace
@@ -1941,7 +1943,8 @@ int mixed_spec_layout(const std::vector<tqec_plan *> &plans, int nt, size_t lds_
         if (p->cp.semiring != TQEC_SEMIRING_SUM)
             return set_error(TQEC_EINVAL, "plan " + std::to_string(i) + " is a max-product plan");
         SpecCfg c;
-        if (p->spec_cfg_set && p->spec_cfg.glog >= 0 && !p->spec_cfg.reg) {
+        const bool tuned = p->spec_cfg_set && p->spec_cfg.glog >= 0 && !p->spec_cfg.reg;
+        if (tuned) {
             c = p->spec_cfg;
         } else {
             // the default tile of the largest size whose LDS image fits the budget
@@ -1963,6 +1966,13 @@ int mixed_spec_layout(const std::vector<tqec_plan *> &plans, int nt, size_t lds_
         if (c.slab) {
             c.glog = std::min(c.glog, 4);
             while (c.glog > 0 && spec_slab_entries(p->cp, c) * 4 > (size_t(4) << 20)) --c.glog;
+        }
+        if (!tuned && c.slab && c.glog == 0) {
+            // the single-plan slab default: blocks of 4 outputs, FP32-MFMA GEMM steps
+            c.unroll = 4;
+            SpecCfg m = c;
+            m.mfma32 = true;
+            c.mfma32 = spec_plan_uses_mfma32(p->cp, m);
         }
         SpecCfg d;
         d.glog = 0;
@@ -2676,6 +2686,10 @@ int tqec_mixed_create(tqec_plan *const *plans, int32_t n_plans, int32_t max_batc
     TQ_HIP(hipFuncSetAttribute((const void *)mixed_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)(lds_limit - 512)));
     m->blocks = n_cu * resident_per_cu(m->lds + 512, kMixedThreads, lds_limit);
+    // the persistent kernel takes any grid: cap its workgroups so that the
+    // slabs of a wide plan (toric L=5: 68 MB per workgroup) stay in budget
+    if (const size_t per = m->slab32_bytes + m->slab64_bytes)
+        m->blocks = (int)std::max<size_t>((size_t)n_cu, std::min<size_t>((size_t)m->blocks, kMixedSlabTotal / per));
     tqec_mixed *mp = m.get();
     auto fail = [&](int code) {
         mixed_free(mp);
@@ -3011,6 +3025,8 @@ int tqec_mixed_set_kernel(tqec_mixed *m, int32_t kind) {
         return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
     }
     m->spec_blocks = n_cu * resident_per_cu(lds, nt, lds_limit);
+    if (const size_t per = s32 * sizeof(float) + s64 * sizeof(double))
+        m->spec_blocks = (int)std::max<size_t>((size_t)n_cu, std::min<size_t>((size_t)m->spec_blocks, kMixedSlabTotal / per));
     std::vector<const void *> t32(m->plans.size()), t64(m->plans.size());
     for (size_t i = 0; i < m->plans.size(); ++i) {
         DeviceData *dd;

@@ -170,3 +170,45 @@ def test_mixed_rejects_bad_tables(setup):
     with pytest.raises(L.TqecError) as e:
         mx.decode_device([(0, 1, 0, 0, 0, 0)])
     assert e.value.code == L.TQEC_ENULL
+
+
+def test_mixed_specialised_with_toric5():
+    """The generated mixed-code kernel with a plan of width 2^21 (toric code
+    L=5, nested-dissection network: per-workgroup slabs, GEMM-shaped steps
+    on FP32 MFMA) beside a small one: the toric L=5 batch against the
+    oracle's marginals in tests/golden/toric5.npz (1e-4 of the row maximum:
+    its last step sums 2^18 products per marginal in FP32), the surface-code
+    batch against the CPU oracle at 2e-5; classes exact on non-tie rows."""
+    import os
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "toric5.npz"))
+    p5 = float(g["p"])
+    t5 = CSSTannerGraph.from_code(ToricCode(5, 5))
+    ct5 = D.compile(D.TNMMAP(mode="fused_f32", optimizer=D.Dissection(), tabulate_bits=0), t5,
+                    iid_error(p5, p5, p5, t5.nq))
+    t3 = CSSTannerGraph.from_code(SurfaceCode(3, 3))
+    em3 = iid_error(0.05 / 3, 0.05 / 3, 0.05 / 3, t3.nq)
+    ct3 = D.compile(D.TNMMAP(mode="fused_f32"), t3, em3)
+    say("case mixed with toric5: create and generate")
+    mx = L.HipMixed([ct3.hip, ct5.hip], max_batches=4, max_rows=4096)
+    mx.set_kernel(True)
+    assert mx.plan_config(1)[1] & 1 == 1           # the toric L=5 tiles work from slabs
+    S3 = css_syndrome_rows(sample_depolarizing(em3, seed=77, count=500), t3)
+    S5 = g["syn"]
+    outs = []
+    for S, nout in ((S3, 4), (S5, 16)):
+        n = len(S)
+        outs.append(dict(n=n, S=S, dS=torch.from_numpy(np.ascontiguousarray(S)).to(dev),
+                         mar=torch.full((n, nout), -1.0, dtype=torch.float64, device=dev),
+                         cls=torch.full((n,), -1, dtype=torch.int32, device=dev),
+                         tie=torch.full((n,), 7, dtype=torch.uint8, device=dev)))
+    tab = [(i, o["n"], o["dS"].data_ptr(), o["mar"].data_ptr(), o["cls"].data_ptr(), o["tie"].data_ptr())
+           for i, o in enumerate(outs)]
+    mx.decode_device(tab)
+    lat, nre = mx.last_call()
+    say(f"  latencies us {np.round(lat, 1).tolist()} rechecks {nre}")
+    ref3, _, _ = OraclePlan(ct3.plan).decode(S3)
+    check_f32(outs[0]["mar"].cpu().numpy(), outs[0]["cls"].cpu().numpy(), ref3, "surface3")
+    err, _ = check_f32(outs[1]["mar"].cpu().numpy(), outs[1]["cls"].cpu().numpy(), g["mar"], "toric5", tol=1e-4)
+    say(f"  toric5 max rel marginal err {err:.2e}")

@@ -11,7 +11,8 @@ src/decoding/tndecoder.jl:160-165, evaluated for every syndrome).
     the same plan and the same syndromes.
   * d=9 on 2,048 syndromes: every fused executor the plan fits.
   * toric code L=5 (16 classes), nested-dissection network of width 2^21:
-    the plain FP64, wide FP32 and specialised FP32 executors on the 16
+    the plain and specialised FP64, wide FP32 and specialised FP32 (with and
+    without FP32-MFMA steps) executors on the 16
     syndromes of tests/golden/toric5.npz (the C oracle's marginals, written
     by tests/golden/make_toric5.py; about a CPU-minute per syndrome).
 Tolerances: tests/_parity.py (FP64 1e-12 relative with exact zeros, tie
@@ -99,7 +100,7 @@ def test_d9_vs_oracle():
         say(f"  max err {err:.2e}")
 
 
-@pytest.mark.parametrize("mode", ["plain_f64", "wide_f32", "spec_f32", "spec_f32_mfma32"])
+@pytest.mark.parametrize("mode", ["plain_f64", "spec_f64", "wide_f32", "spec_f32", "spec_f32_mfma32"])
 def test_toric5_vs_oracle_fixture(mode):
     import os
     torch = pytest.importorskip("torch")
@@ -117,7 +118,7 @@ def test_toric5_vs_oracle_fixture(mode):
         ct.hip.set_spec(1, 256, False, 0, slab=True, unroll=4, mfma32=True)
     dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda:0")
     mar, cls, tie = run(torch, ct, dS, len(S), mode)
-    if mode == "plain_f64":
+    if mode.endswith("_f64"):
         check_f64(mar, cls, tie, ref, mode)
     else:
         # the plan's last step contracts 18 labels: every marginal is one
