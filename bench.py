@@ -831,8 +831,8 @@ def main():
     if pinned:
         cands = [(pinned["seed"], pinned["sc_target"], pinned.get("tabulate_bits", 4))]
     else:
-        cands = [(14, None, 6), (24, None, 6), (30, None, 6), (11, None, 7), (19, None, 6), (31, None, 6), (13, None, 6),
-                 (24, None, 5), (14, None, 5), (24, None, 4)]
+        cands = [(14, None, 7), (11, None, 8), (14, None, 6), (24, None, 6), (30, None, 6), (11, None, 7), (19, None, 6),
+                 (31, None, 6), (13, None, 6), (24, None, 5), (14, None, 5), (24, None, 4)]
     outs = [torch.empty((B, 4), dtype=torch.float64, device=dev), torch.empty(B, dtype=torch.int32, device=dev),
             torch.empty(B, dtype=torch.uint8, device=dev)]
 

@@ -51,7 +51,7 @@ def run(torch, ct, dS, n, mode):
 
 
 @pytest.mark.parametrize("p,seed,tab", [(0.02, 11, 7), (0.05, 11, 7), (0.08, 11, 7),
-                                        (0.05, 14, 6), (0.05, 11, 8)])
+                                        (0.05, 14, 6), (0.05, 14, 7), (0.05, 11, 8)])
 def test_d7_benchmark_configuration_full_batch(p, seed, tab):
     torch = pytest.importorskip("torch")
     n = 65536
