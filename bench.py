@@ -156,7 +156,7 @@ def stream_leg(torch, dev, seed, calls=6, warmup=1, window=48, pool=8192, codes=
         # FP32-MFMA steps: each candidate kernel takes tens of seconds to build
         plans.append(ct)
         info.append(dict(name=name, p=p, ns=t.ns, nout=1 << ct.hip.n_out, S=S, truth=truth,
-                         macs=st["dep_tc"], sc=st["sc"]))
+                         macs=st["dep_tc"], sc=st["sc"], spec=ct.hip.spec_config()))
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     nb_total = (warmup + calls) * window
@@ -280,7 +280,7 @@ def stream_result(info, mx, codes, window, calls, rows_timed, dt, lat, call_ms, 
         "workload": f"uneven stream: surface d=3/5/7/9, toric L={toric}, Steane, Shor; p ~ U[0.01, 0.1] per code; "
                     f"{window} batches per launch, sizes log-uniform in [1, 4096]",
         "codes": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in it.items()
-                   if k in ("name", "p", "ns", "macs", "sc")} | {"tile_slab": mx.plan_config(i)}
+                   if k in ("name", "p", "ns", "macs", "sc", "spec")} | {"tile_slab": mx.plan_config(i)}
                   for i, it in enumerate(info)],
         "syndromes_per_s": rows_timed / dt,
         "calls": calls, "batches_per_call": window, "syndromes_timed": rows_timed,

@@ -1367,6 +1367,9 @@ SpecCfg choose_spec(const tqec_plan *p, bool fp64 = false) {
         SpecCfg m = c;
         m.mfma32 = true;
         c.mfma32 = spec_plan_uses_mfma32(p->cp, m);   // false for FP64 and for tiles of several syndromes
+        // two waves per SIMD (at most 256 registers) let one wave's MFMAs
+        // cover the other's operand loads: toric L=5 1.4x over one wave
+        if (c.mfma32) c.waves = 2;
         const size_t bytes = spec_slab_entries(p->cp, c) * (fp64 ? 8 : 4);
         if ((bytes <= (size_t(4) << 20) || gl == 0) && spec_lds_bytes(p->cp, c) <= p->lds_limit) return c;
     }
@@ -1751,9 +1754,19 @@ int autotune_spec(tqec_plan *p, const uint8_t *d_syn, int64_t n, hipStream_t st,
         time_all(stage2);
         // FP32 MFMA on the GEMM-shaped steps of the winner (tiles of one syndrome)
         if (!rc && !bf16 && !best.reg && best.glog == 0) {
-            SpecCfg c = best;
-            c.mfma32 = true;
-            if (spec_fits(p, c) && spec_plan_uses_mfma32(p->cp, c)) time_all({c});
+            // FP32 MFMA on the GEMM-shaped steps of a one-syndrome winner; the
+            // MFMA steps change the register budget, so the register cap and
+            // the other steps' output blocks are retuned with them
+            std::vector<SpecCfg> stage3;
+            for (int wv : {0, 2, 4})
+                for (int un : {2, 4}) {
+                    SpecCfg c = best;
+                    c.mfma32 = true;
+                    c.waves = wv;
+                    c.unroll = un;
+                    if (spec_fits(p, c) && spec_plan_uses_mfma32(p->cp, c)) stage3.push_back(c);
+                }
+            time_all(stage3);
         }
     }
     p->timing = saved_timing;
