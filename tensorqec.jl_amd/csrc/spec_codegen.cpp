@@ -1727,11 +1727,8 @@ std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threa
                               size_t slab64_entries) {
     std::ostringstream o;
     o << "// mixed-code kernel: plan-specialised tiles of " << plans.size() << " plans (generated)\n";
-    bool any_mfma = false, any_mfma32 = false;
-    for (const MixedSpecPlan &q : plans) {
-        any_mfma = any_mfma || q.f32.mfma || q.f32.mfma32;
-        any_mfma32 = any_mfma32 || q.f32.mfma32;
-    }
+    bool any_mfma = false;
+    for (const MixedSpecPlan &q : plans) any_mfma = any_mfma || q.f32.mfma || q.f32.mfma32;
     if (any_mfma) o << kMfmaPrelude;
     // the layouts of the host's batch table, header and counters
     o << "struct MBatch { const unsigned char *syn; double *mar; int *cls; unsigned char *tie; long long n;\n"
@@ -1743,10 +1740,7 @@ std::string spec_mixed_source(const std::vector<MixedSpecPlan> &plans, int threa
         emit_tile_fn(o, *plans[i].cp, plans[i].f32, "p" + std::to_string(i) + "_f32");
         emit_tile_fn(o, *plans[i].cp, plans[i].f64, "p" + std::to_string(i) + "_f64");
     }
-    // FP32-MFMA tiles: at most 256 registers (two waves per SIMD), the cap
-    // under which their operand loads overlap another wave's MFMAs
-    o << "extern \"C\" __global__ void __launch_bounds__(" << threads << (any_mfma32 ? ", 2" : "")
-      << ") tqec_mixed_spec(\n"
+    o << "extern \"C\" __global__ void __launch_bounds__(" << threads << ") tqec_mixed_spec(\n"
          "    const MBatch *__restrict__ batches, const MHeader *__restrict__ hdr, MCounters *ctr,\n"
          "    unsigned long long *t_done, const float *const *tabs32, const double *const *tabs64, float *slab32,\n"
          "    double *slab64) {\n";
