@@ -375,8 +375,9 @@ void mfma_side_bits(const StepDesc &st, std::vector<int> &aon, std::vector<int> 
 // toric L=5; 4 x 4 spills), so every operand value a lane loads feeds rb (or
 // ra) MFMAs.  The k loop takes ku instructions' worth of contracted values
 // per iteration and loads the next iteration's operands before issuing its
-// own MFMAs (one wave per SIMD leaves no other wave to hide the loads).
-// Layouts (wave64), with T the tile edge and Q = 64 / T:
+// own MFMAs (at one or two waves per SIMD few other waves hide the loads).
+// Layouts (wave64), with T the tile edge (64 / T contracted values per
+// instruction):
 // lane l holds A[l % T][l / T] and B[l / T][l % T]; T = 32: D[v] =
 // C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], v = 0..15; T = 16: D[v] =
 // C[4 (l / 16) + v][l % 16], v = 0..3.  Every index stays inside the
