@@ -727,6 +727,63 @@ def toric5_leg(torch, dev, seed, L=5, p=0.03, n=2048, reps=1):
     return out
 
 
+def toric6_leg(torch, dev, seed, L=6, p=0.03, n=8, target_rank=23):
+    """Toric code L=6 (72 qubits, 16 classes), decoded exactly with label
+    slicing: the nested-dissection network (2^37 multiply-adds per syndrome,
+    widest tensor 2^27) with the labels of its widest intermediates fixed,
+    each syndrome run as 2^k rows and the rows' marginals summed in FP64.
+    FP32 (slab kernels, FP32-MFMA steps) against FP64 on the same syndromes."""
+    import numpy as np
+    from tensorqec import decoders as D
+    from tensorqec import lib as L_
+    from tensorqec.codes import CSSTannerGraph, ToricCode
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+    from tensorqec.network import dissected_css_network
+    from tensorqec.slicing import SlicedDecoder, choose_slices
+    t0 = time.time()
+    t = CSSTannerGraph.from_code(ToricCode(L, L))
+    em = iid_error(p, p, p, t.nq)
+    lx, lz = logical_operator(t)
+    _, plan, _ = D.dissection_plan(lambda s: dissected_css_network(t, em, lxlz=(lx, lz), seed=s), D.Dissection())
+    labels = choose_slices(plan, target_rank)
+    dec = SlicedDecoder(plan, labels)
+    st0, st1 = plan.stats(), dec.plan.stats()
+    smp = L_.HipSampler.css(t, em, lx, lz)
+    S = torch.empty((n, t.ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(n, dtype=torch.int32, device=dev)
+    smp.sample(seed + 23, 0, n, S.data_ptr(), truth.data_ptr())
+    out = {"workload": f"toric code L={L}, code-capacity depolarizing px=py=pz={p}, {n} syndromes, "
+                       f"nested-dissection network sliced on {len(labels)} labels",
+           "plan": {"macs_per_syndrome_unsliced": st0["tc_dep"], "width_unsliced": st0["sc_dep"],
+                    "sliced_labels": len(labels), "macs_per_slice": st1["tc_dep"],
+                    "macs_per_syndrome": st1["tc_dep"] * (1 << len(labels)), "width": st1["sc_dep"]},
+           "setup_s": time.time() - t0, "modes": {}}
+    res = {}
+    for mode in ("spec_f32", "spec_f64"):
+        h0 = time.time()
+        dec.decode_device(torch, S, n, mode)          # compiles the kernels
+        torch.cuda.synchronize()
+        first = time.time() - h0
+        h0 = time.perf_counter()
+        mar, cls, tie = dec.decode_device(torch, S, n, mode)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - h0
+        res[mode] = (mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy())
+        out["modes"][mode] = {"syndromes_per_s": n / dt, "first_call_s": first,
+                              "tflops": 2.0 * st1["tc_dep"] * (n << len(labels)) / dt / 1e12}
+    m32, c32, _ = res["spec_f32"]
+    m64, c64, t64 = res["spec_f64"]
+    valid = m64.max(axis=1) > 0
+    out["modes"]["spec_f32"]["max_rel_marginal_err_vs_fp64"] = float(
+        np.max(np.abs(m32[valid] - m64[valid]) / m64[valid].max(axis=1, keepdims=True)))
+    ok = t64 == 0
+    out["modes"]["spec_f32"]["class_agreement_non_tie"] = float((c32[ok] == c64[ok]).mean())
+    out["total_probability_max"] = float(m64.sum(axis=1).max())
+    out["logical_errors_fp64"] = int((c64 != truth.cpu().numpy()).sum())
+    return out
+
+
 def copy_calibration(torch, dev, mib):
     """Kernels of known traffic: a fill writes and a sum reads `mib` MiB
     once; their WRITE_SIZE / FETCH_SIZE counters calibrate the HBM byte
@@ -764,7 +821,7 @@ def main():
                          "toric L=5, MLP, device loop, TNMAP)")
     ap.add_argument("--only", default=None,
                     choices=["stream", "stream_full", "learned", "learned_d9", "learned_dem", "dem", "mlp", "pipeline",
-                             "tnmap", "toric5"],
+                             "tnmap", "toric5", "toric6"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--dem-d", type=int, default=5, help="distance of the circuit-level DEM workload")
     ap.add_argument("--dem-rounds", type=int, default=2, help="syndrome rounds of the circuit-level DEM workload")
@@ -786,7 +843,7 @@ def main():
               "stream_full": {"codes": STREAM_CODES + (("toric5", ("ToricCode", 5)),), "compare": False}}.get(args.only, {})
         fn = {"stream": stream_leg, "stream_full": stream_leg, "learned": learned_leg, "learned_d9": learned_d9_leg, "learned_dem": learned_dem_leg,
               "dem": dem_leg,
-              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg, "toric5": toric5_leg}[args.only]
+              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg, "toric5": toric5_leg, "toric6": toric6_leg}[args.only]
         print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 

@@ -1307,6 +1307,9 @@ int spec_code(const std::string &src, std::vector<char> &code) {
 
 // Live FP64 tensor registers per lane up to which the register-resident
 // FP64 kernel is offered (512 VGPRs and AGPRs at one wave per SIMD).
+// Slab memory of a single-plan specialised kernel's resident workgroups at
+// most (bytes).
+constexpr size_t kSpecSlabTotal = size_t(48) << 30;
 constexpr int kRegBudgetF64 = 440;
 
 bool spec_fits(const tqec_plan *p, const SpecCfg &c) {
@@ -1421,6 +1424,13 @@ int spec_get(tqec_plan *p, int dev, const SpecCfg &cfg, SpecKernel **out) {
         return set_error(TQEC_EDEVICE, std::string("hipModuleGetFunction: ") + hipGetErrorString(e));
     }
     k.blocks = p->n_cu * resident_per_cu(k.lds, cfg.threads, p->lds_limit);
+    if (cfg.slab) {
+        // the kernel loops over tiles in as many workgroups as run: fewer of
+        // them when their slabs would not fit the budget (plans of width 2^23:
+        // 72 MB per workgroup)
+        const size_t per = spec_slab_entries(p->cp, cfg) * (cfg.fp64 ? 8 : 4);
+        if (per) k.blocks = (int)std::max<size_t>((size_t)p->n_cu, std::min<size_t>((size_t)k.blocks, kSpecSlabTotal / per));
+    }
     if (cfg.reg) {
         // register-bound: the runtime's occupancy of the compiled kernel
         int per_cu = 0;

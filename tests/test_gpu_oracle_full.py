@@ -127,3 +127,30 @@ def test_toric5_vs_oracle_fixture(mode):
         # stay exact on every non-tie syndrome (FP64 re-check below a 1e-3
         # margin)
         check_f32(mar, cls, ref, mode, tol=1e-4)
+
+
+@pytest.mark.parametrize("mode", ["plain_f64", "spec_f32"])
+def test_toric5_sliced_vs_oracle_fixture(mode):
+    """Label slicing on the GPU (tensorqec/slicing.py; the reference's slicer
+    hook is src/nonclifford/tensornetwork.jl:174-192): the toric L=5 plan with
+    its widest intermediates cut by two ranks, each syndrome decoded as 2^k
+    rows with the sliced labels fixed and the rows' marginals summed, against
+    the oracle's marginals in tests/golden/toric5.npz (FP64: 1e-12 relative;
+    FP32: 1e-4 of the row maximum)."""
+    import os
+    from tensorqec.slicing import SlicedDecoder, choose_slices
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "toric5.npz"))
+    S, ref, p = g["syn"], g["mar"], float(g["p"])
+    t = CSSTannerGraph.from_code(ToricCode(5, 5))
+    em = iid_error(p, p, p, t.nq)
+    ct = D.compile(D.TNMMAP(mode=mode, optimizer=D.Dissection(), tabulate_bits=0), t, em)
+    sc = ct.plan.stats()["sc_dep"]
+    labels = choose_slices(ct.plan, sc - 2)
+    dec = SlicedDecoder(ct.plan, labels)
+    assert dec.plan.stats()["sc_dep"] <= sc - 2
+    say(f"case toric L=5 sliced {mode}: {len(labels)} labels, width 2^{dec.plan.stats()['sc_dep']}")
+    mar, cls, tie = dec.decode(S, mode)
+    if mode.endswith("_f64"):
+        check_f64(mar, cls, tie, ref, mode)
+    else:
+        check_f32(mar, cls, ref, mode, tol=1e-4)
