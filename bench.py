@@ -727,40 +727,23 @@ def toric5_leg(torch, dev, seed, L=5, p=0.03, n=2048, reps=1):
     return out
 
 
-def toric6_leg(torch, dev, seed, L=6, p=0.03, n=8, target_rank=23):
-    """Toric code L=6 (72 qubits, 16 classes), decoded exactly with label
-    slicing: the nested-dissection network (2^37 multiply-adds per syndrome,
-    widest tensor 2^27) with the labels of its widest intermediates fixed,
-    each syndrome run as 2^k rows and the rows' marginals summed in FP64.
-    FP32 (slab kernels, FP32-MFMA steps) against FP64 on the same syndromes."""
+def sliced_run(torch, plan, labels, S, truth, n, modes=("spec_f32", "spec_f64"), tabulate_bits=0):
+    """Decode the ``n`` device syndromes ``S`` through a sliced plan (each
+    syndrome as 2^k rows, slices summed in FP64) in each mode: rates, FP32
+    against FP64, logical errors against the sampled classes."""
     import numpy as np
-    from tensorqec import decoders as D
-    from tensorqec import lib as L_
-    from tensorqec.codes import CSSTannerGraph, ToricCode
-    from tensorqec.errors import iid_error
-    from tensorqec.gf2 import logical_operator
-    from tensorqec.network import dissected_css_network
-    from tensorqec.slicing import SlicedDecoder, choose_slices
-    t0 = time.time()
-    t = CSSTannerGraph.from_code(ToricCode(L, L))
-    em = iid_error(p, p, p, t.nq)
-    lx, lz = logical_operator(t)
-    _, plan, _ = D.dissection_plan(lambda s: dissected_css_network(t, em, lxlz=(lx, lz), seed=s), D.Dissection())
-    labels = choose_slices(plan, target_rank)
-    dec = SlicedDecoder(plan, labels)
+    from tensorqec.slicing import SlicedDecoder
+    dec = SlicedDecoder(plan, labels, tabulate_bits=tabulate_bits)
+    k = len(labels)
     st0, st1 = plan.stats(), dec.plan.stats()
-    smp = L_.HipSampler.css(t, em, lx, lz)
-    S = torch.empty((n, t.ns), dtype=torch.uint8, device=dev)
-    truth = torch.empty(n, dtype=torch.int32, device=dev)
-    smp.sample(seed + 23, 0, n, S.data_ptr(), truth.data_ptr())
-    out = {"workload": f"toric code L={L}, code-capacity depolarizing px=py=pz={p}, {n} syndromes, "
-                       f"nested-dissection network sliced on {len(labels)} labels",
-           "plan": {"macs_per_syndrome_unsliced": st0["tc_dep"], "width_unsliced": st0["sc_dep"],
-                    "sliced_labels": len(labels), "macs_per_slice": st1["tc_dep"],
-                    "macs_per_syndrome": st1["tc_dep"] * (1 << len(labels)), "width": st1["sc_dep"]},
-           "setup_s": time.time() - t0, "modes": {}}
+    st1["tc_dep"] = dec.hip.stats()["dep_tc"]        # executed per row (tabulated subtrees excluded)
+    out = {"plan": {"sliced_labels": k, "width": st1["sc_dep"], "macs_per_slice": st1["tc_dep"],
+                    "tabulate_bits": tabulate_bits,
+                    "macs_per_syndrome": st1["tc_dep"] * (1 << k),
+                    "same_tree_unsliced": {"macs_per_syndrome": st0["tc_dep"], "width": st0["sc_dep"]}},
+           "modes": {}}
     res = {}
-    for mode in ("spec_f32", "spec_f64"):
+    for mode in modes:
         h0 = time.time()
         dec.decode_device(torch, S, n, mode)          # compiles the kernels
         torch.cuda.synchronize()
@@ -771,7 +754,8 @@ def toric6_leg(torch, dev, seed, L=6, p=0.03, n=8, target_rank=23):
         dt = time.perf_counter() - h0
         res[mode] = (mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy())
         out["modes"][mode] = {"syndromes_per_s": n / dt, "first_call_s": first,
-                              "tflops": 2.0 * st1["tc_dep"] * (n << len(labels)) / dt / 1e12}
+                              "tflops": 2.0 * st1["tc_dep"] * (n << k) / dt / 1e12}
+        print(f"sliced decode {mode}: {n / dt:.3g} syndromes/s", file=sys.stderr, flush=True)
     m32, c32, _ = res["spec_f32"]
     m64, c64, t64 = res["spec_f64"]
     valid = m64.max(axis=1) > 0
@@ -781,6 +765,69 @@ def toric6_leg(torch, dev, seed, L=6, p=0.03, n=8, target_rank=23):
     out["modes"]["spec_f32"]["class_agreement_non_tie"] = float((c32[ok] == c64[ok]).mean())
     out["total_probability_max"] = float(m64.sum(axis=1).max())
     out["logical_errors_fp64"] = int((c64 != truth.cpu().numpy()).sum())
+    return out
+
+
+def toric6_leg(torch, dev, seed, L=6, p=0.03, n=64, target_rank=21):
+    """Toric code L=6 (72 qubits, 16 classes), decoded exactly with label
+    slicing: the nested-dissection network, its tree annealed together with
+    the labels to slice for a widest per-row tensor of 2^21 entries
+    (2^36.3 multiply-adds per syndrome over the slices), each syndrome run
+    as 2^k rows and the rows' marginals summed in FP64.  FP32 (slab kernels,
+    FP32-MFMA steps) against FP64 on the same syndromes."""
+    from tensorqec import lib as L_
+    from tensorqec.codes import CSSTannerGraph, ToricCode
+    from tensorqec.errors import iid_error
+    from tensorqec.gf2 import logical_operator
+    from tensorqec.network import dissected_css_network
+    from tensorqec.slicing import sliced_dissection_plan
+    t0 = time.time()
+    t = CSSTannerGraph.from_code(ToricCode(L, L))
+    em = iid_error(p, p, p, t.nq)
+    lx, lz = logical_operator(t)
+    plan, labels, _, _ = sliced_dissection_plan(lambda s: dissected_css_network(t, em, lxlz=(lx, lz), seed=s),
+                                                target_rank)
+    plan_s = time.time() - t0
+    print(f"toric L={L}: sliced plan in {plan_s:.0f} s", file=sys.stderr, flush=True)
+    smp = L_.HipSampler.css(t, em, lx, lz)
+    S = torch.empty((n, t.ns), dtype=torch.uint8, device=dev)
+    truth = torch.empty(n, dtype=torch.int32, device=dev)
+    smp.sample(seed + 23, 0, n, S.data_ptr(), truth.data_ptr())
+    out = {"workload": f"toric code L={L}, code-capacity depolarizing px=py=pz={p}, {n} syndromes, "
+                       f"nested-dissection network sliced on {len(labels)} labels",
+           "planning_s": plan_s}
+    out.update(sliced_run(torch, plan, labels, S, truth, n))
+    return out
+
+
+def dem_sliced_leg(torch, dev, seed, d=5, rounds=3, p=0.001, n=32, target_rank=21):
+    """Circuit-level memory-Z experiment at a round count whose exact
+    contraction is wider than one workgroup's slab (d=5, 3 rounds: 72
+    detectors, 821 mechanisms, 2^39.3 multiply-adds per syndrome with a
+    widest tensor of 2^28 entries unsliced), decoded with label slicing: the
+    nested-dissection DEM network, tree and sliced labels annealed together
+    for a widest per-row tensor of 2^21 entries.  FP32 against FP64 on the
+    same syndromes, logical errors against the sampled classes.  Subtrees
+    that depend on at most 4 syndrome bits are tabulated (as the DEM
+    compile does), which keeps the generated kernel source small."""
+    from tensorqec import lib as L
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    from tensorqec.network import dissected_dem_network
+    from tensorqec.slicing import sliced_dissection_plan
+    t0 = time.time()
+    dem = detector_error_model_from_stim(rotated_memory_z(d, rounds, p))
+    plan, labels, _, _ = sliced_dissection_plan(lambda s: dissected_dem_network(dem, seed=s), target_rank)
+    plan_s = time.time() - t0
+    print(f"d={d}, {rounds} rounds: sliced plan in {plan_s:.0f} s", file=sys.stderr, flush=True)
+    smp = L.HipSampler.flips(dem.error_rates, dem.detector_matrix(), dem.logical_matrix())
+    S = torch.empty((n, smp.nsyn), dtype=torch.uint8, device=dev)
+    truth = torch.empty(n, dtype=torch.int32, device=dev)
+    smp.sample(seed + 29, 0, n, S.data_ptr(), truth.data_ptr())
+    out = {"workload": f"circuit-level memory-Z, rotated surface code d={d}, {rounds} rounds, p={p}: "
+                       f"{smp.nsyn} detectors, {len(dem.error_rates)} mechanisms, {n} syndromes, "
+                       f"nested-dissection network sliced on {len(labels)} labels",
+           "planning_s": plan_s}
+    out.update(sliced_run(torch, plan, labels, S, truth, n, tabulate_bits=4))
     return out
 
 
@@ -821,7 +868,7 @@ def main():
                          "toric L=5, MLP, device loop, TNMAP)")
     ap.add_argument("--only", default=None,
                     choices=["stream", "stream_full", "learned", "learned_d9", "learned_dem", "dem", "mlp", "pipeline",
-                             "tnmap", "toric5", "toric6"],
+                             "tnmap", "toric5", "toric6", "dem_sliced"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--dem-d", type=int, default=5, help="distance of the circuit-level DEM workload")
     ap.add_argument("--dem-rounds", type=int, default=2, help="syndrome rounds of the circuit-level DEM workload")
@@ -840,10 +887,12 @@ def main():
     if args.only:
         copy_calibration(torch, torch.device("cuda:0"), args.copy_calibration)
         kw = {"dem": {"d": args.dem_d, "rounds": args.dem_rounds},
+              "dem_sliced": {"d": args.dem_d, "rounds": max(args.dem_rounds, 3)},
               "stream_full": {"codes": STREAM_CODES + (("toric5", ("ToricCode", 5)),), "compare": False}}.get(args.only, {})
         fn = {"stream": stream_leg, "stream_full": stream_leg, "learned": learned_leg, "learned_d9": learned_d9_leg, "learned_dem": learned_dem_leg,
               "dem": dem_leg,
-              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg, "toric5": toric5_leg, "toric6": toric6_leg}[args.only]
+              "mlp": mlp_leg, "pipeline": pipeline_leg, "tnmap": tnmap_leg, "toric5": toric5_leg, "toric6": toric6_leg,
+              "dem_sliced": dem_sliced_leg}[args.only]
         print(json.dumps(fn(torch, torch.device("cuda:0"), args.seed, **kw)))
         return
 

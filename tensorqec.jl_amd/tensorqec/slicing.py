@@ -25,44 +25,155 @@ import numpy as np
 from .planner import Plan
 
 
-def choose_slices(plan: Plan, target_rank: int, max_labels: int = 12) -> List[int]:
-    """Labels to slice so that no per-syndrome intermediate of ``plan`` has
-    more than ``target_rank`` labels.  Slicing a label halves every step
-    whose operands carry it and doubles the number of slices, so the total
-    work grows by sum(steps without it) / sum(all steps); each round takes,
-    among the labels of the current widest intermediates, the one carried by
-    the costliest steps (ties: the smallest label).  Output labels and labels
-    already fixed by syndrome bits are never chosen."""
-    dep = plan.dependent_nodes()
-    fixed = set(plan.evidence_label_bits) | set(plan.out_labels_final)
+def _step_masks(plan: Plan):
+    """Per step: bitmask of the labels it multiplies over (both operands,
+    evidence labels of leaves excluded) and of its output labels; per leaf:
+    whether it depends on the syndrome, and its label bitmask."""
     L = plan.n_leaves
+    ev = plan.evidence_label_bits
 
-    def eff(node):
-        labs = plan.node_labels(node)
-        return [l for l in labs if l not in plan.evidence_label_bits] if node < L else labs
+    def mask(labs):
+        m = 0
+        for l in labs:
+            m |= 1 << l
+        return m
 
-    steps = [(set(eff(plan.lhs[i])) | set(eff(plan.rhs[i])), set(plan.out_labels[i]))
-             for i in range(plan.n_steps) if dep[L + i]]
+    leaf_dep = [plan.leaf_slot[t] >= 0 or any(l in ev for l in plan.leaf_labels[t]) for t in range(L)]
+    leaf_mask = [mask(plan.leaf_labels[t]) for t in range(L)]
+    eff = [mask(l for l in plan.leaf_labels[t] if l not in ev) for t in range(L)] + \
+          [mask(out) for out in plan.out_labels]
+    w = [eff[plan.lhs[i]] | eff[plan.rhs[i]] for i in range(plan.n_steps)]
+    out = eff[L:]
+    return leaf_dep, leaf_mask, w, out
+
+
+def _sliced_cost(plan: Plan, masks, sliced: int):
+    """(per-syndrome multiply-adds summed over the 2^k slices, widest
+    per-row intermediate) of ``plan`` with the labels of bitmask ``sliced``
+    fixed: leaves carrying a sliced label become row-dependent, and so does
+    every step above them."""
+    leaf_dep, leaf_mask, w, out = masks
+    L = plan.n_leaves
+    dep = [leaf_dep[t] or bool(leaf_mask[t] & sliced) for t in range(L)]
+    keep = ~sliced
+    tc, sc = 0.0, 0
+    for i in range(plan.n_steps):
+        d = dep[plan.lhs[i]] or dep[plan.rhs[i]]
+        dep.append(d)
+        if d:
+            tc += 2.0 ** bin(w[i] & keep).count("1")
+            sc = max(sc, bin(out[i] & keep).count("1"))
+    return tc * 2.0 ** bin(sliced).count("1"), sc
+
+
+def choose_slices(plan: Plan, target_rank: int, max_labels: int = 12) -> List[int]:
+    """Labels to slice so that no per-row intermediate of ``plan`` has more
+    than ``target_rank`` labels.  Fixing a label removes it from every step
+    that carries it, doubles the number of rows, and makes every constant
+    (syndrome-independent) subtree that carries it row-dependent; each round
+    takes, among the labels of the current widest per-row intermediates, the
+    one that gives the least total work per syndrome after slicing (ties:
+    the narrower result, then the smallest label).  Output labels and labels
+    already fixed by syndrome bits are never chosen."""
+    masks = _step_masks(plan)
+    _, _, _, out = masks
+    fixed = set(plan.evidence_label_bits) | set(plan.out_labels_final)
     chosen: List[int] = []
-    while steps and len(chosen) < max_labels:
-        top = max(len(out) for _, out in steps)
-        if top <= target_rank:
-            break
-        cand = {l for _, out in steps if len(out) == top for l in out if l not in fixed}
+    sliced = 0
+    tc, sc = _sliced_cost(plan, masks, 0)
+    while sc > target_rank and len(chosen) < max_labels:
+        L = plan.n_leaves
+        dep = _sliced_dep(plan, masks, sliced)
+        keep = ~sliced
+        cand = set()
+        for i in range(plan.n_steps):
+            if dep[L + i] and bin(out[i] & keep).count("1") == sc:
+                m = out[i] & keep
+                while m:
+                    b = m & -m
+                    cand.add(b.bit_length() - 1)
+                    m ^= b
+        cand -= fixed
         if not cand:
             break
-        weight = {l: 0.0 for l in cand}
-        for w, _ in steps:
-            c = 2.0 ** len(w)
-            for l in w & cand:
-                weight[l] += c
-        best = min(cand, key=lambda l: (-weight[l], l))
+        best = min(cand, key=lambda l: (*_sliced_cost(plan, masks, sliced | (1 << l)), l))
         chosen.append(best)
         fixed.add(best)
-        for w, out in steps:
-            w.discard(best)
-            out.discard(best)
+        sliced |= 1 << best
+        tc, sc = _sliced_cost(plan, masks, sliced)
     return chosen
+
+
+def _sliced_dep(plan: Plan, masks, sliced: int) -> List[bool]:
+    leaf_dep, leaf_mask, _, _ = masks
+    dep = [leaf_dep[t] or bool(leaf_mask[t] & sliced) for t in range(plan.n_leaves)]
+    for i in range(plan.n_steps):
+        dep.append(dep[plan.lhs[i]] or dep[plan.rhs[i]])
+    return dep
+
+
+def anneal_sliced(graph, tree, target_rank: int, iters: int = 4, sweeps: int = 100, seed: int = 0,
+                  max_labels: int = 12):
+    """Slices and contraction tree chosen together, for ``graph`` (a network
+    whose evidence labels are fixed by syndrome bits; ``tree`` over its
+    non-evidence tensors): alternately pick the labels to slice on the
+    current tree (:func:`choose_slices`) and re-anneal the tree with those
+    labels fixed and a penalty on intermediates wider than ``target_rank``
+    (``planner.anneal_tree``), keeping the cheapest sliced plan whose widest
+    per-row intermediate fits.  Slicing a fixed tree can cost orders of
+    magnitude where many distinct intermediates are at the top width and
+    share few labels; the re-annealed tree moves those labels onto few
+    intermediates.  The reference interleaves slicing with its TreeSA
+    annealing the same way (`TreeSASlicer`,
+    src/nonclifford/tensornetwork.jl:174-192).
+
+    Returns ``(plan, labels, tc, sc)``: the flattened plan (evidence as
+    slices), the labels to slice, the per-syndrome multiply-adds over all
+    slices and the per-row width."""
+    from .planner import ContractionTree, anneal_tree, flatten, recompute_masks
+    gw = graph.without_evidence_leaves()
+    L = tree.n_leaves
+    ev = set(graph.evidence_labels)
+    ixs = gw.ixs[:L]
+    best = None
+    labels: List[int] = []
+    for it in range(max(1, iters)):
+        plan = flatten(gw, tree, True)
+        labels = choose_slices(plan, target_rank, max_labels=max_labels)
+        tc, sc = _sliced_cost(plan, _step_masks(plan), sum(1 << l for l in labels))
+        if best is None or (sc > target_rank, tc) < (best[3] > target_rank, best[2]):
+            best = (plan, list(labels), tc, sc)
+        if it + 1 == iters:
+            break
+        fixed = ev | set(labels)
+        t = ContractionTree(L, list(tree.children), [])
+        t.masks = recompute_masks(t, [[l for l in ix if l not in fixed] for ix in ixs], graph.iy)
+        t = anneal_tree(t, graph.iy, seed=seed * 1009 + it, sweeps=sweeps, sc_target=target_rank)
+        tree = ContractionTree(L, list(t.children), [])
+        tree.masks = recompute_masks(tree, [[l for l in ix if l not in ev] for ix in ixs], graph.iy)
+    return best
+
+
+def sliced_dissection_plan(build, target_rank: int, seeds: int = 2, sweeps: int = 200, iters: int = 4,
+                           slice_sweeps: int = 100, max_labels: int = 12):
+    """:func:`anneal_sliced` over ``seeds`` dissected networks
+    ``build(seed) -> (graph, tree)`` (each first annealed as
+    ``decoders.dissection_plan`` does): the cheapest result whose widest
+    per-row intermediate fits ``target_rank``, as ``(plan, labels, tc,
+    sc)``."""
+    from .planner import anneal_tree, tree_cost
+    best = None
+    for seed in range(max(1, seeds)):
+        g, tree = build(seed)
+        if sweeps > 0:
+            ref = anneal_tree(tree, g.iy, seed=seed, sweeps=sweeps)
+            if tree_cost(ref)[0] < tree_cost(tree)[0]:
+                tree = ref
+        r = anneal_sliced(g, tree, target_rank, iters=iters, sweeps=slice_sweeps, seed=seed,
+                          max_labels=max_labels)
+        if best is None or (r[3] > target_rank, r[2]) < (best[3] > target_rank, best[2]):
+            best = r
+    return best
 
 
 def slice_plan(plan: Plan, labels: Sequence[int]) -> Plan:

@@ -125,3 +125,16 @@ def test_toric5_fixture_inputs():
     assert np.array_equal(S, g["syn"])
     assert g["mar"].shape == (16, 16)
     assert np.all(g["mar"] > 0) and np.all(g["mar"].sum(axis=1) <= 1.0)
+
+
+def test_toric6_fixture_inputs():
+    """tests/golden/toric6.npz (the oracle's exact marginals of toric L=6,
+    contracted in slices): its syndromes are the seeded sampler's, and its
+    marginals are probabilities of those syndromes."""
+    g = np.load(os.path.join(GOLD, "toric6.npz"))
+    t = CSSTannerGraph.from_code(ToricCode(6, 6))
+    p = float(g["p"])
+    S = css_syndrome_rows(sample_depolarizing(iid_error(p, p, p, t.nq), seed=6, count=8), t)
+    assert np.array_equal(S, g["syn"])
+    assert g["mar"].shape == (8, 16)
+    assert np.all(g["mar"] > 0) and np.all(g["mar"].sum(axis=1) <= 1.0)

@@ -100,3 +100,29 @@ def test_circuit_level_d5_two_rounds_vs_oracle(d5_two_rounds, mode):
         check_f64(r.marginals, r.classes, r.ties, ref, mode)
     else:
         check_f32(r.marginals, r.classes, ref, mode, tol=3e-2 if mode == "spec_bf16" else 2e-5)
+
+
+@pytest.mark.parametrize("mode", ["spec_f64", "spec_f32"])
+def test_circuit_level_d5_two_rounds_sliced_vs_oracle(d5_two_rounds, mode):
+    """Label slicing on a detector error model, with the subtrees that depend
+    on at most 4 syndrome bits tabulated: the d=5, 2-round network with its
+    tree and sliced labels annealed together, each syndrome decoded as 2^k
+    rows and the rows summed, against the oracle marginals of the unsliced
+    plan.  The rows are 2^14 wide (the unsliced plan: 2^19), narrow enough
+    that annealing alone cannot reach them and labels must be sliced."""
+    from _parity import check_f32, check_f64
+    from tensorqec.network import dissected_dem_network
+    from tensorqec.slicing import SlicedDecoder, sliced_dissection_plan
+    ct, S, ref = d5_two_rounds
+    dem = detector_error_model_from_stim(rotated_memory_z(5, 2, 0.001))
+    target = 14
+    plan, labels, _, sc = sliced_dissection_plan(lambda s: dissected_dem_network(dem, seed=s), target,
+                                                 seeds=1, sweeps=100, iters=3)
+    assert sc <= target and labels
+    dec = SlicedDecoder(plan, labels, tabulate_bits=4)
+    print(f"case d5r2 dem sliced {mode}: {len(labels)} labels, width 2^{sc}, {len(S)} syndromes", flush=True)
+    mar, cls, tie = dec.decode(S, mode)
+    if mode == "spec_f64":
+        check_f64(mar, cls, tie, ref, mode)
+    else:
+        check_f32(mar, cls, ref, mode, tol=2e-5)

@@ -154,3 +154,30 @@ def test_toric5_sliced_vs_oracle_fixture(mode):
         check_f64(mar, cls, tie, ref, mode)
     else:
         check_f32(mar, cls, ref, mode, tol=1e-4)
+
+
+@pytest.mark.parametrize("mode", ["spec_f64", "spec_f32"])
+def test_toric6_sliced_vs_oracle_fixture(mode):
+    """Toric L=6 (2^36.3 multiply-adds per syndrome, widest tensor 2^27
+    unsliced), decoded with the tree and the sliced labels annealed together
+    for rows of width 2^21 (tensorqec.slicing.sliced_dissection_plan),
+    against the oracle's marginals in tests/golden/toric6.npz (FP64: 1e-12
+    relative; FP32: 1e-4 of the row maximum)."""
+    import os
+    from tensorqec.gf2 import logical_operator
+    from tensorqec.network import dissected_css_network
+    from tensorqec.slicing import SlicedDecoder, sliced_dissection_plan
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "toric6.npz"))
+    S, ref, p = g["syn"], g["mar"], float(g["p"])
+    t = CSSTannerGraph.from_code(ToricCode(6, 6))
+    em = iid_error(p, p, p, t.nq)
+    lx, lz = logical_operator(t)
+    plan, labels, _, sc = sliced_dissection_plan(lambda s: dissected_css_network(t, em, lxlz=(lx, lz), seed=s), 21)
+    assert sc <= 21
+    dec = SlicedDecoder(plan, labels)
+    say(f"case toric L=6 sliced {mode}: {len(labels)} labels, width 2^{sc}")
+    mar, cls, tie = dec.decode(S, mode)
+    if mode.endswith("_f64"):
+        check_f64(mar, cls, tie, ref, mode)
+    else:
+        check_f32(mar, cls, ref, mode, tol=1e-4)

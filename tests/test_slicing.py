@@ -15,7 +15,8 @@ from tensorqec.errors import css_syndrome_rows, iid_error, sample_depolarizing
 from tensorqec.gf2 import logical_operator
 from tensorqec.network import dissected_css_network
 from tensorqec.planner import flatten
-from tensorqec.slicing import choose_slices, combine, expand_rows, slice_plan
+from tensorqec.slicing import (_sliced_cost, _step_masks, choose_slices, combine, expand_rows, slice_plan,
+                                sliced_dissection_plan)
 
 
 def _plan(code, p, seed=0):
@@ -35,6 +36,11 @@ def test_choose_slices_reaches_the_target_width():
     assert sp.stats()["sc_dep"] <= sc - 3
     assert sp.n_syndrome_bits == plan.n_syndrome_bits + len(labels)
     assert not set(labels) & set(plan.out_labels_final)
+    # the slicer's cost model is the sliced plan's own count: every row-dependent
+    # step (constant subtrees that carry a sliced label included), 2^k rows
+    st = sp.stats()
+    assert _sliced_cost(plan, _step_masks(plan), sum(1 << l for l in labels)) == \
+        (st["tc_dep"] * 2.0 ** len(labels), st["sc_dep"])
     with pytest.raises(ValueError):
         slice_plan(plan, [plan.out_labels_final[0]])
 
@@ -72,3 +78,30 @@ def test_sliced_marginals_equal_unsliced(code, p, cut):
     bad[:, -1, :] = 0
     wrong = bad.sum(axis=1)
     assert np.max(np.abs(wrong[nz] - ref[nz]) / ref[nz]) > 1e-6
+
+
+def test_sliced_dissection_plan_on_a_dem_matches_unsliced():
+    """Tree and slices annealed together (sliced_dissection_plan) on the
+    circuit-level d=3, 3-round detector error model: the per-row width meets
+    the target, and the sliced plan's summed marginals equal the unsliced
+    dissection plan's on 48 syndromes (C oracle on both)."""
+    from tensorqec import decoders as D
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    from tensorqec.network import dissected_dem_network
+    dem = detector_error_model_from_stim(rotated_memory_z(3, 3, 0.001))
+    build = lambda s: dissected_dem_network(dem, seed=s)  # noqa: E731
+    _, ref_plan, _ = D.dissection_plan(build, D.Dissection(seeds=1, sweeps=50))
+    target = ref_plan.stats()["sc_dep"] - 3
+    plan, labels, tc, sc = sliced_dissection_plan(build, target, seeds=1, sweeps=50, iters=3, slice_sweeps=50)
+    sp = slice_plan(plan, labels)
+    assert sc <= target and sp.stats()["sc_dep"] == sc and len(labels) >= 1
+    assert tc == sp.stats()["tc_dep"] * 2.0 ** len(labels)
+    rng = np.random.default_rng(3)
+    e = (rng.random((48, len(dem.error_rates))) < 20 * np.asarray(dem.error_rates)[None, :]).astype(np.int64)
+    S = ((e @ dem.tanner().H.T.astype(np.int64)) & 1).astype(np.uint8)
+    ref, _, _ = OraclePlan(ref_plan).decode(S)
+    part, _, _ = OraclePlan(sp).decode(expand_rows(S, len(labels)))
+    mar, _, _ = combine(part, len(S), len(labels))
+    nz = ref > 0
+    assert np.array_equal(mar == 0, ~nz)
+    assert np.max(np.abs(mar[nz] - ref[nz]) / ref[nz]) < 1e-12
