@@ -105,3 +105,24 @@ def test_sliced_dissection_plan_on_a_dem_matches_unsliced():
     nz = ref > 0
     assert np.array_equal(mar == 0, ~nz)
     assert np.max(np.abs(mar[nz] - ref[nz]) / ref[nz]) < 1e-12
+
+
+def test_slicing_a_plan_with_one_hot_evidence_leaves():
+    """The drop-in network (the reference's, evidence as one-hot leaves
+    selected by syndrome bits, tndecoder.jl:148-158) sliced on two labels:
+    the virtual syndrome bits follow the leaves' own, and the summed slices
+    equal the unsliced plan (C oracle on both)."""
+    from tensorqec import decoders as D
+    t = CSSTannerGraph.from_code(SurfaceCode(3, 3))
+    em = iid_error(0.05, 0.05, 0.05, t.nq)
+    plan = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64", tabulate_bits=0), t, em).plan
+    assert any(s >= 0 for s in plan.leaf_slot) and not plan.evidence_label_bits
+    labels = choose_slices(plan, plan.stats()["sc_dep"] - 2)
+    assert labels
+    S = css_syndrome_rows(sample_depolarizing(em, seed=3, count=32), t)
+    ref, _, _ = OraclePlan(plan).decode(S)
+    part, _, _ = OraclePlan(slice_plan(plan, labels)).decode(expand_rows(S, len(labels)))
+    mar, _, _ = combine(part, len(S), len(labels))
+    nz = ref > 0
+    assert np.array_equal(mar == 0, ~nz)
+    assert np.max(np.abs(mar[nz] - ref[nz]) / ref[nz]) < 1e-12
