@@ -181,3 +181,25 @@ def test_toric6_sliced_vs_oracle_fixture(mode):
         check_f64(mar, cls, tie, ref, mode)
     else:
         check_f32(mar, cls, ref, mode, tol=1e-4)
+
+
+@pytest.mark.parametrize("mode", ["plain_f64", "spec_f64", "spec_f32"])
+def test_sliced_one_hot_leaves_vs_oracle(mode):
+    """The drop-in network (the reference's, evidence as one-hot leaves
+    selected by syndrome bits, tndecoder.jl:148-158) sliced on two labels and
+    decoded on the GPU (the sliced labels become syndrome bits after the
+    leaves' own), against the oracle on the unsliced plan, d=5."""
+    from tensorqec.slicing import SlicedDecoder, choose_slices
+    t = CSSTannerGraph.from_code(SurfaceCode(5, 5))
+    em = iid_error(0.05 / 3, 0.05 / 3, 0.05 / 3, t.nq)
+    plan = D.compile(D.TNMMAP(factorize=False, evidence="leaves", mode="plain_f64", tabulate_bits=0), t, em).plan
+    labels = choose_slices(plan, plan.stats()["sc_dep"] - 2)
+    assert labels
+    S = css_syndrome_rows(sample_depolarizing(em, seed=8, count=256), t)
+    ref, _, _ = OraclePlan(plan).decode(S)
+    say(f"case sliced one-hot leaves d=5 {mode}: {len(labels)} labels")
+    mar, cls, tie = SlicedDecoder(plan, labels).decode(S, mode)
+    if mode.endswith("_f64"):
+        check_f64(mar, cls, tie, ref, mode)
+    else:
+        check_f32(mar, cls, ref, mode, tol=2e-5)
