@@ -756,6 +756,8 @@ def sliced_run(torch, plan, labels, S, truth, n, modes=("spec_f32", "spec_f64"),
         res[mode] = (mar.cpu().numpy(), cls.cpu().numpy(), tie.cpu().numpy())
         out["modes"][mode] = {"syndromes_per_s": n / dt, "first_call_s": first,
                               "tflops": 2.0 * st1["tc_dep"] * (n << k) / dt / 1e12}
+        if mode == "spec_f32":
+            out["modes"][mode]["fp64_rechecked_syndromes"] = dec.last_rechecked
         print(f"sliced decode {mode}: {n / dt:.3g} syndromes/s", file=sys.stderr, flush=True)
     m32, c32, _ = res["spec_f32"]
     m64, c64, t64 = res["spec_f64"]

@@ -221,31 +221,67 @@ def combine(marginals: np.ndarray, n: int, k: int, tie_tol: float = 1e-12):
     return mar, cls, tie
 
 
+# FP64 counterpart of each reduced-precision mode, for the re-check of
+# syndromes whose summed marginals are too close to call.
+_FP64_OF = {"spec_f32": "spec_f64", "spec_bf16": "spec_f64", "fused_f32": "fused_f64", "wide_f32": "plain_f64"}
+
+
+def needs_fp64(totals: np.ndarray, recheck_tol: float) -> np.ndarray:
+    """Rows of summed marginals to decide again in FP64: no positive entry,
+    or a top-2 gap within ``recheck_tol`` of the maximum (the executors'
+    per-syndrome rule, applied to the sum of the slices)."""
+    t = np.asarray(totals, dtype=np.float64)
+    best = t.max(axis=1)
+    flag = ~(best > 0)
+    if t.shape[1] > 1:
+        second = np.sort(t, axis=1)[:, -2]
+        flag |= (best - second) <= recheck_tol * best
+    return flag
+
+
 class SlicedDecoder:
     """A compiled plan decoded in 2^k slices (see the module docstring).
 
     ``hip`` is the library plan of the sliced plan (``tensorqec.lib.HipPlan``);
-    ``decode`` takes host syndrome rows of the original width."""
+    ``decode`` takes host syndrome rows of the original width, and
+    ``decode_device`` device rows.  Precision follows the FP32 executors'
+    contract at the level of the summed marginals: in a reduced-precision
+    mode, a syndrome whose sum has no positive entry or a top-2 gap within
+    ``recheck_tol`` of its maximum has all its slices decoded again in the
+    FP64 counterpart mode.  The executors' own per-row margin re-check is
+    off for slice rows (``slice_recheck_tol`` = 0: a slice's margin decides
+    nothing), while rows that underflowed or have no positive entry are
+    still re-checked in FP64 row by row.  ``last_rechecked`` counts the
+    syndromes the last call decided again."""
 
-    def __init__(self, plan: Plan, labels: Sequence[int], tabulate_bits: int = 0):
+    def __init__(self, plan: Plan, labels: Sequence[int], tabulate_bits: int = 0,
+                 slice_recheck_tol: float = 0.0, recheck_tol: float = 1e-3):
         from . import lib as L
         self.labels = list(labels)
         self.k = len(self.labels)
         self.plan = slice_plan(plan, self.labels)
         self.n_syn = plan.n_syndrome_bits
         self.hip = L.HipPlan(self.plan, tabulate_bits=tabulate_bits)
+        self.hip.set_tolerances(1e-12, float(slice_recheck_tol))
+        self.recheck_tol = float(recheck_tol)
+        self.last_rechecked = 0
 
     def decode(self, syndromes, mode: str = "spec_f32", tie_tol: float = 1e-12):
         S = np.atleast_2d(np.asarray(syndromes, dtype=np.uint8))
         if S.shape[1] != self.n_syn:
             raise ValueError(f"syndrome rows have {S.shape[1]} bits, the plan {self.n_syn}")
         mar, _, _ = self.hip.decode(expand_rows(S, self.k), mode, True)
-        return combine(mar, len(S), self.k, tie_tol)
+        tot, cls, tie = combine(mar, len(S), self.k, tie_tol)
+        self.last_rechecked = 0
+        if mode in _FP64_OF:
+            redo = needs_fp64(tot, self.recheck_tol)
+            self.last_rechecked = int(redo.sum())
+            if redo.any():
+                m64, _, _ = self.hip.decode(expand_rows(S[redo], self.k), _FP64_OF[mode], True)
+                tot[redo], cls[redo], tie[redo] = combine(m64, self.last_rechecked, self.k, tie_tol)
+        return tot, cls, tie
 
-    def decode_device(self, torch, d_syn, n: int, mode: str = "spec_f32", stream: Optional[int] = None,
-                      tie_tol: float = 1e-12):
-        """Device rows in, device (marginals, classes, ties) out; the slices
-        are expanded and summed with torch on the same device."""
+    def _decode_rows(self, torch, d_syn, n, mode, stream):
         dev = d_syn.device
         m = 1 << self.k
         ext = torch.empty((n * m, self.n_syn + self.k), dtype=torch.uint8, device=dev)
@@ -257,11 +293,33 @@ class SlicedDecoder:
         mar = torch.empty((n * m, nout), dtype=torch.float64, device=dev)
         cls = torch.empty(n * m, dtype=torch.int32, device=dev)
         tie = torch.empty(n * m, dtype=torch.uint8, device=dev)
+        self.hip.decode_device(ext.data_ptr(), n * m, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), stream)
+        return mar.view(n, m, nout).sum(dim=1)
+
+    def decode_device(self, torch, d_syn, n: int, mode: str = "spec_f32", stream: Optional[int] = None,
+                      tie_tol: float = 1e-12):
+        """Device rows in, device (marginals, classes, ties) out; the slices
+        are expanded and summed with torch on the same device (the FP64
+        re-check reads the flagged syndromes' count back to the host)."""
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        self.hip.decode_device(ext.data_ptr(), n * m, mode, mar.data_ptr(), cls.data_ptr(), tie.data_ptr(), st)
-        tot = mar.view(n, m, nout).sum(dim=1)
-        top2 = torch.topk(tot, 2, dim=1).values if nout > 1 else None
+        tot = self._decode_rows(torch, d_syn, n, mode, st)
+        self.last_rechecked = 0
+        if mode in _FP64_OF and n > 0:
+            best = tot.max(dim=1).values
+            flag = ~(best > 0)
+            if tot.shape[1] > 1:
+                second = torch.topk(tot, 2, dim=1).values[:, 1]
+                flag |= (best - second) <= self.recheck_tol * best
+            idx = torch.nonzero(flag).flatten()
+            self.last_rechecked = int(idx.numel())
+            if self.last_rechecked:
+                sub = d_syn[:n].index_select(0, idx).contiguous()
+                tot[idx] = self._decode_rows(torch, sub, self.last_rechecked, _FP64_OF[mode], st)
+        nout = tot.shape[1]
         c = torch.argmax(tot, dim=1).to(torch.int32)
-        t = ((top2[:, 0] - top2[:, 1]) <= tie_tol * top2[:, 0]).to(torch.uint8) if nout > 1 else \
-            torch.zeros(n, dtype=torch.uint8, device=dev)
+        if nout > 1:
+            top2 = torch.topk(tot, 2, dim=1).values
+            t = ((top2[:, 0] - top2[:, 1]) <= tie_tol * top2[:, 0]).to(torch.uint8)
+        else:
+            t = torch.zeros(n, dtype=torch.uint8, device=tot.device)
         return tot, c, t

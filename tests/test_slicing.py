@@ -126,3 +126,60 @@ def test_slicing_a_plan_with_one_hot_evidence_leaves():
     nz = ref > 0
     assert np.array_equal(mar == 0, ~nz)
     assert np.max(np.abs(mar[nz] - ref[nz]) / ref[nz]) < 1e-12
+
+
+def test_sliced_decoder_decides_close_sums_again_in_fp64():
+    """SlicedDecoder's precision rule, with the library's decode replaced by
+    the oracle on the sliced plan: in "spec_f32" the marginal of each
+    syndrome's leading class is planted 3e-4 too low in every slice;
+    syndromes whose summed top-2 gap is within 1e-3 of the maximum (toric
+    L=3, p=0.03: two of 128 exactly, the closest at 1.0e-4, and whichever
+    the planted error pushes below it) are decoded again in "spec_f64"
+    (exact) and only those, so every class matches the oracle.  With the
+    re-check switched off the planted error flips the closest syndrome's
+    class."""
+    from tensorqec.slicing import SlicedDecoder, needs_fp64
+    t, em, lx, lz, plan = _plan(ToricCode(3, 3), 0.03, seed=1)
+    labels = choose_slices(plan, plan.stats()["sc_dep"] - 2)
+    S = css_syndrome_rows(sample_depolarizing(em, seed=9, count=128), t)
+    ref, _, _ = OraclePlan(plan).decode(S)
+    flagged = needs_fp64(ref, 1e-3)
+    assert flagged.sum() == 2
+    ok = O.tie_flags(ref, 1e-12) == 0
+    m = 1 << len(labels)
+
+    def decoder(recheck_tol):
+        dec = SlicedDecoder(plan, labels, recheck_tol=recheck_tol)
+        orc = OraclePlan(dec.plan)
+        calls = []
+
+        class OracleLibrary:
+            def decode(self, rows, mode, want_marginals):
+                calls.append((mode, len(rows)))
+                mar, cls, tie = orc.decode(rows)
+                if mode == "spec_f32":
+                    g = mar.reshape(-1, m, mar.shape[1])
+                    top = g.sum(axis=1).argmax(axis=1)
+                    g[np.arange(len(g))[:, None], np.arange(m)[None, :], top[:, None]] *= 1.0 - 3e-4
+                    mar = g.reshape(mar.shape)
+                return mar, cls, tie
+
+        dec.hip = OracleLibrary()
+        return dec, calls
+
+    dec, calls = decoder(1e-3)
+    tot, cls, tie = dec.decode(S, "spec_f32")
+    # the flags come from the FP32 sums: every close syndrome, plus any the
+    # planted error pushed below the margin, and nothing else
+    n_re = dec.last_rechecked
+    assert 2 <= n_re <= 4
+    assert calls == [("spec_f32", len(S) * m), ("spec_f64", n_re * m)]
+    redone = np.all(np.abs(tot - ref) <= 1e-12 * ref.max(axis=1, keepdims=True), axis=1)
+    assert redone.sum() == n_re and np.all(redone[flagged])
+    assert np.array_equal(cls[ok], O.findmax(ref)[ok])
+    assert np.array_equal(tie, O.tie_flags(ref, 1e-12))
+    # planted mistake: no re-check of close sums
+    dec0, _ = decoder(0.0)
+    _, cls0, _ = dec0.decode(S, "spec_f32")
+    assert dec0.last_rechecked == 0
+    assert not np.array_equal(cls0[ok], O.findmax(ref)[ok])
