@@ -73,7 +73,8 @@ mutable struct HipPlan
     handle::Ptr{Cvoid}
     n_out::Int
     n_syn::Int
-    function HipPlan(h, n_out, n_syn)
+    # typed, so that it is more specific than the outer HipPlan(code, tensors, n_syndrome::Int)
+    function HipPlan(h::Ptr{Cvoid}, n_out::Int, n_syn::Int)
         p = new(h, n_out, n_syn)
         finalizer(p -> ccall((:tqec_plan_destroy, libtqec), Cvoid, (Ptr{Cvoid},), p.handle), p)
         return p
@@ -89,8 +90,9 @@ Post-order list of the binary contractions of an OMEinsum `NestedEinsum`
 selects its one-hot vector: the evidence vectors of `update_syndrome!`
 (tndecoder.jl:148-158, 240-246).  `sliced` maps a label to the syndrome bit
 that fixes it (TNMAP's evidence variables); such labels stay on the leaves
-(`ixs`, by default the tree's own leaf labels) and never appear in the
-tree.  Labels are shifted to 0-based.
+(`ixs`, by default the tree's own leaf labels) and are dropped from every
+intermediate (an OMEinsum `SlicedEinsum` keeps its sliced labels in the
+inner tree).  Labels are shifted to 0-based.
 """
 function flatten(code, tensors; evidence::Dict{Int,Int} = Dict{Int,Int}(),
                  sliced::Dict{Int,Int} = Dict{Int,Int}(), ixs = OMEinsum.getixsv(code))
@@ -101,7 +103,7 @@ function flatten(code, tensors; evidence::Dict{Int,Int} = Dict{Int,Int}(),
         length(ne.args) == 2 || error("the contraction tree must be binary")
         a = visit(ne.args[1])
         b = visit(ne.args[2])
-        iy = OMEinsum.getiyv(ne.eins)
+        iy = filter(l -> !haskey(sliced, l), OMEinsum.getiyv(ne.eins))
         push!(lhs, a); push!(rhs, b)
         push!(outr, Int32(length(iy)))
         append!(outl, Int32.(iy .- 1))
@@ -140,52 +142,110 @@ function HipPlan(code, tensors, n_syndrome::Int; tabulate_bits::Int = 4, evidenc
     return HipPlan(h[], length(f.iy), n_syndrome)
 end
 
+"""
+    HipPlan(code::SlicedEinsum, tensors, n_syndrome; kwargs...)
+
+A contraction with sliced labels (`optimize_code(...; slicer = TreeSASlicer(...))`,
+the reference's slicer hook, src/nonclifford/tensornetwork.jl:174-192): the
+j-th sliced label becomes syndrome bit `n_syndrome + j - 1` of the inner
+tree's plan, so one library row decodes one slice; `sliced_rows` expands
+syndromes into their 2^k slice rows and the marginals of a syndrome are
+the sum over its rows.
+"""
+function HipPlan(code::OMEinsum.SlicedEinsum, tensors, n_syndrome::Int; tabulate_bits::Int = 4,
+                 evidence::Dict{Int,Int} = Dict{Int,Int}(), semiring::Int32 = SEMIRING_SUM)
+    semiring == SEMIRING_SUM || error("sliced contractions are summed: max-product plans cannot be sliced")
+    k = length(code.slicing)
+    sliced = Dict(l => n_syndrome + j - 1 for (j, l) in enumerate(code.slicing))
+    p = HipPlan(code.eins, tensors, n_syndrome + k; tabulate_bits, evidence, sliced,
+                ixs = OMEinsum.getixsv(code.eins))
+    # a slice's top-2 margin decides nothing: only rows that underflowed are
+    # re-checked by the library; close syndromes are re-decided on the sum
+    check(ccall((:tqec_plan_set_tolerances, libtqec), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), p.handle, 1e-12, 0.0))
+    return SlicedPlan(p, k, n_syndrome, p.n_out)
+end
+
+struct SlicedPlan
+    plan::HipPlan
+    k::Int          # sliced labels; 2^k rows per syndrome
+    n_syn::Int      # syndrome bits of the unsliced problem
+    n_out::Int      # class bits, as HipPlan's
+end
+
+# Row r = (b - 1) * 2^k + s of a sliced batch: syndrome b followed by bit j of
+# the slice index s as syndrome bit n_syn + j.
+function sliced_rows(rows::Matrix{UInt8}, k::Int)
+    n_syn, B = size(rows)
+    m = 1 << k
+    out = Matrix{UInt8}(undef, n_syn + k, B * m)
+    for b in 1:B, s in 0:m-1
+        c = (b - 1) * m + s + 1
+        out[1:n_syn, c] = rows[:, b]
+        for j in 0:k-1
+            out[n_syn + j + 1, c] = UInt8((s >> j) & 1)
+        end
+    end
+    return out
+end
+
 # The last n tensors of a compiled TNMMAP network are its evidence vectors,
 # set per syndrome bit in order by `update_syndrome!`.
 last_as_evidence(L::Int, n::Int) = Dict(L - n + i => i - 1 for i in 1:n)
 
 """
-    TNMMAPHIP(; optimizer=TreeSA(), factorize=true, mode=MODE_SPEC_F32)
+    TNMMAPHIP(; optimizer=TreeSA(), factorize=true, mode=MODE_SPEC_F32, slicer=nothing)
 
 `TNMMAP` (src/decoding/tndecoder.jl:77-80) executed by the HIP library.
 `factorize` applies to detector error models, as in the reference (its CSS
-network is never factorized).
+network is never factorized).  `slicer` (e.g.
+`TreeSASlicer(score = ScoreFunction(sc_target = 21))`) re-optimizes the
+reference's tree with sliced labels for networks too wide for one tensor
+per workgroup (toric L=6, multi-round circuit-level models); the compiled
+decoder then holds a `SlicedPlan`.
 """
 Base.@kwdef struct TNMMAPHIP <: TensorQEC.AbstractGeneralDecoder
     optimizer::OMEinsum.CodeOptimizer = TreeSA()
     factorize::Bool = true
     mode::Int32 = MODE_SPEC_F32
+    slicer = nothing
 end
 
-struct CompiledTNMMAPHIP <: TensorQEC.CompiledDecoder
+struct CompiledTNMMAPHIP{P} <: TensorQEC.CompiledDecoder
     ct::TensorQEC.CompiledTNMMAP
-    plan::HipPlan
+    plan::P
     mode::Int32
 end
 
 tabulation(mode) = mode == MODE_PLAIN_F64 ? 0 : 4
 
-function TensorQEC.compile(decoder::TNMMAPHIP, problem::TensorQEC.IndependentDepolarizingDecodingProblem)
-    ct = TensorQEC.compile(TNMMAP(; optimizer = decoder.optimizer, factorize = false), problem)
+# The reference's tree, or, with a slicer, the same network re-optimized
+# with sliced labels (an OMEinsum SlicedEinsum).
+function hip_plan(decoder::TNMMAPHIP, ct)
     ns = length(ct.syndrome_indices)
-    plan = HipPlan(ct.code, ct.tensors, ns; tabulate_bits = tabulation(decoder.mode),
+    code = ct.code
+    if decoder.slicer !== nothing
+        code = optimize_code(OMEinsum.flatten(code), uniformsize(code, 2), decoder.optimizer;
+                             slicer = decoder.slicer)
+    end
+    return HipPlan(code, ct.tensors, ns; tabulate_bits = tabulation(decoder.mode),
                    evidence = last_as_evidence(length(ct.tensors), ns))
-    return CompiledTNMMAPHIP(ct, plan, decoder.mode)
 end
 
-struct CompiledDEMTNMMAPHIP <: TensorQEC.CompiledDecoder
+function TensorQEC.compile(decoder::TNMMAPHIP, problem::TensorQEC.IndependentDepolarizingDecodingProblem)
+    ct = TensorQEC.compile(TNMMAP(; optimizer = decoder.optimizer, factorize = false), problem)
+    return CompiledTNMMAPHIP(ct, hip_plan(decoder, ct), decoder.mode)
+end
+
+struct CompiledDEMTNMMAPHIP{P} <: TensorQEC.CompiledDecoder
     ct::TensorQEC.CompiledDEMTNMMAP
-    plan::HipPlan
+    plan::P
     mode::Int32
 end
 
 # compile(::TNMMAP, ::DetectorErrorModel), src/decoding/tndecoder.jl:186-219
 function TensorQEC.compile(decoder::TNMMAPHIP, dem::TensorQEC.DetectorErrorModel)
     ct = TensorQEC.compile(TNMMAP(; optimizer = decoder.optimizer, factorize = decoder.factorize), dem)
-    ns = length(ct.syndrome_indices)
-    plan = HipPlan(ct.code, ct.tensors, ns; tabulate_bits = tabulation(decoder.mode),
-                   evidence = last_as_evidence(length(ct.tensors), ns))
-    return CompiledDEMTNMMAPHIP(ct, plan, decoder.mode)
+    return CompiledDEMTNMMAPHIP(ct, hip_plan(decoder, ct), decoder.mode)
 end
 
 syndrome_bytes(s::CSSSyndrome) = UInt8[x.x for x in vcat(s.sx, s.sz)]
@@ -211,6 +271,43 @@ function decode_classes(plan::HipPlan, mode::Int32, syndromes)
                 (Ptr{Cvoid}, Ptr{UInt8}, Int64, Int32, Ptr{Float64}, Ptr{Int32}, Ptr{UInt8}, Ptr{Cvoid}),
                 plan.handle, rows, B, mode, mar, cls, tie, C_NULL))
     return cls
+end
+
+# FP64 counterpart of each reduced-precision mode, for re-deciding close sums.
+fp64_mode(mode) = mode == MODE_FUSED_F32 ? MODE_FUSED_F64 : mode == MODE_WIDE_F32 ? MODE_PLAIN_F64 :
+                  mode in (MODE_SPEC_F32, MODE_SPEC_BF16) ? MODE_SPEC_F64 : mode
+
+# Summed marginals (2^n_out x B) of syndrome columns `rows` over their slices.
+function sliced_marginals(p::SlicedPlan, mode::Int32, rows::Matrix{UInt8})
+    B = size(rows, 2)
+    m = 1 << p.k
+    ext = sliced_rows(rows, p.k)
+    nout = 1 << p.n_out
+    mar = Matrix{Float64}(undef, nout, B * m)
+    cls = Vector{Int32}(undef, B * m)
+    tie = Vector{UInt8}(undef, B * m)
+    check(ccall((:tqec_decode_batch, libtqec), Cint,
+                (Ptr{Cvoid}, Ptr{UInt8}, Int64, Int32, Ptr{Float64}, Ptr{Int32}, Ptr{UInt8}, Ptr{Cvoid}),
+                p.plan.handle, ext, B * m, mode, mar, cls, tie, C_NULL))
+    return dropdims(sum(reshape(mar, nout, m, B); dims = 2); dims = 2)
+end
+
+# Sliced plans: classes from the sums over the slices; a syndrome whose sum
+# has no positive entry or a top-2 gap within 1e-3 of its maximum (the
+# executors' re-check margin) is decided again in FP64.
+function decode_classes(p::SlicedPlan, mode::Int32, syndromes)
+    rows = syndrome_rows(syndromes, p.n_syn)
+    tot = sliced_marginals(p, mode, rows)
+    if fp64_mode(mode) != mode
+        close = map(eachcol(tot)) do c
+            s = sort(c; rev = true)
+            !(s[1] > 0) || (length(s) > 1 && s[1] - s[2] <= 1e-3 * s[1])
+        end
+        if any(close)
+            tot[:, close] = sliced_marginals(p, fp64_mode(mode), rows[:, close])
+        end
+    end
+    return Int32[argmax(c) - 1 for c in eachcol(tot)]    # first maximum, column-major class index
 end
 
 """

@@ -9,7 +9,9 @@ of the NestedEinsum from its root, each intermediate in the label order
 OMEinsum chose (here: descending, unlike this package's planner), the root
 in the network's `iy`, and `n_sliced = 0`, tabulation depth 4 — except for
 TNMAP, whose evidence variables are sliced labels of the UAI model's leaves
-(tndecoder.jl:33-57) in the max-product semiring.  Covered networks: the CSS
+(tndecoder.jl:33-57) in the max-product semiring, and for a contraction
+re-optimized with a slicer (an OMEinsum `SlicedEinsum`), whose sliced
+labels become syndrome bits after the evidence vectors' own.  Covered networks: the CSS
 TNMMAP network without factorization (tndecoder.jl:97-146), the DEM network
 with the reference's default chain factorization (:186-238), and the TNMAP
 UAI model.
@@ -160,6 +162,38 @@ def test_dem_and_tnmap_julia_plans_accepted():
     t, em, g, tree, f = tnmap_case()
     mp = JuliaShapedPlan(f, t.ns, tabulate_bits=0, semiring=1)
     assert mp.stats()["n_out"] == 0
+
+
+def test_julia_sliced_einsum_plan_matches_sliced_package_plan():
+    """TensorQECHIP.jl `HipPlan(::SlicedEinsum, ...)`: the inner tree
+    flattened with the sliced labels dropped from every intermediate and
+    mapped to syndrome bits ns + j after the evidence vectors' own bits, one
+    row per slice (`sliced_rows`: syndrome b, slice s -> row b * 2^k + s,
+    bit j of s as syndrome bit ns + j).  The library accepts it, and it is
+    the package's sliced plan of the same tree (tensorqec.slicing), whose
+    decode the GPU tests check against the oracle."""
+    from tensorqec.slicing import choose_slices, expand_rows, slice_plan
+    t, em, g, tree, _ = css_case(5, 0.05 / 3)
+    ns, nt = t.ns, len(g.ixs)
+    plan = flatten(g, tree, False)
+    labels = choose_slices(plan, plan.stats()["sc_dep"] - 2)
+    k = len(labels)
+    assert k >= 1
+    f = julia_flatten(g, tree, evidence={nt - ns + i: i for i in range(ns)},
+                      sliced={l: ns + j for j, l in enumerate(labels)})
+    jp = JuliaShapedPlan(f, ns + k)
+    pp = L.HipPlan(slice_plan(plan, labels), tabulate_bits=4)
+    a, b = jp.stats(), pp.stats()
+    for key in ("tc", "rw", "sc", "n_steps", "n_dep_steps", "dep_tc", "n_out", "tabulate_bits"):
+        assert a[key] == b[key], (key, a[key], b[key])
+    # the binding's row expansion, column by column as `sliced_rows` writes it
+    S = css_syndrome_rows(sample_depolarizing(em, seed=4, count=3), t)
+    m = 1 << k
+    cols = []
+    for b_ in range(len(S)):
+        for s_ in range(m):
+            cols.append(np.concatenate([S[b_], [(s_ >> j) & 1 for j in range(k)]]).astype(np.uint8))
+    assert np.array_equal(np.array(cols), expand_rows(S, k))
 
 
 @pytest.mark.gpu
