@@ -877,6 +877,8 @@ def main():
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--dem-d", type=int, default=5, help="distance of the circuit-level DEM workload")
     ap.add_argument("--dem-rounds", type=int, default=2, help="syndrome rounds of the circuit-level DEM workload")
+    ap.add_argument("--plan-out", default=None,
+                    help="write the chosen plan (tree, leaf tensors, per-syndrome work) to this JSON file")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed steps (HBM counter calibration)")
     args = ap.parse_args()
@@ -1011,6 +1013,12 @@ def main():
             if ms < best[0]:
                 best = (ms, ct, dict(cfg, mode="spec_f32", **sc))
     chosen, plan_cfg = best[1], best[2]
+    if args.plan_out:
+        from tensorqec.planio import save_plan
+        save_plan(chosen.plan, args.plan_out, tabulate_bits=[plan_cfg["tabulate_bits"]],
+                  meta={"code": f"rotated surface code d={d}", "noise": f"code-capacity depolarizing, p={p} (p/3 each)",
+                        "tree": f"TreeSA(seed={plan_cfg['seed']}, sc_target={plan_cfg['sc_target']})",
+                        "tabulate_bits": plan_cfg["tabulate_bits"], "mode": plan_cfg["mode"]})
     fused_ms = None
     if plan_cfg["mode"] == "spec_f32":
         plan_cfg = dict(plan_cfg, **chosen.hip.spec_config())   # the configuration the plan now holds
