@@ -299,9 +299,13 @@ class SlicedDecoder:
     def decode_device(self, torch, d_syn, n: int, mode: str = "spec_f32", stream: Optional[int] = None,
                       tie_tol: float = 1e-12):
         """Device rows in, device (marginals, classes, ties) out; the slices
-        are expanded and summed with torch on the same device (the FP64
-        re-check reads the flagged syndromes' count back to the host)."""
-        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        are expanded and summed with torch on the same device and stream as
+        the library's launches (the FP64 re-check reads the flagged
+        syndromes' count back to the host)."""
+        if stream is not None and stream != torch.cuda.current_stream().cuda_stream:
+            with torch.cuda.stream(torch.cuda.ExternalStream(stream)):
+                return self.decode_device(torch, d_syn, n, mode, None, tie_tol)
+        st = torch.cuda.current_stream().cuda_stream
         tot = self._decode_rows(torch, d_syn, n, mode, st)
         self.last_rechecked = 0
         if mode in _FP64_OF and n > 0:
