@@ -29,8 +29,8 @@ def _bits(mask: int) -> List[int]:
     return out
 
 
-def _pc(mask: int) -> int:
-    return bin(mask).count("1")
+# population count of a label mask
+_pc = int.bit_count
 
 
 @dataclass

@@ -61,9 +61,9 @@ def _sliced_cost(plan: Plan, masks, sliced: int):
         d = dep[plan.lhs[i]] or dep[plan.rhs[i]]
         dep.append(d)
         if d:
-            tc += 2.0 ** bin(w[i] & keep).count("1")
-            sc = max(sc, bin(out[i] & keep).count("1"))
-    return tc * 2.0 ** bin(sliced).count("1"), sc
+            tc += 2.0 ** (w[i] & keep).bit_count()
+            sc = max(sc, (out[i] & keep).bit_count())
+    return tc * 2.0 ** sliced.bit_count(), sc
 
 
 def choose_slices(plan: Plan, target_rank: int, max_labels: int = 12) -> List[int]:
@@ -87,7 +87,7 @@ def choose_slices(plan: Plan, target_rank: int, max_labels: int = 12) -> List[in
         keep = ~sliced
         cand = set()
         for i in range(plan.n_steps):
-            if dep[L + i] and bin(out[i] & keep).count("1") == sc:
+            if dep[L + i] and (out[i] & keep).bit_count() == sc:
                 m = out[i] & keep
                 while m:
                     b = m & -m
