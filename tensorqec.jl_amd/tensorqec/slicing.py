@@ -4,16 +4,19 @@ batch and summed.
 
 This is the slicing step of the reference's planner
 (`TreeSASlicer`, src/nonclifford/tensornetwork.jl:174-192: fix a few labels,
-contract every assignment, add the results), applied to a fixed contraction
-tree: the labels are chosen greedily among those of the widest
-intermediates, each becomes a *virtual syndrome bit* of the plan (the plan
-compiler already fixes labels by syndrome bits: that is how evidence is
-sliced), and a syndrome row is expanded into 2^k rows carrying its slice
-index in those bits.  The marginals of the slices are summed in FP64; the
-class is the first maximum of the sum (`findmax`, tndecoder.jl:163) and the
-tie flag uses the executors' rule.  Work grows by the slicing overhead
-(shared subtrees are recomputed per slice); the widest tensor shrinks by a
-factor 2 per sliced label.
+contract every assignment, add the results).  The labels are chosen greedily
+among those of the widest intermediates (:func:`choose_slices`, on a fixed
+tree), or together with the tree, which is re-annealed around them
+(:func:`anneal_sliced`, :func:`sliced_dissection_plan`).  Each becomes a
+*virtual syndrome bit* of the plan (the plan compiler already fixes labels by
+syndrome bits: that is how evidence is sliced), and a syndrome row is
+expanded into 2^k rows carrying its slice index in those bits.  The
+marginals of the slices are summed in FP64; the class is the first maximum
+of the sum (`findmax`, tndecoder.jl:163) and the tie flag uses the
+executors' rule; in reduced precision, syndromes whose sum is close to a
+tie are decided again in FP64 (:class:`SlicedDecoder`).  Work grows by the
+slicing overhead (subtrees that carry no sliced label are recomputed per
+slice); the widest tensor shrinks by a factor 2 per sliced label.
 """
 from __future__ import annotations
 
