@@ -299,12 +299,12 @@ function decode_classes(p::SlicedPlan, mode::Int32, syndromes)
     rows = syndrome_rows(syndromes, p.n_syn)
     tot = sliced_marginals(p, mode, rows)
     if fp64_mode(mode) != mode
-        close = map(eachcol(tot)) do c
+        near = map(eachcol(tot)) do c
             s = sort(c; rev = true)
             !(s[1] > 0) || (length(s) > 1 && s[1] - s[2] <= 1e-3 * s[1])
         end
-        if any(close)
-            tot[:, close] = sliced_marginals(p, fp64_mode(mode), rows[:, close])
+        if any(near)
+            tot[:, near] = sliced_marginals(p, fp64_mode(mode), rows[:, near])
         end
     end
     return Int32[argmax(c) - 1 for c in eachcol(tot)]    # first maximum, column-major class index
