@@ -870,7 +870,7 @@ def main():
     ap.add_argument("--no-spec", action="store_true", help="do not consider the plan-specialised kernel")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the secondary workloads (uneven stream, learned front-ends, circuit-level DEM, "
-                         "toric L=5, MLP, device loop, TNMAP)")
+                         "toric L=5 and L=6, MLP, device loop, TNMAP)")
     ap.add_argument("--only", default=None,
                     choices=["stream", "stream_full", "learned", "learned_d9", "learned_dem", "dem", "mlp", "pipeline",
                              "tnmap", "toric5", "toric6", "dem_sliced"],
@@ -1206,6 +1206,13 @@ def main():
         t0 = time.time()
         res["toric5"] = toric5_leg(torch, dev, args.seed)
         print(f"[bench] toric5 leg {time.time() - t0:.1f}s: {res['toric5']['modes']}", file=sys.stderr, flush=True)
+        t0 = time.time()
+        try:
+            res["toric6"] = toric6_leg(torch, dev, args.seed)
+        except Exception as e:      # recorded in the JSON line, which still prints
+            res["toric6"] = {"error": f"{type(e).__name__}: {e}"}
+        print(f"[bench] toric6 leg {time.time() - t0:.1f}s: {res['toric6'].get('modes', res['toric6'])}",
+              file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
