@@ -138,3 +138,20 @@ def test_toric6_fixture_inputs():
     assert np.array_equal(S, g["syn"])
     assert g["mar"].shape == (8, 16)
     assert np.all(g["mar"] > 0) and np.all(g["mar"].sum(axis=1) <= 1.0)
+
+
+
+def test_dem_d5r3_fixture_inputs():
+    """tests/golden/dem_d5r3.npz (the oracle's exact marginals of the
+    circuit-level d=5, 3-round model, contracted in slices): its syndromes
+    are the generator's (mechanism flips at 10x their rates, seed 31), and
+    its marginals are probabilities of those syndromes."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_dem_d5r3", os.path.join(GOLD, "make_dem_d5r3.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    g = np.load(os.path.join(GOLD, "dem_d5r3.npz"))
+    dem = detector_error_model_from_stim(rotated_memory_z(5, 3, float(g["p"])))
+    assert np.array_equal(mk.syndromes(dem), g["syn"])
+    assert g["syn"].shape == (2, 72) and g["mar"].shape == (2, 2)
+    assert np.all(g["mar"] > 0) and np.all(g["mar"].sum(axis=1) <= 1.0)
