@@ -84,3 +84,27 @@ def test_committed_plan_computes_the_reference_marginals(p):
     assert np.max(np.abs(mar - ref) / ref.max(axis=1, keepdims=True)) < 1e-12
     ok = O.tie_flags(ref, 1e-12) == 0
     assert np.array_equal(O.findmax(mar)[ok], O.findmax(ref)[ok])
+
+
+def test_dem_and_sliced_plans_round_trip():
+    """A detector-error-model plan (evidence as label slices, nested
+    dissection) and its sliced form (virtual syndrome bits) survive the file
+    round trip, and the library compiles the copy to the same plan."""
+    import tempfile
+    from tensorqec import decoders as D
+    from tensorqec import lib as L
+    from tensorqec.circuits import detector_error_model_from_stim, rotated_memory_z
+    from tensorqec.planio import save_plan
+    from tensorqec.slicing import choose_slices, slice_plan
+    dem = detector_error_model_from_stim(rotated_memory_z(3, 3, 0.001))
+    plan = D.compile(D.TNMMAP(), dem).plan
+    sliced = slice_plan(plan, choose_slices(plan, plan.stats()["sc_dep"] - 2))
+    for p_ in (plan, sliced):
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "plan.json")
+            save_plan(p_, path, meta={"k": 1}, tabulate_bits=(0, 4))
+            back, meta, work = load_plan(path)
+        assert plan_to_dict(back) == plan_to_dict(p_) and meta == {"k": 1}
+        assert back.evidence_label_bits == p_.evidence_label_bits
+        a, b = L.HipPlan(back, tabulate_bits=4).stats(), L.HipPlan(p_, tabulate_bits=4).stats()
+        assert a == b and work["4"]["dep_tc"] == a["dep_tc"]
